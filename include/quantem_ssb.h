@@ -1,0 +1,157 @@
+/*
+ * quantem_ssb.h -- C ABI of the MI355X single-sideband (SSB) ptychography
+ * engine (libquantem_ssb_hip.so, built for gfx950).
+ *
+ * Every entry point returns an int status (0 = success, negative = error;
+ * see QSSB_ERR_*); ssb_last_error() returns the message of the calling
+ * thread's last failure.  Array arguments are device pointers unless the
+ * name ends in _host.  `stream` is a hipStream_t (NULL = legacy default
+ * stream); work is enqueued asynchronously unless a function says it
+ * synchronises.  A plan is not thread-safe: use one plan per stream.
+ *
+ * The entry points replace the CUDA backend of bobleesj/quantem.gpu behind
+ * its SSBProtocol (src/quantem/gpu/ssb/compute/protocol.py:61-125); each
+ * declaration cites the reference code whose role it takes.  Paths are
+ * relative to that repository; cuda/ = src/quantem/gpu/ssb/compute/cuda/.
+ */
+#ifndef QUANTEM_SSB_H
+#define QUANTEM_SSB_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QSSB_OK 0
+#define QSSB_ERR_ARG (-1)         /* bad shape, size or pointer -> ValueError            */
+#define QSSB_ERR_OOM (-2)         /* device allocation failed   -> MemoryError           */
+#define QSSB_ERR_UNSUPPORTED (-3) /* unsupported scan size etc. -> NotImplementedError   */
+#define QSSB_ERR_LAUNCH (-4)      /* HIP launch / runtime error -> RuntimeError          */
+#define QSSB_ERR_STATE (-5)       /* call order, empty BF disk  -> ValueError            */
+
+typedef struct ssb_plan ssb_plan;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int ssb_version(void);
+
+/* Message of this thread's last error ("" if none). */
+const char* ssb_last_error(void);
+
+/*
+ * Create a plan for square scans of scan_n x scan_n (128, 256 or 512) and a
+ * det_rows x det_cols detector; max_bf bounds the bright-field pixel count.
+ * Detector and optics follow CudaSSBBackend.__init__ (cuda/backend.py:168-365)
+ * and SSBEngine.__init__ (cuda/engine.py:414-495): wavelength in Angstrom,
+ * semiangle and per-pixel angular sampling in radians, scan sampling in
+ * Angstrom (scan frequencies are fftfreq(scan_n, sampling) as float32,
+ * cuda/backend.py:44-82).
+ */
+int ssb_plan_create(int scan_n, int det_rows, int det_cols, int max_bf, double wavelength_A,
+                    double semiangle_rad, double ang_row_rad, double ang_col_rad,
+                    double scan_sampling_row_A, double scan_sampling_col_A, ssb_plan** plan);
+
+/*
+ * Override the detector reciprocal sampling (1/Angstrom per pixel) used for
+ * the BF k coordinates; by default it is ang/lambda.  SSBEngine derives it as
+ * 1/(sampling*gpts) independently of the angular sampling used by the
+ * aperture (cuda/engine.py:791-793, 819-830).
+ */
+int ssb_set_detector_sampling(ssb_plan* plan, double recip_row, double recip_col);
+
+/*
+ * Replace the scan frequencies (float32 [scan_n] each, host) -- the
+ * q_row[:, 0] / q_col[0, :] of SSBEngine (cuda/engine.py:815-816).
+ */
+int ssb_set_scan_frequencies(ssb_plan* plan, const float* qrow_host, const float* qcol_host);
+
+/* Release all device memory of the plan (CudaSSBBackend.free, cuda/backend.py:395-418). */
+int ssb_plan_destroy(ssb_plan* plan);
+
+/*
+ * Exact mean diffraction pattern of `frames` uint16 frames (uint64 sums,
+ * float32 mean = f32(sum)/f32(frames)); cuda/backend.py:420-446 and
+ * qg/detector/compute/cuda/probe.py:126-147.  sums_out / mean_out may be NULL;
+ * the result is always kept in the plan for ssb_select_bf.
+ */
+int ssb_mean_dp_u16(ssb_plan* plan, const uint16_t* data, int64_t frames, uint64_t* sums_out,
+                    float* mean_out, void* stream);
+
+/*
+ * Bright-field selection on the device from the plan's mean DP
+ * (cuda/backend.py:448-518).  bf_radius <= 0 selects the automatic
+ * mean+std radius.  No host synchronisation; see ssb_get_bf.
+ */
+int ssb_select_bf(ssb_plan* plan, float threshold, float bf_radius, void* stream);
+
+/*
+ * Set the bright-field pixels from the host (raster order), with the disk
+ * centre and dc = mean_b G_b(0,0) (SSBEngine(..., bf_inds_row, bf_inds_col,
+ * bf_center, dc_value), cuda/engine.py:414-495).
+ */
+int ssb_set_bf(ssb_plan* plan, const int32_t* rows_host, const int32_t* cols_host, int nbf,
+               float center_row, float center_col, float dc_re, float dc_im, void* stream);
+
+/*
+ * Copy the selection to the host (synchronises the stream).  info6_host
+ * receives {center_row, center_col, radius^2, dc_re, dc_im, status}; status
+ * 0 = ok, 1 = no candidates, 2 = empty disk, 3 = more than max_bf pixels.
+ * rows_host / cols_host may be NULL; otherwise they hold `capacity` ints.
+ */
+int ssb_get_bf(ssb_plan* plan, int32_t* rows_host, int32_t* cols_host, int capacity, int* nbf_host,
+               float* info6_host, void* stream);
+
+/* Per-BF geometry for a scan rotation (SSBEngine.cache_rotation, cuda/engine.py:784-848). */
+int ssb_cache_rotation(ssb_plan* plan, double rotation_rad, void* stream);
+
+/* Device copies of the first `count` cached per-BF kx, ky (1/Angstrom) and aperture A(k). */
+int ssb_get_geometry(ssb_plan* plan, float* kx_out, float* ky_out, float* aperture_out, int count,
+                     void* stream);
+
+/*
+ * Resident Fourier stack from raw uint16 data (CudaSSBBackend._extract_gqk,
+ * cuda/backend.py:520-589): BF gather, u16 -> f32, forward 2D FFT over the
+ * scan axes, Hermitian half-plane kept in the plan.
+ */
+int ssb_prepare_g_u16(ssb_plan* plan, const uint16_t* data, void* stream);
+
+/*
+ * Export the resident stack in the reference layout G[b][q_row][q_col <= N/2]
+ * (complex64); nbf must equal the resident BF count.
+ */
+int ssb_export_g(ssb_plan* plan, void* g_out, int nbf, void* stream);
+
+/*
+ * Import a stack G[b][q_row][q_col] (complex64, gcols = N/2+1 Hermitian or N
+ * full columns) for nbf pixels; the BF selection must already be set.
+ * Mirrors constructing SSBEngine from G_qk (cuda/engine.py:414-495).
+ */
+int ssb_import_g(ssb_plan* plan, const void* g_in, int nbf, int gcols, void* stream);
+
+/*
+ * Complex object and phase from the resident stack for C10, C12 (Angstrom)
+ * and phi12 (rad) at the cached rotation: the Fourier-sum object path
+ * (SSBEngine._reconstruct_object_fourier_sum, cuda/engine.py:1047-1107,
+ * trotter kernel cuda/kernels/fft256.py:967-1030) and
+ * phase = angle(object) (qg/ssb/results.py:117-122).  obj_out
+ * ([N][N] complex64) and phase_out ([N][N] float32) may be NULL.
+ */
+int ssb_object(ssb_plan* plan, double c10, double c12, double phi12, void* obj_out, float* phase_out,
+               void* stream);
+
+/*
+ * End-to-end plain SSB from resident raw data without host round trips:
+ * mean DP, bright-field selection, geometry, gather + scan FFT fused with the
+ * trotter reduction, inverse FFT and angle.  Equivalent to
+ * CudaSSBBackend(data, ...) followed by result() (cuda/backend.py:168-851)
+ * for C10/C12/phi12 aberrations.
+ */
+int ssb_recon_u16(ssb_plan* plan, const uint16_t* data, double c10, double c12, double phi12,
+                  double rotation_rad, float threshold, float bf_radius, void* obj_out, float* phase_out,
+                  void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QUANTEM_SSB_H */

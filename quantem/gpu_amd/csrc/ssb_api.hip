@@ -1,0 +1,527 @@
+// C ABI of the MI355X SSB engine: plan management and pipeline launches.
+// Declarations and reference citations: include/quantem_ssb.h.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../../include/quantem_ssb.h"
+#include "ssb_internal.h"
+
+using namespace qssb;
+
+struct ssb_plan {
+  int n = 0;
+  int det_rows = 0, det_cols = 0, D = 0;
+  int max_bf = 0;
+  int max_groups = 0;
+  int max_ranges = 256;
+  int nbf_hint = 0;
+  int gcols = 0;            // columns of the resident stack (0 = none)
+  int g_nbf = 0;
+  double wavelength = 0, semiangle = 0, ang_y = 0, ang_x = 0;
+  double sampling_row = 0, sampling_col = 0;
+  double recip_y = 0, recip_x = 0;
+  bool geometry_valid = false;
+  double rotation = 0;
+  Optics optics{};
+  // device buffers
+  uint32_t* dp_partial = nullptr;
+  unsigned long long* dp_sums = nullptr;
+  float* dp_mean = nullptr;
+  int* bf_rows = nullptr;
+  int* bf_cols = nullptr;
+  BfInfo* info = nullptr;
+  float* kx = nullptr;
+  float* kxl = nullptr;
+  float* ky = nullptr;
+  float* kyl = nullptr;
+  float* ak = nullptr;
+  float* qx = nullptr;
+  float* qxl = nullptr;
+  float* qy = nullptr;
+  float* qyl = nullptr;
+  float2* tw = nullptr;
+  float2* H = nullptr;
+  float2* S_part = nullptr;
+  float2* S = nullptr;
+  float2* Gt = nullptr;
+  size_t Gt_bytes = 0;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* where) {
+  std::string msg = std::string(where) + ": " + hipGetErrorString(e);
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return fail(QSSB_ERR_OOM, msg);
+  return fail(QSSB_ERR_LAUNCH, msg);
+}
+
+#define HIP_TRY(expr, where)              \
+  do {                                    \
+    hipError_t _e = (expr);               \
+    if (_e != hipSuccess) return hip_fail(_e, where); \
+  } while (0)
+
+template <typename T>
+hipError_t dmalloc(T** p, size_t count) {
+  return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T) > 0 ? count * sizeof(T) : 1);
+}
+
+void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+df to_df(double v) {
+  const float hi = (float)v;
+  return df{hi, (float)(v - (double)hi)};
+}
+
+GeomPtrs geom_ptrs(const ssb_plan* p) {
+  return GeomPtrs{p->kx, p->kxl, p->ky, p->kyl, p->ak, p->qx, p->qxl, p->qy, p->qyl};
+}
+
+QuadChi quad_chi(const ssb_plan* p, double c10, double c12, double phi12) {
+  const double f = M_PI * p->wavelength;
+  const double c = std::cos(2.0 * phi12), s = std::sin(2.0 * phi12);
+  QuadChi m;
+  m.mxx = (float)(f * (c10 + c12 * c));
+  m.mxy = (float)(f * (c12 * s));
+  m.myy = (float)(f * (c10 - c12 * c));
+  return m;
+}
+
+int check_plan(ssb_plan* p) {
+  if (!p) return fail(QSSB_ERR_ARG, "plan is NULL");
+  g_last_error.clear();
+  return QSSB_OK;
+}
+
+int ensure_geometry(ssb_plan* p, hipStream_t s) {
+  const df ca = to_df(std::cos(-p->rotation)), sa = to_df(std::sin(-p->rotation));
+  hipError_t e = launch_bf_geometry(p->bf_rows, p->bf_cols, p->info, p->max_bf, to_df(p->recip_y),
+                                    to_df(p->recip_x), ca, sa, p->optics, p->kx, p->kxl, p->ky, p->kyl, p->ak, s);
+  if (e != hipSuccess) return hip_fail(e, "bf_geometry");
+  p->geometry_valid = true;
+  return QSSB_OK;
+}
+
+// Gt layout [b][c][q] (q contiguous) <-> reference layout [b][q][c].
+__global__ void k_export_g(const float2* __restrict__ Gt, int gcols, int n, int nbf, float2* __restrict__ out,
+                           int out_cols) {
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long total = (long long)nbf * n * out_cols;
+  if (idx >= total) return;
+  const int c = (int)(idx % out_cols);
+  const long long t = idx / out_cols;
+  const int q = (int)(t % n);
+  const long long b = t / n;
+  out[idx] = Gt[(b * gcols + c) * n + q];
+}
+
+__global__ void k_import_g(const float2* __restrict__ in, int gcols, int n, int nbf, float2* __restrict__ Gt) {
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long total = (long long)nbf * n * gcols;
+  if (idx >= total) return;
+  const int c = (int)(idx % gcols);
+  const long long t = idx / gcols;
+  const int q = (int)(t % n);
+  const long long b = t / n;
+  Gt[(b * gcols + c) * n + q] = in[idx];
+}
+
+}  // namespace
+
+extern "C" {
+
+int ssb_version(void) { return 100; }
+
+const char* ssb_last_error(void) { return g_last_error.c_str(); }
+
+int ssb_plan_create(int scan_n, int det_rows, int det_cols, int max_bf, double wavelength_A,
+                    double semiangle_rad, double ang_row_rad, double ang_col_rad, double scan_sampling_row_A,
+                    double scan_sampling_col_A, ssb_plan** out) {
+  g_last_error.clear();
+  if (!out) return fail(QSSB_ERR_ARG, "plan output pointer is NULL");
+  *out = nullptr;
+  if (scan_n != 128 && scan_n != 256 && scan_n != 512)
+    return fail(QSSB_ERR_UNSUPPORTED, "scan size " + std::to_string(scan_n) +
+                                          " is not supported; use 128, 256 or 512");
+  if (det_rows < 1 || det_cols < 1 || max_bf < 1)
+    return fail(QSSB_ERR_ARG, "detector shape and max_bf must be positive");
+  if (!(wavelength_A > 0) || !(semiangle_rad > 0) || !(ang_row_rad > 0) || !(ang_col_rad > 0) ||
+      !(scan_sampling_row_A > 0) || !(scan_sampling_col_A > 0))
+    return fail(QSSB_ERR_ARG, "optics and sampling must be positive");
+  ssb_plan* p = new ssb_plan();
+  p->n = scan_n;
+  p->det_rows = det_rows;
+  p->det_cols = det_cols;
+  p->D = det_rows * det_cols;
+  p->max_bf = max_bf;
+  p->nbf_hint = max_bf;
+  p->max_groups = group_count(scan_n, max_bf);
+  p->wavelength = wavelength_A;
+  p->semiangle = semiangle_rad;
+  p->ang_y = ang_row_rad;
+  p->ang_x = ang_col_rad;
+  p->sampling_row = scan_sampling_row_A;
+  p->sampling_col = scan_sampling_col_A;
+  // reciprocal sampling: ang/lambda, re-derived through the real-space
+  // sampling 1/(recip*det) exactly as the engine does (cuda/backend.py:316-327)
+  const double ry = ang_row_rad / wavelength_A, rx = ang_col_rad / wavelength_A;
+  const double sy = 1.0 / (ry * det_rows), sx = 1.0 / (rx * det_cols);
+  p->recip_y = 1.0 / (sy * det_rows);
+  p->recip_x = 1.0 / (sx * det_cols);
+  Optics& o = p->optics;
+  o.wavelength = (float)wavelength_A;
+  o.semiangle = (float)semiangle_rad;
+  o.ang_y = (float)ang_row_rad;
+  o.ang_x = (float)ang_col_rad;
+  {
+    const float max_ang = std::fmax(o.ang_y, o.ang_x);
+    const float inner = (o.semiangle - 0.5f * max_ang) / o.wavelength;
+    o.inner2 = (inner > 0.0f) ? inner * inner : 0.0f;
+    const float outer = (o.semiangle + 0.5f * max_ang) / o.wavelength;
+    o.outer2 = outer * outer * 1.0001f;
+    o.lam = to_df(wavelength_A);
+    o.semi = to_df(semiangle_rad);
+    o.angy = to_df(ang_row_rad);
+    o.angx = to_df(ang_col_rad);
+  }
+  const int n = scan_n;
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = dmalloc(&p->dp_partial, (size_t)p->max_ranges * p->D);
+  if (e == hipSuccess) e = dmalloc(&p->dp_sums, (size_t)p->D);
+  if (e == hipSuccess) e = dmalloc(&p->dp_mean, (size_t)p->D);
+  if (e == hipSuccess) e = dmalloc(&p->bf_rows, (size_t)max_bf);
+  if (e == hipSuccess) e = dmalloc(&p->bf_cols, (size_t)max_bf);
+  if (e == hipSuccess) e = dmalloc(&p->info, 1);
+  if (e == hipSuccess) e = dmalloc(&p->kx, (size_t)max_bf);
+  if (e == hipSuccess) e = dmalloc(&p->kxl, (size_t)max_bf);
+  if (e == hipSuccess) e = dmalloc(&p->ky, (size_t)max_bf);
+  if (e == hipSuccess) e = dmalloc(&p->kyl, (size_t)max_bf);
+  if (e == hipSuccess) e = dmalloc(&p->ak, (size_t)max_bf);
+  if (e == hipSuccess) e = dmalloc(&p->qx, (size_t)n);
+  if (e == hipSuccess) e = dmalloc(&p->qxl, (size_t)n);
+  if (e == hipSuccess) e = dmalloc(&p->qy, (size_t)n);
+  if (e == hipSuccess) e = dmalloc(&p->qyl, (size_t)n);
+  if (e == hipSuccess) e = dmalloc(&p->tw, (size_t)n);
+  if (e == hipSuccess) e = dmalloc(&p->H, (size_t)max_bf * n * (n / 2));
+  if (e == hipSuccess) e = dmalloc(&p->S_part, (size_t)p->max_groups * n * n);
+  if (e == hipSuccess) e = dmalloc(&p->S, (size_t)n * n);
+  if (e != hipSuccess) {
+    ssb_plan_destroy(p);
+    return hip_fail(e, "ssb_plan_create allocation");
+  }
+  std::vector<float> qx(n), qy(n), qxl(n), qyl(n);
+  std::vector<float2> tw(n);
+  for (int i = 0; i < n; ++i) {
+    const int k = (i < (n + 1) / 2) ? i : i - n;   // numpy fftfreq ordering
+    const df fx = to_df((double)k / (n * scan_sampling_row_A));
+    const df fy = to_df((double)k / (n * scan_sampling_col_A));
+    qx[i] = fx.hi;
+    qxl[i] = fx.lo;
+    qy[i] = fy.hi;
+    qyl[i] = fy.lo;
+    const double a = -2.0 * M_PI * (double)i / (double)n;
+    tw[i] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  BfInfo zero{};
+  if (e == hipSuccess) e = hipMemcpy(p->qx, qx.data(), n * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->qy, qy.data(), n * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->qxl, qxl.data(), n * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->qyl, qyl.data(), n * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->tw, tw.data(), n * sizeof(float2), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->info, &zero, sizeof(BfInfo), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    ssb_plan_destroy(p);
+    return hip_fail(e, "ssb_plan_create upload");
+  }
+  *out = p;
+  return QSSB_OK;
+}
+
+int ssb_set_detector_sampling(ssb_plan* p, double recip_row, double recip_col) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!(recip_row > 0) || !(recip_col > 0)) return fail(QSSB_ERR_ARG, "reciprocal sampling must be positive");
+  p->recip_y = recip_row;
+  p->recip_x = recip_col;
+  p->geometry_valid = false;
+  return QSSB_OK;
+}
+
+int ssb_set_scan_frequencies(ssb_plan* p, const float* qrow_host, const float* qcol_host) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!qrow_host || !qcol_host) return fail(QSSB_ERR_ARG, "scan frequencies are NULL");
+  HIP_TRY(hipMemcpy(p->qx, qrow_host, (size_t)p->n * 4, hipMemcpyHostToDevice), "set_scan_frequencies");
+  HIP_TRY(hipMemcpy(p->qy, qcol_host, (size_t)p->n * 4, hipMemcpyHostToDevice), "set_scan_frequencies");
+  HIP_TRY(hipMemset(p->qxl, 0, (size_t)p->n * 4), "set_scan_frequencies");
+  HIP_TRY(hipMemset(p->qyl, 0, (size_t)p->n * 4), "set_scan_frequencies");
+  return QSSB_OK;
+}
+
+int ssb_plan_destroy(ssb_plan* p) {
+  if (!p) return QSSB_OK;
+  dfree(p->dp_partial);
+  dfree(p->dp_sums);
+  dfree(p->dp_mean);
+  dfree(p->bf_rows);
+  dfree(p->bf_cols);
+  dfree(p->info);
+  dfree(p->kx);
+  dfree(p->kxl);
+  dfree(p->ky);
+  dfree(p->kyl);
+  dfree(p->ak);
+  dfree(p->qx);
+  dfree(p->qxl);
+  dfree(p->qy);
+  dfree(p->qyl);
+  dfree(p->tw);
+  dfree(p->H);
+  dfree(p->S_part);
+  dfree(p->S);
+  dfree(p->Gt);
+  delete p;
+  return QSSB_OK;
+}
+
+int ssb_mean_dp_u16(ssb_plan* p, const uint16_t* data, int64_t frames, uint64_t* sums_out, float* mean_out,
+                    void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!data || frames < 1) return fail(QSSB_ERR_ARG, "data is NULL or has no frames");
+  hipStream_t s = as_stream(stream);
+  HIP_TRY(launch_mean_dp(data, frames, p->D, p->dp_partial, p->max_ranges, p->dp_sums, p->dp_mean, s),
+          "mean_dp");
+  if (sums_out)
+    HIP_TRY(hipMemcpyAsync(sums_out, p->dp_sums, (size_t)p->D * 8, hipMemcpyDeviceToDevice, s), "mean_dp copy");
+  if (mean_out)
+    HIP_TRY(hipMemcpyAsync(mean_out, p->dp_mean, (size_t)p->D * 4, hipMemcpyDeviceToDevice, s), "mean_dp copy");
+  return QSSB_OK;
+}
+
+int ssb_select_bf(ssb_plan* p, float threshold, float bf_radius, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  hipStream_t s = as_stream(stream);
+  HIP_TRY(launch_select_bf(p->dp_mean, p->dp_sums, p->det_rows, p->det_cols, threshold, bf_radius, p->max_bf,
+                           p->bf_rows, p->bf_cols, p->info, s),
+          "select_bf");
+  p->geometry_valid = false;
+  p->gcols = 0;
+  return QSSB_OK;
+}
+
+int ssb_set_bf(ssb_plan* p, const int32_t* rows_host, const int32_t* cols_host, int nbf, float center_row,
+               float center_col, float dc_re, float dc_im, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!rows_host || !cols_host || nbf < 1) return fail(QSSB_ERR_ARG, "BF rows/cols must be non-empty");
+  if (nbf > p->max_bf)
+    return fail(QSSB_ERR_ARG, "nbf " + std::to_string(nbf) + " exceeds plan capacity " + std::to_string(p->max_bf));
+  for (int i = 0; i < nbf; ++i) {
+    if (rows_host[i] < 0 || rows_host[i] >= p->det_rows || cols_host[i] < 0 || cols_host[i] >= p->det_cols)
+      return fail(QSSB_ERR_ARG, "BF coordinate outside the detector");
+  }
+  hipStream_t s = as_stream(stream);
+  BfInfo info{};
+  info.nbf = nbf;
+  info.status = 0;
+  info.center_r = center_row;
+  info.center_c = center_col;
+  info.radius2 = 0.0f;
+  info.dc_re = dc_re;
+  info.dc_im = dc_im;
+  HIP_TRY(hipMemcpyAsync(p->bf_rows, rows_host, (size_t)nbf * 4, hipMemcpyHostToDevice, s), "set_bf");
+  HIP_TRY(hipMemcpyAsync(p->bf_cols, cols_host, (size_t)nbf * 4, hipMemcpyHostToDevice, s), "set_bf");
+  HIP_TRY(hipMemcpyAsync(p->info, &info, sizeof(BfInfo), hipMemcpyHostToDevice, s), "set_bf");
+  HIP_TRY(hipStreamSynchronize(s), "set_bf sync");
+  p->nbf_hint = nbf;
+  p->geometry_valid = false;
+  p->gcols = 0;
+  return QSSB_OK;
+}
+
+int ssb_get_bf(ssb_plan* p, int32_t* rows_host, int32_t* cols_host, int capacity, int* nbf_host,
+               float* info6_host, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  hipStream_t s = as_stream(stream);
+  BfInfo info{};
+  HIP_TRY(hipMemcpyAsync(&info, p->info, sizeof(BfInfo), hipMemcpyDeviceToHost, s), "get_bf");
+  HIP_TRY(hipStreamSynchronize(s), "get_bf sync");
+  if (nbf_host) *nbf_host = info.nbf;
+  if (info6_host) {
+    info6_host[0] = info.center_r;
+    info6_host[1] = info.center_c;
+    info6_host[2] = info.radius2;
+    info6_host[3] = info.dc_re;
+    info6_host[4] = info.dc_im;
+    info6_host[5] = (float)info.status;
+  }
+  if (info.nbf > 0) p->nbf_hint = info.nbf;
+  const int n = info.nbf < capacity ? info.nbf : capacity;
+  if (rows_host && n > 0)
+    HIP_TRY(hipMemcpy(rows_host, p->bf_rows, (size_t)n * 4, hipMemcpyDeviceToHost), "get_bf rows");
+  if (cols_host && n > 0)
+    HIP_TRY(hipMemcpy(cols_host, p->bf_cols, (size_t)n * 4, hipMemcpyDeviceToHost), "get_bf cols");
+  return QSSB_OK;
+}
+
+int ssb_cache_rotation(ssb_plan* p, double rotation_rad, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  p->rotation = rotation_rad;
+  return ensure_geometry(p, as_stream(stream));
+}
+
+int ssb_get_geometry(ssb_plan* p, float* kx_out, float* ky_out, float* ak_out, int count, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (count < 0 || count > p->max_bf) return fail(QSSB_ERR_ARG, "count exceeds the plan capacity");
+  hipStream_t s = as_stream(stream);
+  if (!p->geometry_valid) {
+    rc = ensure_geometry(p, s);
+    if (rc) return rc;
+  }
+  const size_t bytes = (size_t)count * 4;
+  if (kx_out) HIP_TRY(hipMemcpyAsync(kx_out, p->kx, bytes, hipMemcpyDeviceToDevice, s), "get_geometry");
+  if (ky_out) HIP_TRY(hipMemcpyAsync(ky_out, p->ky, bytes, hipMemcpyDeviceToDevice, s), "get_geometry");
+  if (ak_out) HIP_TRY(hipMemcpyAsync(ak_out, p->ak, bytes, hipMemcpyDeviceToDevice, s), "get_geometry");
+  return QSSB_OK;
+}
+
+static int alloc_gt(ssb_plan* p, int gcols, int nbf) {
+  const size_t need = (size_t)nbf * gcols * p->n * sizeof(float2);
+  if (p->Gt_bytes < need) {
+    dfree(p->Gt);
+    p->Gt = nullptr;
+    p->Gt_bytes = 0;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->Gt), need);
+    if (e != hipSuccess) return hip_fail(e, "resident G allocation");
+    p->Gt_bytes = need;
+  }
+  return QSSB_OK;
+}
+
+int ssb_prepare_g_u16(ssb_plan* p, const uint16_t* data, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!data) return fail(QSSB_ERR_ARG, "data is NULL");
+  hipStream_t s = as_stream(stream);
+  {
+    // the resident stack is sized by the current selection: read it back
+    BfInfo info{};
+    HIP_TRY(hipMemcpyAsync(&info, p->info, sizeof(BfInfo), hipMemcpyDeviceToHost, s), "prepare_g info");
+    HIP_TRY(hipStreamSynchronize(s), "prepare_g sync");
+    if (info.status != 0 || info.nbf < 1) return fail(QSSB_ERR_STATE, "no valid bright-field selection");
+    p->nbf_hint = info.nbf;
+  }
+  rc = alloc_gt(p, p->n / 2 + 1, p->max_bf);
+  if (rc) return rc;
+  HIP_TRY(launch_gather_rowfft(p->n, data, p->D, p->det_cols, p->bf_rows, p->bf_cols, p->info, p->nbf_hint, p->tw,
+                               p->H, s),
+          "gather_rowfft");
+  HIP_TRY(launch_colfft_trotter(p->n, p->H, p->info, p->nbf_hint, geom_ptrs(p), QuadChi{}, p->optics, p->tw,
+                                p->S_part, p->max_groups, p->Gt, false, s),
+          "colfft");
+  p->gcols = p->n / 2 + 1;
+  p->g_nbf = p->nbf_hint;
+  return QSSB_OK;
+}
+
+int ssb_export_g(ssb_plan* p, void* g_out, int nbf, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!p->gcols) return fail(QSSB_ERR_STATE, "no resident Fourier stack; call ssb_prepare_g_u16 first");
+  if (!g_out) return fail(QSSB_ERR_ARG, "output is NULL");
+  if (nbf != p->g_nbf)
+    return fail(QSSB_ERR_ARG, "output holds " + std::to_string(nbf) + " BF planes; the resident stack has " +
+                                  std::to_string(p->g_nbf));
+  const int oc = p->n / 2 + 1;
+  const long long total = (long long)p->g_nbf * p->n * oc;
+  hipLaunchKernelGGL(k_export_g, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, as_stream(stream), p->Gt,
+                     p->gcols, p->n, p->g_nbf, reinterpret_cast<float2*>(g_out), oc);
+  HIP_TRY(hipGetLastError(), "export_g");
+  return QSSB_OK;
+}
+
+int ssb_import_g(ssb_plan* p, const void* g_in, int nbf, int gcols, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!g_in) return fail(QSSB_ERR_ARG, "input is NULL");
+  if (gcols != p->n && gcols != p->n / 2 + 1)
+    return fail(QSSB_ERR_ARG, "G must have N or N/2+1 columns");
+  if (nbf != p->nbf_hint) return fail(QSSB_ERR_ARG, "G first dimension must match the BF selection");
+  rc = alloc_gt(p, gcols, nbf);
+  if (rc) return rc;
+  const long long total = (long long)nbf * p->n * gcols;
+  hipLaunchKernelGGL(k_import_g, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, as_stream(stream),
+                     reinterpret_cast<const float2*>(g_in), gcols, p->n, nbf, p->Gt);
+  HIP_TRY(hipGetLastError(), "import_g");
+  p->gcols = gcols;
+  p->g_nbf = nbf;
+  return QSSB_OK;
+}
+
+int ssb_object(ssb_plan* p, double c10, double c12, double phi12, void* obj_out, float* phase_out, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!p->gcols) return fail(QSSB_ERR_STATE, "no resident Fourier stack");
+  hipStream_t s = as_stream(stream);
+  if (!p->geometry_valid) {
+    rc = ensure_geometry(p, s);
+    if (rc) return rc;
+  }
+  const QuadChi m = quad_chi(p, c10, c12, phi12);
+  HIP_TRY(launch_trotter_from_g(p->n, p->gcols, p->Gt, p->info, p->nbf_hint, geom_ptrs(p), m, p->optics,
+                                p->S_part, p->max_groups, s),
+          "trotter_from_g");
+  HIP_TRY(launch_finish_object(p->n, p->S_part, p->info, p->nbf_hint, p->max_groups, p->tw, p->S,
+                               reinterpret_cast<float2*>(obj_out), phase_out, s),
+          "finish_object");
+  return QSSB_OK;
+}
+
+int ssb_recon_u16(ssb_plan* p, const uint16_t* data, double c10, double c12, double phi12, double rotation_rad,
+                  float threshold, float bf_radius, void* obj_out, float* phase_out, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!data) return fail(QSSB_ERR_ARG, "data is NULL");
+  hipStream_t s = as_stream(stream);
+  const long long frames = (long long)p->n * p->n;
+  HIP_TRY(launch_mean_dp(data, frames, p->D, p->dp_partial, p->max_ranges, p->dp_sums, p->dp_mean, s), "mean_dp");
+  HIP_TRY(launch_select_bf(p->dp_mean, p->dp_sums, p->det_rows, p->det_cols, threshold, bf_radius, p->max_bf,
+                           p->bf_rows, p->bf_cols, p->info, s),
+          "select_bf");
+  p->rotation = rotation_rad;
+  rc = ensure_geometry(p, s);
+  if (rc) return rc;
+  p->gcols = 0;
+  HIP_TRY(launch_gather_rowfft(p->n, data, p->D, p->det_cols, p->bf_rows, p->bf_cols, p->info, p->nbf_hint, p->tw,
+                               p->H, s),
+          "gather_rowfft");
+  const QuadChi m = quad_chi(p, c10, c12, phi12);
+  HIP_TRY(launch_colfft_trotter(p->n, p->H, p->info, p->nbf_hint, geom_ptrs(p), m, p->optics, p->tw, p->S_part,
+                                p->max_groups, nullptr, true, s),
+          "colfft_trotter");
+  HIP_TRY(launch_finish_object(p->n, p->S_part, p->info, p->nbf_hint, p->max_groups, p->tw, p->S,
+                               reinterpret_cast<float2*>(obj_out), phase_out, s),
+          "finish_object");
+  return QSSB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,487 @@
+// Data-preparation kernels of the MI355X SSB engine: the uint16 mean
+// diffraction pattern stream, the on-device bright-field selection, the
+// per-BF detector geometry, and the BF gather fused with the scan-row FFT.
+//
+// Reference behaviour (paths relative to bobleesj/quantem.gpu):
+//   mean DP / mask     src/quantem/gpu/ssb/compute/cuda/backend.py:420-518
+//   geometry           src/quantem/gpu/ssb/compute/cuda/engine.py:784-848
+//   G extraction       src/quantem/gpu/ssb/compute/cuda/backend.py:520-589
+#include "ssb_internal.h"
+
+namespace qssb {
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// ------------------------------------------------------------------------
+// Mean DP, pass 1: per frame-range uint32 partial sums.
+// grid = (pixel tiles of 256*VEC, frame ranges).  Every workgroup streams a
+// contiguous 256*VEC-pixel slice of `frames_per_range` consecutive frames,
+// so each wave issues 16-byte coalesced non-temporal loads.  A range holds
+// at most 65536 frames, so uint32 cannot overflow (65536 * 65535 < 2^32).
+// ------------------------------------------------------------------------
+template <int VEC>
+__global__ void __launch_bounds__(256)
+k_dp_partial(const uint16_t* __restrict__ data, long long frames, int D, int frames_per_range,
+             uint32_t* __restrict__ partial) {
+  const int pix0 = (blockIdx.x * 256 + threadIdx.x) * VEC;
+  const long long f0 = (long long)blockIdx.y * frames_per_range;
+  long long f1 = f0 + frames_per_range;
+  if (f1 > frames) f1 = frames;
+  if (pix0 >= D) return;
+  uint32_t acc[VEC];
+#pragma unroll
+  for (int e = 0; e < VEC; ++e) acc[e] = 0u;
+  const uint16_t* p = data + f0 * (long long)D + pix0;
+  if constexpr (VEC == 8) {
+    long long f = f0;
+    for (; f + 4 <= f1; f += 4) {
+      u32x4 v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        v[u] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p + (long long)u * D));
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+          acc[2 * w] += v[u][w] & 0xFFFFu;
+          acc[2 * w + 1] += v[u][w] >> 16;
+        }
+      }
+      p += 4LL * D;
+    }
+    for (; f < f1; ++f) {
+      u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        acc[2 * w] += v[w] & 0xFFFFu;
+        acc[2 * w + 1] += v[w] >> 16;
+      }
+      p += D;
+    }
+    uint32_t* out = partial + (long long)blockIdx.y * D + pix0;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) out[e] = acc[e];
+  } else {
+    for (long long f = f0; f < f1; ++f) {
+      acc[0] += p[0];
+      p += D;
+    }
+    partial[(long long)blockIdx.y * D + pix0] = acc[0];
+  }
+}
+
+// Pass 2: fold the frame ranges into exact uint64 sums; mean = f32(sum)/f32(N)
+// with one correctly rounded division (backend.py:438-441).
+__global__ void __launch_bounds__(256)
+k_dp_reduce(const uint32_t* __restrict__ partial, int ranges, int D, long long frames,
+            unsigned long long* __restrict__ sums, float* __restrict__ mean) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= D) return;
+  unsigned long long s = 0ull;
+  for (int r = 0; r < ranges; ++r) s += partial[(long long)r * D + i];
+  sums[i] = s;
+  mean[i] = (float)s / (float)frames;
+}
+
+// ------------------------------------------------------------------------
+// Bright-field selection in one workgroup (1024 threads).
+// backend.py:448-518, see select_bf() in the oracle for the float32 rules.
+// ------------------------------------------------------------------------
+#define MASK_THREADS 1024
+
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
+  s = a + b;
+  float bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+// Block reduction of a compensated (s, c) pair.
+__device__ float block_sum_comp(float s, float c, float* sh_s, float* sh_c) {
+  const int tid = threadIdx.x;
+  sh_s[tid] = s;
+  sh_c[tid] = c;
+  __syncthreads();
+  for (int off = MASK_THREADS / 2; off > 0; off >>= 1) {
+    if (tid < off) {
+      float hs, he;
+      two_sum(sh_s[tid], sh_s[tid + off], hs, he);
+      sh_s[tid] = hs;
+      sh_c[tid] = sh_c[tid] + sh_c[tid + off] + he;
+    }
+    __syncthreads();
+  }
+  float r = sh_s[0] + sh_c[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ float block_max(float v, float* sh) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int off = MASK_THREADS / 2; off > 0; off >>= 1) {
+    if (tid < off) sh[tid] = fmaxf(sh[tid], sh[tid + off]);
+    __syncthreads();
+  }
+  float r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ long long block_sum_i64(long long v, long long* sh) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int off = MASK_THREADS / 2; off > 0; off >>= 1) {
+    if (tid < off) sh[tid] += sh[tid + off];
+    __syncthreads();
+  }
+  long long r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// float32 squared distance without contraction into FMA (matches the
+// separately rounded CuPy elementwise ops of backend.py:509-511).
+__device__ __forceinline__ float dist2_f32(float r, float c, float cr, float cc) {
+#pragma clang fp contract(off)
+  float a = r - cr;
+  float b = c - cc;
+  float aa = a * a;
+  float bb = b * b;
+  return aa + bb;
+}
+
+// f32(n / pi) from an exact product with 1/pi split in hi + lo parts.
+__device__ __forceinline__ float count_over_pi(float n) {
+  const float inv_pi_hi = 0.3183098733425140380859375f;       // f32(1/pi)
+  const float inv_pi_lo = 1.284127663345183e-08f;               // 1/pi - hi
+  float p = n * inv_pi_hi;
+  float e = fmaf(n, inv_pi_hi, -p);
+  return p + (e + n * inv_pi_lo);
+}
+
+__global__ void __launch_bounds__(MASK_THREADS)
+k_select_bf(const float* __restrict__ mean, const unsigned long long* __restrict__ sums,
+            int det_rows, int det_cols, float threshold, float bf_radius, int max_bf,
+            int* __restrict__ bf_rows, int* __restrict__ bf_cols, BfInfo* __restrict__ info) {
+  __shared__ float sh_a[MASK_THREADS];
+  __shared__ float sh_b[MASK_THREADS];
+  __shared__ long long sh_i[MASK_THREADS];
+  const int tid = threadIdx.x;
+  const int D = det_rows * det_cols;
+  const int per = (D + MASK_THREADS - 1) / MASK_THREADS;
+  const int p0 = tid * per;
+  const int p1 = min(D, p0 + per);
+
+  float lmax = -INFINITY;
+  for (int i = p0; i < p1; ++i) lmax = fmaxf(lmax, mean[i]);
+  const float mmax = block_max(lmax, sh_a);
+  const float cut = mmax * threshold;
+
+  float center_r, center_c, r2;
+  int status = 0;
+  if (bf_radius <= 0.0f) {
+    // mean + std threshold (population std), compensated sums
+    Kahan k;
+    k.init();
+    for (int i = p0; i < p1; ++i) k.add(mean[i]);
+    const float total = block_sum_comp(k.s, -k.c, sh_a, sh_b);
+    const float mu = total / (float)D;
+    Kahan q;
+    q.init();
+    for (int i = p0; i < p1; ++i) {
+      float d = mean[i] - mu;
+      q.add(d * d);
+    }
+    const float ss = block_sum_comp(q.s, -q.c, sh_a, sh_b);
+    const float sd = sqrtf(ss / (float)D);
+    const float thr = mu + sd;
+    long long cnt = 0, sr = 0, sc = 0;
+    for (int i = p0; i < p1; ++i) {
+      if (mean[i] > thr) {
+        cnt += 1;
+        sr += i / det_cols;
+        sc += i % det_cols;
+      }
+    }
+    cnt = block_sum_i64(cnt, sh_i);
+    sr = block_sum_i64(sr, sh_i);
+    sc = block_sum_i64(sc, sh_i);
+    if (cnt > 0) {
+      center_r = (float)sr / (float)cnt;
+      center_c = (float)sc / (float)cnt;
+      r2 = count_over_pi((float)cnt);
+    } else {
+      center_r = det_rows * 0.5f;
+      center_c = det_cols * 0.5f;
+      float rad = fminf((float)det_rows, (float)det_cols) * 0.25f;
+      r2 = rad * rad;
+    }
+  } else {
+    Kahan kw, kr, kc;
+    kw.init();
+    kr.init();
+    kc.init();
+    long long n = 0, sr = 0, sc = 0;
+    for (int i = p0; i < p1; ++i) {
+      const float w = mean[i];
+      if (w > cut) {
+        const int r = i / det_cols, c = i % det_cols;
+        kw.add(w);
+        kr.add((float)r * w);
+        kc.add((float)c * w);
+        n += 1;
+        sr += r;
+        sc += c;
+      }
+    }
+    const float wsum = block_sum_comp(kw.s, -kw.c, sh_a, sh_b);
+    const float rsum = block_sum_comp(kr.s, -kr.c, sh_a, sh_b);
+    const float csum = block_sum_comp(kc.s, -kc.c, sh_a, sh_b);
+    n = block_sum_i64(n, sh_i);
+    sr = block_sum_i64(sr, sh_i);
+    sc = block_sum_i64(sc, sh_i);
+    if (wsum > 0.0f) {
+      center_r = rsum / wsum;
+      center_c = csum / wsum;
+    } else if (n > 0) {
+      center_r = (float)sr / (float)n;
+      center_c = (float)sc / (float)n;
+    } else {
+      center_r = 0.f;
+      center_c = 0.f;
+    }
+    r2 = bf_radius * bf_radius;
+  }
+
+  // Keep candidate pixels inside the disk, compacted in raster order.
+  int local = 0;
+  long long any_cand = 0;
+  for (int i = p0; i < p1; ++i) {
+    if (mean[i] > cut) {
+      any_cand += 1;
+      if (dist2_f32((float)(i / det_cols), (float)(i % det_cols), center_r, center_c) <= r2) local += 1;
+    }
+  }
+  any_cand = block_sum_i64(any_cand, sh_i);
+  // exclusive scan of `local`
+  sh_i[tid] = local;
+  __syncthreads();
+  for (int off = 1; off < MASK_THREADS; off <<= 1) {
+    long long v = (tid >= off) ? sh_i[tid - off] : 0;
+    __syncthreads();
+    sh_i[tid] += v;
+    __syncthreads();
+  }
+  const long long incl = sh_i[tid];
+  const long long nbf = sh_i[MASK_THREADS - 1];
+  __syncthreads();
+  long long pos = incl - local;
+  unsigned long long dcs = 0ull;
+  for (int i = p0; i < p1; ++i) {
+    if (mean[i] > cut &&
+        dist2_f32((float)(i / det_cols), (float)(i % det_cols), center_r, center_c) <= r2) {
+      if (pos < max_bf) {
+        bf_rows[pos] = i / det_cols;
+        bf_cols[pos] = i % det_cols;
+      }
+      dcs += sums[i];
+      ++pos;
+    }
+  }
+  const long long dc_total = block_sum_i64((long long)dcs, sh_i);
+  if (tid == 0) {
+    if (any_cand == 0) status = QSSB_DEV_NO_CANDIDATES;
+    else if (nbf == 0) status = QSSB_DEV_EMPTY_DISK;
+    else if (nbf > max_bf) status = QSSB_DEV_CAPACITY;
+    const int nb = (status == 0) ? (int)nbf : 0;
+    info->nbf = nb;
+    info->status = status;
+    info->center_r = center_r;
+    info->center_c = center_c;
+    info->radius2 = r2;
+    float dc = 0.0f;
+    if (nb > 0) {
+      // G_b(0,0) is the exact integer column sum; dc = mean_b G_b(0,0).
+      const unsigned long long tot = (unsigned long long)dc_total;
+      const unsigned long long qd = tot / (unsigned long long)nb;
+      const unsigned long long rm = tot % (unsigned long long)nb;
+      dc = (float)qd + (float)rm / (float)nb;
+    }
+    info->dc_re = dc;
+    info->dc_im = 0.0f;
+  }
+}
+
+// ------------------------------------------------------------------------
+// Per-BF geometry for one scan rotation (engine.py:784-848), evaluated in
+// double-float: k = R(-rotation) * (index - centre) * recip, and A(k).
+// ------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_bf_geometry(const int* __restrict__ bf_rows, const int* __restrict__ bf_cols,
+              const BfInfo* __restrict__ info, int max_bf, df recip_y, df recip_x, df cos_a, df sin_a, Optics o,
+              float* __restrict__ kx_out, float* __restrict__ kxl_out, float* __restrict__ ky_out,
+              float* __restrict__ kyl_out, float* __restrict__ ak_out) {
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  const int nbf = info->nbf;
+  if (b >= nbf || b >= max_bf) return;
+  const df iy = two_sum((float)bf_rows[b], -info->center_r);
+  const df ix = two_sum((float)bf_cols[b], -info->center_c);
+  const df kxa = df_mul(iy, recip_y);
+  const df kya = df_mul(ix, recip_x);
+  const df kx = df_add(df_mul(kxa, cos_a), df_mul(kya, sin_a));
+  const df ky = df_sub(df_mul(kya, cos_a), df_mul(kxa, sin_a));
+  kx_out[b] = kx.hi;
+  kxl_out[b] = kx.lo;
+  ky_out[b] = ky.hi;
+  kyl_out[b] = ky.lo;
+  ak_out[b] = aperture_band(kx, ky, o);
+}
+
+// ------------------------------------------------------------------------
+// BF gather + scan-row real FFT.
+// Work item = (scan row R_r, chunk of 2*PAIRS BF pixels).  The chunk's
+// pixels from the N frames of the scan row are read (u16 -> f32), paired as
+// re/im of PAIRS complex sequences along R_c, transformed with one complex
+// FFT each, and split into the two real-input half spectra.  Output H is
+// [b][R_r][slot], slot 0 packing the purely real bins 0 and N/2 as
+// (X[0], X[N/2]); slots 1..N/2-1 hold X[k].
+// ------------------------------------------------------------------------
+template <int N, int PAIRS>
+__global__ void __launch_bounds__(256)
+k_gather_rowfft(const uint16_t* __restrict__ data, int D, int det_cols,
+                const int* __restrict__ bf_rows, const int* __restrict__ bf_cols,
+                const BfInfo* __restrict__ info, const float2* __restrict__ tw_g,
+                float2* __restrict__ H) {
+  constexpr int STRIDE = N + 1;
+  constexpr int CH = 2 * PAIRS;
+  __shared__ float2 z[PAIRS * STRIDE];
+  __shared__ float2 tw[N];
+  __shared__ int offs[CH];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < N; i += 256) tw[i] = tw_g[i];
+  const int nbf = info->nbf;
+  const int nchunks = (nbf + CH - 1) / CH;
+  const long long items = (long long)nchunks * N;
+  for (long long item = blockIdx.x; item < items; item += gridDim.x) {
+    const int chunk = (int)(item / N);
+    const int rr = (int)(item - (long long)chunk * N);
+    const int b0 = chunk * CH;
+    __syncthreads();
+    if (tid < CH) {
+      const int b = b0 + tid;
+      offs[tid] = (b < nbf) ? bf_rows[b] * det_cols + bf_cols[b] : -1;
+    }
+    __syncthreads();
+    // gather: lane -> BF pixel within the chunk, lane group -> frame
+    {
+      constexpr int FR = 256 / CH;
+      const int j = tid % CH;
+      const int off = offs[j];
+      float* zf = reinterpret_cast<float*>(z);
+      const long long frame0 = (long long)rr * N;
+      for (int c = tid / CH; c < N; c += FR) {
+        float v = 0.0f;
+        if (off >= 0) v = (float)data[(frame0 + c) * (long long)D + off];
+        zf[2 * ((j >> 1) * STRIDE + c) + (j & 1)] = v;
+      }
+    }
+    __syncthreads();
+    fft_lds<N, PAIRS, 256, STRIDE, false>(z, tw);
+    // split the pairs into half spectra and store
+    constexpr int HALF = N / 2;
+    for (int it = tid; it < CH * HALF; it += 256) {
+      const int jb = it / HALF;           // BF within chunk
+      const int k = it - jb * HALF;       // slot
+      const int b = b0 + jb;
+      if (b >= nbf) continue;
+      const int p = jb >> 1;
+      const float2 zk = z[p * STRIDE + k];
+      float2 out;
+      if (k == 0) {
+        const float2 zh = z[p * STRIDE + HALF];
+        out = (jb & 1) ? make_float2(zk.y, zh.y) : make_float2(zk.x, zh.x);
+      } else {
+        const float2 zm = z[p * STRIDE + (N - k)];
+        if ((jb & 1) == 0) {
+          out = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+        } else {
+          // (Z[k] - conj(Z[N-k])) / (2i)
+          const float dr = zk.x - zm.x, di = zk.y + zm.y;
+          out = make_float2(0.5f * di, -0.5f * dr);
+        }
+      }
+      H[((long long)b * N + rr) * HALF + k] = out;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// Launch wrappers
+// ------------------------------------------------------------------------
+hipError_t launch_mean_dp(const uint16_t* data, long long frames, int D, uint32_t* partial,
+                          int max_ranges, unsigned long long* sums, float* mean, hipStream_t s) {
+  int ranges = (int)((frames + 255) / 256);
+  if (ranges > max_ranges) ranges = max_ranges;
+  if (ranges < 1) ranges = 1;
+  int fpr = (int)((frames + ranges - 1) / ranges);
+  ranges = (int)((frames + fpr - 1) / fpr);
+  const bool vec = (D % 8 == 0) && ((reinterpret_cast<uintptr_t>(data) & 15u) == 0);
+  if (vec) {
+    dim3 grid((D / 8 + 255) / 256, ranges);
+    hipLaunchKernelGGL(k_dp_partial<8>, grid, dim3(256), 0, s, data, frames, D, fpr, partial);
+  } else {
+    dim3 grid((D + 255) / 256, ranges);
+    hipLaunchKernelGGL(k_dp_partial<1>, grid, dim3(256), 0, s, data, frames, D, fpr, partial);
+  }
+  hipLaunchKernelGGL(k_dp_reduce, dim3((D + 255) / 256), dim3(256), 0, s, partial, ranges, D, frames,
+                     sums, mean);
+  return hipGetLastError();
+}
+
+hipError_t launch_select_bf(const float* mean, const unsigned long long* sums, int det_rows, int det_cols,
+                            float threshold, float bf_radius, int max_bf, int* rows, int* cols,
+                            BfInfo* info, hipStream_t s) {
+  hipLaunchKernelGGL(k_select_bf, dim3(1), dim3(MASK_THREADS), 0, s, mean, sums, det_rows, det_cols,
+                     threshold, bf_radius, max_bf, rows, cols, info);
+  return hipGetLastError();
+}
+
+hipError_t launch_bf_geometry(const int* rows, const int* cols, const BfInfo* info, int max_bf, df recip_y,
+                              df recip_x, df cos_a, df sin_a, const Optics& o, float* kx, float* kxl, float* ky,
+                              float* kyl, float* ak, hipStream_t s) {
+  hipLaunchKernelGGL(k_bf_geometry, dim3((max_bf + 255) / 256), dim3(256), 0, s, rows, cols, info, max_bf,
+                     recip_y, recip_x, cos_a, sin_a, o, kx, kxl, ky, kyl, ak);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_rowfft(int N, const uint16_t* data, int D, int det_cols, const int* rows,
+                                const int* cols, const BfInfo* info, int nbf_hint, const float2* tw,
+                                float2* H, hipStream_t s) {
+  auto grid_for = [&](int ch) {
+    long long items = (long long)((nbf_hint + ch - 1) / ch) * N;
+    if (items < 1) items = 1;
+    if (items > 16384) items = 16384;
+    return dim3((unsigned)items);
+  };
+  switch (N) {
+    case 128:
+      hipLaunchKernelGGL((k_gather_rowfft<128, 16>), grid_for(32), dim3(256), 0, s, data, D, det_cols, rows,
+                         cols, info, tw, H);
+      break;
+    case 256:
+      hipLaunchKernelGGL((k_gather_rowfft<256, 16>), grid_for(32), dim3(256), 0, s, data, D, det_cols, rows,
+                         cols, info, tw, H);
+      break;
+    case 512:
+      hipLaunchKernelGGL((k_gather_rowfft<512, 8>), grid_for(16), dim3(256), 0, s, data, D, det_cols, rows,
+                         cols, info, tw, H);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace qssb

@@ -1,0 +1,270 @@
+"""HIP SSB engine: a plan on one MI355X plus the operations of the reference engine.
+
+Mirrors the seam the reference's tests drive, ``SSBEngine``
+(``cuda/engine.py:402-2295``): construction from a Fourier stack and BF
+geometry, ``cache_rotation``, ``reconstruct_object``, ``reconstruct``,
+``reconstruct_with_loss``, ``variance_loss(_batch)``, ``export_state``.
+Device memory and streams come from PyTorch; every computation runs in the
+native library through ``_abi``.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("The hip SSB backend requires a ROCm GPU; no CPU fallback exists.")
+    return torch
+
+
+@dataclass(frozen=True)
+class PlanSpec:
+    scan_n: int
+    det_rows: int
+    det_cols: int
+    max_bf: int
+    wavelength_A: float
+    semiangle_rad: float
+    ang_row_rad: float
+    ang_col_rad: float
+    scan_sampling_row_A: float
+    scan_sampling_col_A: float
+
+
+class SSBPlan:
+    """Owns one native plan (device buffers for one scan/detector geometry)."""
+
+    def __init__(self, spec: PlanSpec, device=None):
+        torch = _torch()
+        self.spec = spec
+        dev = torch.device(device if device is not None else "cuda")
+        if dev.index is None:
+            dev = torch.device("cuda", torch.cuda.current_device())
+        self.device = dev
+        handle = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _abi.call("ssb_plan_create", spec.scan_n, spec.det_rows, spec.det_cols, spec.max_bf,
+                      spec.wavelength_A, spec.semiangle_rad, spec.ang_row_rad, spec.ang_col_rad,
+                      spec.scan_sampling_row_A, spec.scan_sampling_col_A, ctypes.byref(handle))
+        self._h = handle
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        if self._h is None:
+            raise ValueError("SSB plan is closed")
+        return self._h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None:
+            _abi.lib().ssb_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class BrightFieldState:
+    rows: np.ndarray
+    cols: np.ndarray
+    center: tuple[float, float]
+    radius2: float
+    dc: complex
+
+
+class HipSSBEngine:
+    """SSB computations on one plan; arrays are torch tensors on the plan's GPU."""
+
+    backend = "hip"
+
+    def __init__(self, plan: SSBPlan):
+        self.plan = plan
+        self.torch = _torch()
+        self._bf: BrightFieldState | None = None
+        self._rotation: float | None = None
+        self._has_g = False
+
+    # ------------------------------------------------------------------ utils
+    @property
+    def n(self) -> int:
+        return self.plan.spec.scan_n
+
+    def _stream(self):
+        return _abi.stream_handle(self.torch.cuda.current_stream(self.plan.device))
+
+    def _check_u16(self, data):
+        torch = self.torch
+        if not isinstance(data, torch.Tensor) or data.dtype != torch.uint16:
+            raise ValueError("data must be a torch.uint16 tensor on the GPU")
+        if not data.is_cuda or data.device != self.plan.device:
+            raise ValueError(f"data must live on {self.plan.device}")
+        if not data.is_contiguous():
+            raise ValueError("data must be contiguous")
+        s = self.plan.spec
+        if data.ndim != 4 or tuple(data.shape) != (s.scan_n, s.scan_n, s.det_rows, s.det_cols):
+            raise ValueError(
+                f"data must have shape ({s.scan_n}, {s.scan_n}, {s.det_rows}, {s.det_cols}); "
+                f"got {tuple(data.shape)}"
+            )
+
+    # ------------------------------------------------------------ preparation
+    def mean_dp(self, data):
+        """(uint64 sums, float32 mean DP) of resident uint16 data."""
+        torch = self.torch
+        self._check_u16(data)
+        s = self.plan.spec
+        sums = torch.empty((s.det_rows, s.det_cols), dtype=torch.uint64, device=self.plan.device)
+        mean = torch.empty((s.det_rows, s.det_cols), dtype=torch.float32, device=self.plan.device)
+        _abi.call("ssb_mean_dp_u16", self.plan.handle, _abi.ptr(data), s.scan_n * s.scan_n,
+                  _abi.ptr(sums), _abi.ptr(mean), self._stream())
+        return sums, mean
+
+    def select_bf(self, threshold: float = 0.0, bf_radius: float | None = None) -> BrightFieldState:
+        """Device BF selection from the last mean DP; reads the result back."""
+        _abi.call("ssb_select_bf", self.plan.handle, float(threshold),
+                  float(bf_radius) if bf_radius is not None else 0.0, self._stream())
+        self._has_g = False
+        self._rotation = None
+        return self.bf_state()
+
+    def bf_state(self) -> BrightFieldState:
+        cap = self.plan.spec.max_bf
+        rows = np.zeros(cap, dtype=np.int32)
+        cols = np.zeros(cap, dtype=np.int32)
+        nbf = ctypes.c_int(0)
+        info = np.zeros(6, dtype=np.float32)
+        _abi.call("ssb_get_bf", self.plan.handle,
+                  rows.ctypes.data_as(_abi._ip), cols.ctypes.data_as(_abi._ip), cap,
+                  ctypes.byref(nbf), info.ctypes.data_as(_abi._fp), self._stream())
+        status = int(info[5])
+        if status == 1:
+            raise ValueError("No bright-field pixels found with the given threshold. Check that the "
+                             "data contains a visible BF disk, or lower the threshold.")
+        if status == 2:
+            raise ValueError("No bright-field pixels within bf_radius. Increase bf_radius or check "
+                             "detector geometry.")
+        if status == 3:
+            raise ValueError(f"More bright-field pixels than the plan capacity ({cap}).")
+        k = int(nbf.value)
+        self._bf = BrightFieldState(rows[:k].copy(), cols[:k].copy(),
+                                    (float(info[0]), float(info[1])), float(info[2]),
+                                    complex(float(info[3]), float(info[4])))
+        return self._bf
+
+    def set_bf(self, rows, cols, center, dc) -> None:
+        rows = np.ascontiguousarray(np.asarray(rows, dtype=np.int32).reshape(-1))
+        cols = np.ascontiguousarray(np.asarray(cols, dtype=np.int32).reshape(-1))
+        if rows.shape != cols.shape or rows.size == 0:
+            raise ValueError("bf rows and cols must be non-empty matching vectors")
+        dc = complex(dc)
+        _abi.call("ssb_set_bf", self.plan.handle, rows.ctypes.data_as(_abi._ip),
+                  cols.ctypes.data_as(_abi._ip), int(rows.size), float(center[0]), float(center[1]),
+                  float(dc.real), float(dc.imag), self._stream())
+        self._bf = BrightFieldState(rows.copy(), cols.copy(), (float(center[0]), float(center[1])),
+                                    0.0, dc)
+        self._has_g = False
+        self._rotation = None
+
+    @property
+    def num_bf(self) -> int:
+        if self._bf is None:
+            raise ValueError("no bright-field selection")
+        return int(self._bf.rows.size)
+
+    def set_detector_sampling(self, recip_row: float, recip_col: float) -> None:
+        _abi.call("ssb_set_detector_sampling", self.plan.handle, float(recip_row), float(recip_col))
+        self._rotation = None
+
+    def set_scan_frequencies(self, q_row, q_col) -> None:
+        qr = np.ascontiguousarray(np.asarray(q_row, dtype=np.float32).reshape(-1))
+        qc = np.ascontiguousarray(np.asarray(q_col, dtype=np.float32).reshape(-1))
+        if qr.size != self.n or qc.size != self.n:
+            raise ValueError(f"scan frequencies must have {self.n} entries")
+        _abi.call("ssb_set_scan_frequencies", self.plan.handle, qr.ctypes.data_as(_abi._fp),
+                  qc.ctypes.data_as(_abi._fp))
+
+    def cache_rotation(self, rotation_rad: float, force: bool = False) -> None:
+        if self._rotation == rotation_rad and not force:
+            return
+        _abi.call("ssb_cache_rotation", self.plan.handle, float(rotation_rad), self._stream())
+        self._rotation = float(rotation_rad)
+
+    def geometry(self):
+        """(kx, ky, aperture) float32 tensors of the cached rotation."""
+        torch = self.torch
+        nb = self.num_bf
+        out = [torch.empty(nb, dtype=torch.float32, device=self.plan.device) for _ in range(3)]
+        _abi.call("ssb_get_geometry", self.plan.handle, *(_abi.ptr(t) for t in out), nb, self._stream())
+        return tuple(out)
+
+    def prepare_g(self, data) -> None:
+        self._check_u16(data)
+        _abi.call("ssb_prepare_g_u16", self.plan.handle, _abi.ptr(data), self._stream())
+        self._has_g = True
+
+    def import_g(self, g_qk) -> None:
+        torch = self.torch
+        g = g_qk
+        if not isinstance(g, torch.Tensor):
+            g = torch.as_tensor(np.asarray(g, dtype=np.complex64))
+        g = g.to(device=self.plan.device, dtype=torch.complex64).contiguous()
+        if g.ndim != 3 or g.shape[1] != self.n or g.shape[2] not in (self.n, self.n // 2 + 1):
+            raise ValueError(
+                f"G_qk must have shape (num_bf, {self.n}, {self.n}) or Hermitian "
+                f"(num_bf, {self.n}, {self.n // 2 + 1})"
+            )
+        if g.shape[0] != self.num_bf:
+            raise ValueError(
+                "G_qk first dimension must match bf_inds_row and bf_inds_col lengths; "
+                f"got G_qk.shape[0]={g.shape[0]}, num_bf={self.num_bf}."
+            )
+        _abi.call("ssb_import_g", self.plan.handle, _abi.ptr(g), int(g.shape[0]), int(g.shape[2]),
+                  self._stream())
+        self._g_keepalive = g
+        self.torch.cuda.current_stream(self.plan.device).synchronize()
+        self._g_keepalive = None
+        self._has_g = True
+
+    @property
+    def G_qk(self):
+        """Resident Fourier stack in the reference layout ``[b, q_row, q_col<=N/2]``."""
+        torch = self.torch
+        out = torch.empty((self.num_bf, self.n, self.n // 2 + 1), dtype=torch.complex64,
+                          device=self.plan.device)
+        _abi.call("ssb_export_g", self.plan.handle, _abi.ptr(out), self.num_bf, self._stream())
+        return out
+
+    # -------------------------------------------------------- reconstruction
+    def reconstruct_object(self, c10: float, c12: float, phi12: float, phase_out=None):
+        """Complex object (complex64 [N, N]) from the resident stack."""
+        torch = self.torch
+        if not self._has_g:
+            raise ValueError("no resident Fourier stack")
+        if self._rotation is None:
+            self.cache_rotation(0.0)
+        obj = torch.empty((self.n, self.n), dtype=torch.complex64, device=self.plan.device)
+        _abi.call("ssb_object", self.plan.handle, float(c10), float(c12), float(phi12),
+                  _abi.ptr(obj), _abi.ptr(phase_out), self._stream())
+        return obj
+
+    def recon_u16(self, data, c10: float, c12: float, phi12: float, rotation_rad: float = 0.0,
+                  threshold: float = 0.0, bf_radius: float | None = None, obj_out=None, phase_out=None):
+        """End-to-end plain SSB from resident raw data (no host round trip)."""
+        self._check_u16(data)
+        _abi.call("ssb_recon_u16", self.plan.handle, _abi.ptr(data), float(c10), float(c12),
+                  float(phi12), float(rotation_rad), float(threshold),
+                  float(bf_radius) if bf_radius is not None else 0.0,
+                  _abi.ptr(obj_out), _abi.ptr(phase_out), self._stream())
+        self._rotation = float(rotation_rad)
+        self._has_g = False
