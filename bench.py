@@ -1,0 +1,179 @@
+#!/usr/bin/env python
+"""Plain SSB throughput on one MI355X: resident uint16 I[256,256,128,128] -> float32 phase.
+
+One step is one end-to-end reconstruction at the R1 boundary: exact uint64
+mean diffraction pattern over the full 2 GiB stream, on-device bright-field
+selection, BF gather + scan FFT fused with the trotter reduction, inverse
+FFT and angle.  The input is the seeded synthetic weak-phase dataset of the
+benchmark configuration (300 kV, 20 mrad, 1.25 mrad/px, defocus C10 = 200 A,
+5e4 e-/probe), generated on the GPU before timing.
+
+Prints one JSON line (see the repository contract): value = recon/s,
+roofline of the dominant kernel (the mean-DP stream) from hipEvents recorded
+inside the timed region, and the CPU baseline (the FP32 NumPy/SciPy
+restatement timed on this host on one full reconstruction).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO))
+
+METRIC = "SSB recon/s 256²×128² u16; aberration-fit s, learned vs search, same phase err"
+HBM_PEAK_GBS = 8000.0
+SCAN, DET = 256, 128
+C10 = 200.0                      # 20 nm defocus in the engine's numeric (Angstrom) convention
+TRAFFIC_FILE = REPO / "profiles" / "mean_dp_stream_traffic.txt"
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def read_traffic() -> float | None:
+    """HBM bytes per launch of the mean-DP stream, from the committed rocprofv3 counter summary."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    for line in TRAFFIC_FILE.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1].strip())
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline(data_host, cfg) -> dict:
+    """Time the FP32 NumPy/SciPy restatement of the same reconstruction on this host."""
+    import numpy as np
+    from oracle import ssb_oracle as O
+
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    p = O.prepare(data_host, voltage_kV=300, semiangle_mrad=cfg.semiangle_mrad,
+                  scan_sampling=cfg.step_A, det_sampling=cfg.det_sampling_mrad, dtype=np.float32)
+    obj = O.object_wave(p.G, p.dc, p.geom, C10, 0.0, 0.0)
+    _ = np.angle(obj)
+    dt = time.perf_counter() - t0
+    return {"value": 1.0 / dt, "unit": "recon/s", "cores": cores, "kind": "port",
+            "sample": f"1 full reconstruction of the same 256x256x128x128 u16 input "
+                      f"({dt:.2f} s; FP32 NumPy/SciPy restatement, scipy.fft workers={cores})"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true",
+                    help="skip the CPU baseline (used for profiler runs)")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one MI355X: the SSB path runs on one GPU and this build has no "
+              "multi-GPU mode (see DESIGN.md for how the path would shard).", file=sys.stderr)
+        return 2
+
+    import torch
+    from quantem.gpu_amd.ssb.synthetic import SyntheticConfig, generate
+    from quantem.gpu_amd.ssb.compute.hip.engine import SSBPlan, PlanSpec, HipSSBEngine
+    from quantem.gpu_amd.optics import electron_wavelength_angstrom
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        return 1
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    cfg = SyntheticConfig(scan=SCAN, det=DET, aberrations={"C10": C10}, dose=5e4, seed=2026)
+    log("generating synthetic 256x256x128x128 u16 input on the GPU")
+    data = generate(cfg, device="cuda")
+    torch.cuda.synchronize()
+
+    lam = electron_wavelength_angstrom(300e3)
+    spec = PlanSpec(SCAN, DET, DET, 4096, lam, cfg.semiangle_mrad * 1e-3,
+                    cfg.det_sampling_mrad * 1e-3, cfg.det_sampling_mrad * 1e-3, cfg.step_A, cfg.step_A)
+    eng = HipSSBEngine(SSBPlan(spec, device=dev))
+    phase = torch.empty((SCAN, SCAN), dtype=torch.float32, device=dev)
+
+    def step():
+        eng.recon_u16(data, C10, 0.0, 0.0, 0.0, 0.0, None, phase_out=phase)
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    torch.cuda.synchronize()
+    eng.mean_dp(data)
+    nbf = eng.select_bf(0.0, None).rows.size
+    log(f"warmup done; N_bf = {nbf}")
+
+    stream = torch.cuda.current_stream(dev)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    eng.timing_begin(args.steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    stages = eng.timing_end()
+    gpu_ms = e0.elapsed_time(e1)
+    ms_per_step = gpu_ms / args.steps
+    value = args.steps / (gpu_ms / 1e3)
+    log(f"{args.steps} steps: {ms_per_step:.4f} ms/recon (wall {wall * 1e3 / args.steps:.4f} ms)")
+
+    n_r, n_k = SCAN * SCAN, DET * DET
+    dom_bytes = 2.0 * n_r * n_k                       # raw u16 read once by the stream
+    dom_ms = stages["mean_dp_stream"] / stages["calls"]
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    recon_bytes = 2.0 * n_r * n_k + 2.0 * n_r * nbf + 4.0 * n_r
+    recon_achieved = recon_bytes / (ms_per_step * 1e-3) / 1e9
+    traffic = read_traffic()
+
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "recon/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded weak-phase forward model, Poisson, u16)",
+        "config": {"workload": "plain SSB, R1: resident u16 I[256,256,128,128] -> f32 phase[256,256]",
+                   "scan": [SCAN, SCAN], "detector": [DET, DET], "n_bf": int(nbf),
+                   "aberrations_A": {"C10": C10}, "voltage_kV": 300, "semiangle_mrad": 20.0,
+                   "det_sampling_mrad": 1.25, "dose": 5e4},
+        "roofline": {"bound": "hbm", "kernel": "k_dp_partial (mean_dp_stream)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms},
+        "recon_roofline": {"bound": "hbm", "achieved": recon_achieved, "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": recon_achieved / HBM_PEAK_GBS,
+                           "algorithmic_bytes_per_recon": recon_bytes},
+        "stages_ms_per_recon": {k: v / stages["calls"] for k, v in stages.items() if k != "calls"},
+    }
+    if not args.no_cpu_baseline:
+        log("timing the CPU baseline (FP32 NumPy/SciPy restatement, one reconstruction)")
+        host = data.cpu().numpy()
+        result["cpu_baseline"] = cpu_baseline(host, cfg)
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -150,6 +150,19 @@ int ssb_recon_u16(ssb_plan* plan, const uint16_t* data, double c10, double c12, 
                   double rotation_rad, float threshold, float bf_radius, void* obj_out, float* phase_out,
                   void* stream);
 
+/*
+ * Stage timing of ssb_recon_u16 with hipEvents recorded on the call's own
+ * stream (the reference keeps wall-clock stage timings in SSBResult.timings,
+ * qg/ssb/results.py:58-80).  ssb_timing_begin reserves events for up to
+ * max_calls recon calls; ssb_timing_end synchronises, writes the summed
+ * milliseconds of each stage (ssb_stage_count() entries) and the number of
+ * recorded calls, and stops timing.
+ */
+int ssb_stage_count(void);
+const char* ssb_stage_name(int stage);
+int ssb_timing_begin(ssb_plan* plan, int max_calls);
+int ssb_timing_end(ssb_plan* plan, float* stage_ms_host, int n_stages, int* n_calls_host);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,9 +46,23 @@ struct ssb_plan {
   float2* H = nullptr;
   float2* S_part = nullptr;
   float2* S = nullptr;
+  float2* rowph = nullptr;
+  float2* colph = nullptr;
+  float2* echi = nullptr;
   float2* Gt = nullptr;
   size_t Gt_bytes = 0;
+  // stage timing
+  std::vector<hipEvent_t> events;   // (NSTAGES + 1) per recorded call
+  int timing_calls = 0;
+  int timing_max = 0;
 };
+
+namespace {
+// Stages of ssb_recon_u16, in launch order.
+const char* const kStageNames[] = {"mean_dp_stream", "mean_dp_reduce", "select_bf", "geometry",
+                                   "gather_rowfft", "colfft_trotter", "finish_object"};
+constexpr int NSTAGES = 7;
+}
 
 namespace {
 
@@ -142,6 +156,8 @@ __global__ void k_import_g(const float2* __restrict__ in, int gcols, int n, int 
 
 }  // namespace
 
+static void release_events(ssb_plan* p);
+
 extern "C" {
 
 int ssb_version(void) { return 100; }
@@ -169,7 +185,7 @@ int ssb_plan_create(int scan_n, int det_rows, int det_cols, int max_bf, double w
   p->D = det_rows * det_cols;
   p->max_bf = max_bf;
   p->nbf_hint = max_bf;
-  p->max_groups = group_count(scan_n, max_bf);
+  p->max_groups = object_groups(scan_n);
   p->wavelength = wavelength_A;
   p->semiangle = semiangle_rad;
   p->ang_y = ang_row_rad;
@@ -219,6 +235,9 @@ int ssb_plan_create(int scan_n, int det_rows, int det_cols, int max_bf, double w
   if (e == hipSuccess) e = dmalloc(&p->H, (size_t)max_bf * n * (n / 2));
   if (e == hipSuccess) e = dmalloc(&p->S_part, (size_t)p->max_groups * n * n);
   if (e == hipSuccess) e = dmalloc(&p->S, (size_t)n * n);
+  if (e == hipSuccess) e = dmalloc(&p->rowph, (size_t)max_bf * n);
+  if (e == hipSuccess) e = dmalloc(&p->colph, (size_t)max_bf * (n / 2 + 1));
+  if (e == hipSuccess) e = dmalloc(&p->echi, (size_t)n * n);
   if (e != hipSuccess) {
     ssb_plan_destroy(p);
     return hip_fail(e, "ssb_plan_create allocation");
@@ -293,7 +312,11 @@ int ssb_plan_destroy(ssb_plan* p) {
   dfree(p->H);
   dfree(p->S_part);
   dfree(p->S);
+  dfree(p->rowph);
+  dfree(p->colph);
+  dfree(p->echi);
   dfree(p->Gt);
+  release_events(p);
   delete p;
   return QSSB_OK;
 }
@@ -435,8 +458,8 @@ int ssb_prepare_g_u16(ssb_plan* p, const uint16_t* data, void* stream) {
   HIP_TRY(launch_gather_rowfft(p->n, data, p->D, p->det_cols, p->bf_rows, p->bf_cols, p->info, p->nbf_hint, p->tw,
                                p->H, s),
           "gather_rowfft");
-  HIP_TRY(launch_colfft_trotter(p->n, p->H, p->info, p->nbf_hint, geom_ptrs(p), QuadChi{}, p->optics, p->tw,
-                                p->S_part, p->max_groups, p->Gt, false, s),
+  HIP_TRY(launch_trotter_slots(p->n, true, p->H, p->info, geom_ptrs(p), p->rowph, p->colph, p->echi, p->optics, p->tw,
+                               p->S_part, p->Gt, false, s),
           "colfft");
   p->gcols = p->n / 2 + 1;
   p->g_nbf = p->nbf_hint;
@@ -487,10 +510,16 @@ int ssb_object(ssb_plan* p, double c10, double c12, double phi12, void* obj_out,
     if (rc) return rc;
   }
   const QuadChi m = quad_chi(p, c10, c12, phi12);
-  HIP_TRY(launch_trotter_from_g(p->n, p->gcols, p->Gt, p->info, p->nbf_hint, geom_ptrs(p), m, p->optics,
-                                p->S_part, p->max_groups, s),
-          "trotter_from_g");
-  HIP_TRY(launch_finish_object(p->n, p->S_part, p->info, p->nbf_hint, p->max_groups, p->tw, p->S,
+  HIP_TRY(launch_phasors(p->n, p->info, p->nbf_hint, geom_ptrs(p), m, p->rowph, p->colph, p->echi, s), "phasors");
+  if (p->gcols == p->n / 2 + 1) {
+    HIP_TRY(launch_trotter_slots(p->n, false, p->Gt, p->info, geom_ptrs(p), p->rowph, p->colph, p->echi, p->optics, p->tw,
+                                 p->S_part, nullptr, true, s),
+            "trotter");
+  } else {
+    HIP_TRY(launch_trotter_full(p->n, p->Gt, p->info, geom_ptrs(p), p->rowph, m, p->optics, p->S_part, s),
+            "trotter_full");
+  }
+  HIP_TRY(launch_finish_object(p->n, p->S_part, p->info, p->tw, p->S,
                                reinterpret_cast<float2*>(obj_out), phase_out, s),
           "finish_object");
   return QSSB_OK;
@@ -502,25 +531,96 @@ int ssb_recon_u16(ssb_plan* p, const uint16_t* data, double c10, double c12, dou
   if (rc) return rc;
   if (!data) return fail(QSSB_ERR_ARG, "data is NULL");
   hipStream_t s = as_stream(stream);
+  hipEvent_t* ev = nullptr;
+  if (p->timing_max > 0 && p->timing_calls < p->timing_max) {
+    ev = &p->events[(size_t)p->timing_calls * (NSTAGES + 1)];
+    p->timing_calls += 1;
+  }
+  auto mark = [&](int i) {
+    if (ev) (void)hipEventRecord(ev[i], s);
+  };
   const long long frames = (long long)p->n * p->n;
-  HIP_TRY(launch_mean_dp(data, frames, p->D, p->dp_partial, p->max_ranges, p->dp_sums, p->dp_mean, s), "mean_dp");
+  mark(0);
+  HIP_TRY(launch_mean_dp_stream(data, frames, p->D, p->dp_partial, p->max_ranges, s), "mean_dp");
+  mark(1);
+  HIP_TRY(launch_mean_dp_reduce(frames, p->D, p->dp_partial, p->max_ranges, p->dp_sums, p->dp_mean, s), "mean_dp");
+  mark(2);
   HIP_TRY(launch_select_bf(p->dp_mean, p->dp_sums, p->det_rows, p->det_cols, threshold, bf_radius, p->max_bf,
                            p->bf_rows, p->bf_cols, p->info, s),
           "select_bf");
+  mark(3);
   p->rotation = rotation_rad;
   rc = ensure_geometry(p, s);
   if (rc) return rc;
   p->gcols = 0;
+  mark(4);
   HIP_TRY(launch_gather_rowfft(p->n, data, p->D, p->det_cols, p->bf_rows, p->bf_cols, p->info, p->nbf_hint, p->tw,
                                p->H, s),
           "gather_rowfft");
+  mark(5);
   const QuadChi m = quad_chi(p, c10, c12, phi12);
-  HIP_TRY(launch_colfft_trotter(p->n, p->H, p->info, p->nbf_hint, geom_ptrs(p), m, p->optics, p->tw, p->S_part,
-                                p->max_groups, nullptr, true, s),
+  HIP_TRY(launch_phasors(p->n, p->info, p->nbf_hint, geom_ptrs(p), m, p->rowph, p->colph, p->echi, s), "phasors");
+  HIP_TRY(launch_trotter_slots(p->n, true, p->H, p->info, geom_ptrs(p), p->rowph, p->colph, p->echi, p->optics, p->tw, p->S_part,
+                               nullptr, true, s),
           "colfft_trotter");
-  HIP_TRY(launch_finish_object(p->n, p->S_part, p->info, p->nbf_hint, p->max_groups, p->tw, p->S,
+  mark(6);
+  HIP_TRY(launch_finish_object(p->n, p->S_part, p->info, p->tw, p->S,
                                reinterpret_cast<float2*>(obj_out), phase_out, s),
           "finish_object");
+  mark(7);
+  return QSSB_OK;
+}
+
+int ssb_stage_count(void) { return NSTAGES; }
+
+const char* ssb_stage_name(int stage) {
+  if (stage < 0 || stage >= NSTAGES) return "";
+  return kStageNames[stage];
+}
+
+static void release_events(ssb_plan* p) {
+  for (hipEvent_t e : p->events) (void)hipEventDestroy(e);
+  p->events.clear();
+  p->timing_max = 0;
+  p->timing_calls = 0;
+}
+
+int ssb_timing_begin(ssb_plan* p, int max_calls) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (max_calls < 1 || max_calls > 100000) return fail(QSSB_ERR_ARG, "max_calls must be in [1, 100000]");
+  release_events(p);
+  p->events.resize((size_t)max_calls * (NSTAGES + 1));
+  for (auto& e : p->events) {
+    hipError_t err = hipEventCreate(&e);
+    if (err != hipSuccess) {
+      e = nullptr;
+      release_events(p);
+      return hip_fail(err, "timing events");
+    }
+  }
+  p->timing_max = max_calls;
+  p->timing_calls = 0;
+  return QSSB_OK;
+}
+
+int ssb_timing_end(ssb_plan* p, float* stage_ms_host, int n_stages, int* n_calls_host) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!stage_ms_host || n_stages < NSTAGES) return fail(QSSB_ERR_ARG, "stage output too small");
+  for (int i = 0; i < NSTAGES; ++i) stage_ms_host[i] = 0.0f;
+  const int calls = p->timing_calls;
+  for (int c = 0; c < calls; ++c) {
+    hipEvent_t* ev = &p->events[(size_t)c * (NSTAGES + 1)];
+    HIP_TRY(hipEventSynchronize(ev[NSTAGES]), "timing sync");
+    for (int i = 0; i < NSTAGES; ++i) {
+      float ms = 0.0f;
+      HIP_TRY(hipEventElapsedTime(&ms, ev[i], ev[i + 1]), "timing elapsed");
+      stage_ms_host[i] += ms;
+    }
+  }
+  if (n_calls_host) *n_calls_host = calls;
+  release_events(p);
   return QSSB_OK;
 }
 

@@ -53,6 +53,11 @@ struct QuadChi {
 
 hipError_t launch_mean_dp(const uint16_t* data, long long frames, int D, uint32_t* partial, int max_ranges,
                           unsigned long long* sums, float* mean, hipStream_t s);
+// the two passes of launch_mean_dp, separately (stage timing)
+hipError_t launch_mean_dp_stream(const uint16_t* data, long long frames, int D, uint32_t* partial, int max_ranges,
+                                 hipStream_t s);
+hipError_t launch_mean_dp_reduce(long long frames, int D, const uint32_t* partial, int max_ranges,
+                                 unsigned long long* sums, float* mean, hipStream_t s);
 hipError_t launch_select_bf(const float* mean, const unsigned long long* sums, int det_rows, int det_cols,
                             float threshold, float bf_radius, int max_bf, int* rows, int* cols, BfInfo* info,
                             hipStream_t s);
@@ -63,23 +68,39 @@ hipError_t launch_gather_rowfft(int N, const uint16_t* data, int D, int det_cols
                                 const int* cols, const BfInfo* info, int nbf_hint, const float2* tw, float2* H,
                                 hipStream_t s);
 
-// Column FFT of H fused with the trotter reduction (object path from raw
-// data).  Writes per-BF-group partial sums S_part[g][c][q]; optionally the
-// transposed Hermitian stack Gt[b][c][q], c = 0..N/2.
-hipError_t launch_colfft_trotter(int N, const float2* H, const BfInfo* info, int nbf_hint, const GeomPtrs& gp,
-                                 QuadChi m, const Optics& o, const float2* tw, float2* S_part, int max_groups,
-                                 float2* Gt, bool do_trotter, hipStream_t s);
-// Trotter reduction from a resident stack Gt[b][c][q] with `gcols` stored
-// columns (N/2+1 Hermitian, or N full).
-hipError_t launch_trotter_from_g(int N, int gcols, const float2* Gt, const BfInfo* info, int nbf_hint,
-                                 const GeomPtrs& gp, QuadChi m, const Optics& o, float2* S_part, int max_groups,
-                                 hipStream_t s);
+// Constants of the isotropic aperture evaluated from d = Rout^2 - |u|^2.
+struct ApIso {
+  float rout2;      // Rout^2 (FP32 copy)
+  float d_in;       // d above which |u| <= inner radius: A == 1
+  float d_out;      // d below which A == 0 (negative margin)
+  float rout;       // Rout
+  float scale;      // lambda / ang
+};
+
+// Number of BF groups (partial planes) of the object-path trotter.
+int object_groups(int N);
+bool optics_isotropic(const Optics& o);
+ApIso iso_aperture(const Optics& o, df& rout2_df);
+
+// Phasor tables rowph[b][q] = e^{i qx[q] a_b}, colph[b][c] = e^{i qy[c] b_b}
+// (c = 0..N/2) and the plane echi[c][q] = e^{i chi(Q)} for the current
+// aberrations.
+hipError_t launch_phasors(int N, const BfInfo* info, int nbf_hint, const GeomPtrs& gp, QuadChi m, float2* rowph,
+                          float2* colph, float2* echi, hipStream_t s);
+// Trotter over Hermitian slot columns: from the row-transformed stack H
+// (fused scan-column FFT; from_h) or from the resident stack Gt[b][c][q],
+// c = 0..N/2.  Writes partial planes S_part[g][c][q] (do_trotter) and, if Gt
+// is given with from_h, the resident stack itself.
+hipError_t launch_trotter_slots(int N, bool from_h, const float2* src, const BfInfo* info, const GeomPtrs& gp,
+                                const float2* rowph, const float2* colph, const float2* echi, const Optics& o,
+                                const float2* tw, float2* S_part, float2* Gt, bool do_trotter, hipStream_t s);
+// Trotter from an imported full-plane (non-Hermitian) stack Gt[b][c][q], c < N.
+hipError_t launch_trotter_full(int N, const float2* Gt, const BfInfo* info, const GeomPtrs& gp, const float2* rowph,
+                               QuadChi m, const Optics& o, float2* S_part, hipStream_t s);
 // Sum the partial planes (Kahan, fixed order), divide by N_bf, set the DC
 // bin, inverse 2D FFT (1/N^2), write object (row-major c64) and phase.
-hipError_t launch_finish_object(int N, const float2* S_part, const BfInfo* info, int nbf_hint, int max_groups,
-                                const float2* tw, float2* S, float2* obj, float* phase, hipStream_t s);
+hipError_t launch_finish_object(int N, const float2* S_part, const BfInfo* info, const float2* tw, float2* S,
+                                float2* obj, float* phase, hipStream_t s);
 
-int group_count(int N, int nbf);
-int colfft_group_size(int N);
 
 }  // namespace qssb

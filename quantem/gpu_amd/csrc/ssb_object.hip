@@ -1,6 +1,7 @@
-// Object-path kernels of the MI355X SSB engine: the scan-column FFT fused
-// with the trotter reduction over bright-field pixels, the same reduction
-// from a resident Fourier stack, and the final inverse FFT to the phase.
+// Object-path kernels of the MI355X SSB engine: the trotter reduction over
+// bright-field pixels (fused with the scan-column FFT when it starts from the
+// row-transformed stack H, or reading a resident Fourier stack), and the
+// final inverse FFT to the object and its phase.
 //
 // Reference behaviour (paths relative to bobleesj/quantem.gpu):
 //   Fourier-sum trotter   src/quantem/gpu/ssb/compute/cuda/kernels/fft256.py:967-1030
@@ -11,146 +12,293 @@
 
 namespace qssb {
 
-// BF pixels per partial plane of the trotter reduction.
-constexpr int BG = 16;
+int object_groups(int N) { return N == 128 ? 64 : (N == 256 ? 32 : 8); }
 
-int group_count(int N, int nbf) {
-  (void)N;
-  return (nbf + BG - 1) / BG;
+// Balanced split of [0, nbf) into `groups` contiguous ranges.
+__device__ __forceinline__ void group_range(int g, int groups, int nbf, int& b0, int& b1) {
+  b0 = (int)(((long long)g * nbf) / groups);
+  b1 = (int)(((long long)(g + 1) * nbf) / groups);
 }
 
 __device__ __forceinline__ float chi_quad(const QuadChi& m, float x, float y) {
   return m.mxx * x * x + 2.0f * m.mxy * x * y + m.myy * y * y;
 }
 
+// Aperture from d = Rout^2 - |u|^2 (double-float), where Rout is the radius
+// at which the soft edge reaches 0.  For isotropic detector sampling the
+// reference edge (semi - lambda|u|)/ang + 1/2 equals (Rout - |u|)*lambda/ang
+// = d * (lambda/ang) / (Rout + |u|): the cancellation sits entirely in d,
+// which is exact to ~1e-13 here, so ties (d == 0, A(Q-k) == A(Q+k)) resolve
+// as exact arithmetic resolves them.
+__device__ __forceinline__ float aperture_from_d(df d, const ApIso& a) {
+  const float dh = d.hi + d.lo;
+  if (dh >= a.d_in) return 1.0f;
+  if (dh <= a.d_out) return 0.0f;
+  const float r = sqrtf(fmaxf(a.rout2 - dh, 0.0f));
+  const float e = dh * a.scale / (a.rout + r);
+  return fminf(fmaxf(e, 0.0f), 1.0f);
+}
+
+// Phasor tables of the factorised overlap e^{i theta} = e^{i qx a_b} e^{i qy b_b},
+// (a_b, b_b) = 2 M k_b: rowph[b][q] for all rows, colph[b][c] for columns
+// c = 0..N/2 (the other columns are their conjugates); and the plane
+// echi[c][q] = e^{i chi(Q)} (column-major, like the partial sums).
+__global__ void __launch_bounds__(256)
+k_phasors(const BfInfo* __restrict__ info, GeomPtrs gp, QuadChi m, int N, float2* __restrict__ rowph,
+          float2* __restrict__ colph, float2* __restrict__ echi) {
+  const int nbf = info->nbf;
+  const int ncol = N / 2 + 1;
+  const long long per_b = (long long)N + ncol;
+  const long long total = (long long)nbf * per_b;
+  const long long NN = (long long)N * N;
+  for (long long idx = (long long)blockIdx.x * 256 + threadIdx.x; idx < NN; idx += (long long)gridDim.x * 256) {
+    const int c = (int)(idx / N), q = (int)(idx - (long long)c * N);
+    float sn, cs;
+    sincosf(chi_quad(m, gp.qx[q], gp.qy[c]), &sn, &cs);
+    echi[idx] = make_float2(cs, sn);
+  }
+  for (long long idx = (long long)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (long long)gridDim.x * 256) {
+    const int b = (int)(idx / per_b);
+    const int r = (int)(idx - (long long)b * per_b);
+    const float kx = gp.kx[b], ky = gp.ky[b];
+    float sn, cs;
+    if (r < N) {
+      const float ta = 2.0f * (m.mxx * kx + m.mxy * ky);
+      sincosf(gp.qx[r] * ta, &sn, &cs);
+      rowph[(long long)b * N + r] = make_float2(cs, sn);
+    } else {
+      const int c = r - N;
+      const float tb = 2.0f * (m.mxy * kx + m.myy * ky);
+      sincosf(gp.qy[c] * tb, &sn, &cs);
+      colph[(long long)b * ncol + c] = make_float2(cs, sn);
+    }
+  }
+}
+
+// Aperture pair A(Q - k), A(Q + k).  ISO: from d = (Rout^2 - |k|^2) - |Q|^2
+// +/- 2 Q.k; otherwise the general double-float soft-edge expression.
+template <bool ISO>
+__device__ __forceinline__ void aperture_pair(df qx, df qy, df kx, df ky, df cb, df q2, const ApIso& ap,
+                                              const Optics& o, float& am, float& app) {
+  if constexpr (ISO) {
+    const df qk2 = df_mul(df_add(df_mul(qx, kx), df_mul(qy, ky)), df{2.0f, 0.0f});
+    const df base = df_sub(cb, q2);
+    am = aperture_from_d(df_add(base, qk2), ap);
+    app = aperture_from_d(df_sub(base, qk2), ap);
+  } else {
+    am = aperture_at<false>(qx.hi, qx.lo, qy.hi, qy.lo, kx.hi, kx.lo, ky.hi, ky.lo, o);
+    app = aperture_at<true>(qx.hi, qx.lo, qy.hi, qy.lo, kx.hi, kx.lo, ky.hi, ky.lo, o);
+  }
+}
+
+
 // ------------------------------------------------------------------------
-// Scan-column FFT of H + trotter.  Work item = (BF group g, slot group sg).
-// For each BF pixel of the group the SG slot columns of H[b] (N rows each)
-// are staged in LDS, transformed along R_r, and every resulting G_b(q, j)
-// contributes to S(q, j) and, through Hermitian symmetry, to
-// S(-q, N-j) = sum_b conj(G_b(q, j)) conj(Gamma^_b(-q, N-j)).  Slot 0 packs
-// the two real columns 0 and N/2 and is separated after the FFT.
-// BLOCK is a multiple of N, so each thread keeps one scan frequency row.
+// Trotter over Hermitian slot columns.  Work item = (BF group g, slot group
+// sg of SG slots).  Per BF pixel the slot columns G_b(:, j) are staged in
+// LDS -- from H through the scan-column FFT (FROM_H) or copied from the
+// resident stack Gt[b][c][q] -- and each G_b(q, j) contributes to
+//   output A = S(q, j)          with G_b(q, j), and
+//   output B = S(-q, N-j)       with conj(G_b(q, j))        (j >= 1),
+// or, for slot 0, A = S(q, 0) with G_b(q, 0) and B = S(q, N/2) with
+// G_b(q, N/2) (slot 0 of H packs the two real columns 0 and N/2).
+// For j >= 1 and q != N/2, Q_B = -Q_A, so chi(Q_B) = chi(Q_A),
+// theta_B = -theta_A, A(Q_B -/+ k) = A(Q_A +/- k): one aperture pair and one
+// phasor serve both outputs.  Points whose two apertures vanish contribute
+// exactly zero and are skipped.
+// BLOCK is a multiple of N: each thread keeps one scan-frequency row q.
 // ------------------------------------------------------------------------
-template <int N, int SG, int BLOCK>
+template <int N, int SG, int BLOCK, bool FROM_H, bool ISO>
 __global__ void __launch_bounds__(BLOCK)
-k_colfft_trotter(const float2* __restrict__ H, const BfInfo* __restrict__ info, GeomPtrs gp, QuadChi m,
-                 Optics o, const float2* __restrict__ tw_g, float2* __restrict__ S_part,
-                 float2* __restrict__ Gt, int do_trotter) {
+k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info, GeomPtrs gp,
+                const float2* __restrict__ rowph_g, const float2* __restrict__ colph_g,
+                const float2* __restrict__ echi_g, ApIso ap, df rout2, Optics o,
+                const float2* __restrict__ tw_g, int groups, float2* __restrict__ S_part, float2* __restrict__ Gt,
+                int do_trotter) {
   constexpr int STRIDE = N + 1;
   constexpr int HALF = N / 2;
   constexpr int NSG = HALF / SG;
-  constexpr int P = SG * N / BLOCK;       // points per thread
-  constexpr int SSTEP = BLOCK / N;        // slot step between a thread's points
+  constexpr int P = SG * N / BLOCK;      // points per thread
+  constexpr int SSTEP = BLOCK / N;       // slot step between a thread's points
   constexpr int GCOLS = HALF + 1;
+  constexpr int LOADS = (N * SG) / BLOCK;
   __shared__ float2 tile[SG * STRIDE];
   __shared__ float2 tw[N];
-  __shared__ float2 colph[2 * SG];
-  __shared__ float qyS[2 * SG];
-  __shared__ float qySl[2 * SG];
+  __shared__ float2 colph[SG + 1];       // e^{i qy[j] b_b} per slot, plus column N/2
+  __shared__ float qyS[SG + 1], qySl[SG + 1];
+  // output-B constants where Q_B != -Q_A: column N/2 (slot 0) per row, and
+  // the Nyquist row q = N/2 per slot
+  __shared__ float2 ehalfc[N];
+  __shared__ df q2halfc[N];
+  __shared__ float2 enyqr[SG];
 
   const int tid = threadIdx.x;
-  for (int i = tid; i < N; i += BLOCK) tw[i] = tw_g[i];
+  if (FROM_H)
+    for (int i = tid; i < N; i += BLOCK) tw[i] = tw_g[i];
   const int nbf = info->nbf;
-  const float2 dc = make_float2(info->dc_re, info->dc_im);
-  (void)dc;
-  const int ngroups = (nbf + BG - 1) / BG;
-  const long long items = (long long)ngroups * NSG;
+  const long long items = (long long)groups * NSG;
   const int q = tid % N;
   const int s0 = tid / N;
   const int qm = (N - q) & (N - 1);
-  const float qxA = gp.qx[q], qxAl = gp.qxl[q];
-  const float qxBm = gp.qx[qm], qxBml = gp.qxl[qm];
+  const df qxA{gp.qx[q], gp.qxl[q]};
+  const df qxB0{gp.qx[qm], gp.qxl[qm]};   // row of output B when j >= 1
 
   for (long long item = blockIdx.x; item < items; item += gridDim.x) {
     const int g = (int)(item / NSG);
     const int sg = (int)(item - (long long)g * NSG);
-    const int b0 = g * BG;
-    const int b1 = min(nbf, b0 + BG);
+    int b0, b1;
+    group_range(g, groups, nbf, b0, b1);
     __syncthreads();
-    if (tid < 2 * SG) {
-      const int s = tid % SG;
-      const int j = sg * SG + s;
-      int col;
-      if (tid < SG) col = j;
-      else col = (j == 0) ? HALF : N - j;
+    if (tid <= SG) {
+      const int col = (tid < SG) ? sg * SG + tid : HALF;
       qyS[tid] = gp.qy[col];
       qySl[tid] = gp.qyl[col];
     }
+    if (do_trotter) {
+      if (sg == 0 && tid < N) {
+        const df qx{gp.qx[tid], gp.qxl[tid]};
+        const df qh{gp.qy[HALF], gp.qyl[HALF]};   // not qyS: written in this same phase
+        q2halfc[tid] = df_add(df_mul(qx, qx), df_mul(qh, qh));
+        ehalfc[tid] = echi_g[(long long)HALF * N + tid];
+      }
+      if (tid < SG) {
+        const int j = sg * SG + tid;
+        enyqr[tid] = echi_g[(long long)((N - j) & (N - 1)) * N + HALF];
+      }
+    }
     __syncthreads();
-    // per-output constants: e^{i chi(Q)} for outputs A and B of each point
-    float2 echiA[P], echiB[P];
+    // per-point constants
+    float2 echiA[P];
+    df q2A[P];
     Kahan arA[P], aiA[P], arB[P], aiB[P];
 #pragma unroll
     for (int i = 0; i < P; ++i) {
       const int s = s0 + i * SSTEP;
-      const int j = sg * SG + s;
-      const float qxB = (j == 0) ? qxA : qxBm;
-      float sn, cs;
-      sincosf(chi_quad(m, qxA, qyS[s]), &sn, &cs);
-      echiA[i] = make_float2(cs, sn);
-      sincosf(chi_quad(m, qxB, qyS[SG + s]), &sn, &cs);
-      echiB[i] = make_float2(cs, sn);
+      const df qy{qyS[s], qySl[s]};
+      q2A[i] = df_add(df_mul(qxA, qxA), df_mul(qy, qy));
+      echiA[i] = echi_g[(long long)(sg * SG + s) * N + q];
       arA[i].init();
       aiA[i].init();
       arB[i].init();
       aiB[i].init();
     }
+
     for (int b = b0; b < b1; ++b) {
       __syncthreads();
-      const float2* hb = H + (long long)b * N * HALF + sg * SG;
-      for (int idx = tid; idx < N * SG; idx += BLOCK) {
-        const int s = idx % SG;
-        const int r = idx / SG;
-        tile[s * STRIDE + r] = hb[(long long)r * HALF + s];
+#pragma unroll
+      for (int l = 0; l < LOADS; ++l) {
+        const int idx = tid + l * BLOCK;
+        if (FROM_H) {
+          const int s = idx % SG, r = idx / SG;
+          tile[s * STRIDE + r] = src[((long long)b * N + r) * HALF + sg * SG + s];
+        } else {
+          const int s = idx / N, r = idx % N;
+          tile[s * STRIDE + r] = src[((long long)b * GCOLS + sg * SG + s) * N + r];
+        }
       }
-      const Coord kc{gp.kx[b], gp.kxl[b], gp.ky[b], gp.kyl[b]};
-      const float kx = kc.x, ky = kc.y, ak = gp.ak[b];
-      const float ta = 2.0f * (m.mxx * kx + m.mxy * ky);
-      const float tb = 2.0f * (m.mxy * kx + m.myy * ky);
-      if (do_trotter && tid < 2 * SG) {
-        float sn, cs;
-        sincosf(qyS[tid] * tb, &sn, &cs);
-        colph[tid] = make_float2(cs, sn);
+      const float kxh = gp.kx[b], kyh = gp.ky[b];
+      const df kx{kxh, gp.kxl[b]}, ky{kyh, gp.kyl[b]};
+      const float ak = gp.ak[b];
+      if (do_trotter && tid <= SG) {
+        const int col = (tid < SG) ? sg * SG + tid : HALF;
+        colph[tid] = colph_g[(long long)b * GCOLS + col];
       }
       __syncthreads();
-      fft_lds<N, SG, BLOCK, STRIDE, false>(tile, tw);
-      float2 rowA = make_float2(1.f, 0.f), rowBc = rowA;
-      if (do_trotter) {
-        float sn, cs;
-        sincosf(qxA * ta, &sn, &cs);
-        rowA = make_float2(cs, sn);
-        rowBc = (q == HALF || q == 0) ? rowA : cconj(rowA);
+      if (FROM_H) fft_lds<N, SG, BLOCK, STRIDE, false>(tile, tw);
+      // Gt[b] column N/2 (needed only for the resident, non-FFT source)
+      float2 gHalf = make_float2(0.f, 0.f);
+      if (!FROM_H && sg == 0) gHalf = src[((long long)b * GCOLS + HALF) * N + q];
+      if (!do_trotter) {
+        if (Gt) {
+          float2* gb = Gt + (long long)b * GCOLS * N;
+#pragma unroll
+          for (int i = 0; i < P; ++i) {
+            const int s = s0 + i * SSTEP;
+            const int j = sg * SG + s;
+            const float2 z = tile[s * STRIDE + q];
+            if (j == 0) {
+              const float2 zm = tile[s * STRIDE + qm];
+              gb[q] = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
+              gb[(long long)HALF * N + q] = make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x));
+            } else {
+              gb[(long long)j * N + q] = z;
+            }
+          }
+        }
+        continue;
       }
+      const df kk = df_add(df_mul(kx, kx), df_mul(ky, ky));
+      const df cb = df_sub(rout2, kk);           // Rout^2 - |k|^2
+      const float2 rowA = rowph_g[(long long)b * N + q];
 #pragma unroll
       for (int i = 0; i < P; ++i) {
         const int s = s0 + i * SSTEP;
         const int j = sg * SG + s;
-        float2 GA, GB;
         const float2 z = tile[s * STRIDE + q];
+        float2 GA, GB;
         if (j == 0) {
-          const float2 zm = tile[s * STRIDE + qm];
-          GA = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
-          GB = make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x));
+          if (FROM_H) {
+            const float2 zm = tile[s * STRIDE + qm];
+            GA = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
+            GB = make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x));
+          } else {
+            GA = z;
+            GB = gHalf;
+          }
         } else {
           GA = z;
           GB = cconj(z);
+        }
+        const df qy{qyS[s], qySl[s]};
+        float am, app;
+        aperture_pair<ISO>(qxA, qy, kx, ky, cb, q2A[i], ap, o, am, app);
+        const float2 eA = cmul(rowA, colph[s]);
+        const bool mirror = (j != 0) && (q != HALF);
+        float bm, bp;
+        float2 eB, echiB;
+        if (mirror) {
+          bm = app;
+          bp = am;
+          eB = cconj(eA);
+          echiB = echiA[i];
+        } else {
+          // j == 0: row q, column N/2; q == N/2: row N/2, column N-j
+          const df qyB = (j == 0) ? df{qyS[SG], qySl[SG]} : df{-qy.hi, -qy.lo};
+          const df qxB = (j == 0) ? qxA : qxB0;
+          const df q2B = (j == 0) ? q2halfc[q] : q2A[i];
+          echiB = (j == 0) ? ehalfc[q] : enyqr[s];
+          aperture_pair<ISO>(qxB, qyB, kx, ky, cb, q2B, ap, o, bm, bp);
+          const float2 rowB = (j == 0) ? rowA : rowph_g[(long long)b * N + qm];
+          const float2 colB = (j == 0) ? colph[SG] : cconj(colph[s]);
+          eB = cmul(rowB, colB);
+        }
+        if (am != 0.0f || app != 0.0f) {
+          const float wr = (am - app) * echiA[i].x;
+          const float wi = -(am + app) * echiA[i].y;
+          float gr = ak * (eA.x * wr - eA.y * wi);
+          float gi = ak * (eA.x * wi + eA.y * wr);
+          const float mag2 = gr * gr + gi * gi;
+          const float sc = (mag2 > 1e-16f) ? rsqrtf(mag2) : 1e8f;
+          gr *= sc;
+          gi *= sc;
+          arA[i].add(GA.x * gr + GA.y * gi);
+          aiA[i].add(GA.y * gr - GA.x * gi);
+        }
+        if (bm != 0.0f || bp != 0.0f) {
+          const float wr = (bm - bp) * echiB.x;
+          const float wi = -(bm + bp) * echiB.y;
+          float gr = ak * (eB.x * wr - eB.y * wi);
+          float gi = ak * (eB.x * wi + eB.y * wr);
+          const float mag2 = gr * gr + gi * gi;
+          const float sc = (mag2 > 1e-16f) ? rsqrtf(mag2) : 1e8f;
+          gr *= sc;
+          gi *= sc;
+          arB[i].add(GB.x * gr + GB.y * gi);
+          aiB[i].add(GB.y * gr - GB.x * gi);
         }
         if (Gt) {
           float2* gb = Gt + (long long)b * GCOLS * N;
           gb[(long long)j * N + q] = GA;
           if (j == 0) gb[(long long)HALF * N + q] = GB;
-        }
-        if (do_trotter) {
-          const float2 rowB = (j == 0) ? rowA : rowBc;
-          const Coord qa{qxA, qxAl, qyS[s], qySl[s]};
-          const Coord qb{(j == 0) ? qxA : qxBm, (j == 0) ? qxAl : qxBml, qyS[SG + s], qySl[SG + s]};
-          const float2 tA = trotter_term(GA, qa, kc, ak, cmul(rowA, colph[s]), echiA[i], o);
-          const float2 tB = trotter_term(GB, qb, kc, ak, cmul(rowB, colph[SG + s]), echiB[i], o);
-          arA[i].add(tA.x);
-          aiA[i].add(tA.y);
-          arB[i].add(tB.x);
-          aiB[i].add(tB.y);
         }
       }
     }
@@ -160,10 +308,9 @@ k_colfft_trotter(const float2* __restrict__ H, const BfInfo* __restrict__ info, 
       for (int i = 0; i < P; ++i) {
         const int s = s0 + i * SSTEP;
         const int j = sg * SG + s;
-        const int colA = j;
         const int colB = (j == 0) ? HALF : N - j;
         const int rowB = (j == 0) ? q : qm;
-        sp[(long long)colA * N + q] = make_float2(arA[i].s - arA[i].c, aiA[i].s - aiA[i].c);
+        sp[(long long)j * N + q] = make_float2(arA[i].s - arA[i].c, aiA[i].s - aiA[i].c);
         sp[(long long)colB * N + rowB] = make_float2(arB[i].s - arB[i].c, aiB[i].s - aiB[i].c);
       }
     }
@@ -171,34 +318,25 @@ k_colfft_trotter(const float2* __restrict__ H, const BfInfo* __restrict__ info, 
 }
 
 // ------------------------------------------------------------------------
-// Trotter reduction from a resident stack Gt[b][c][q] (gcols columns).
-// Work item = (BF group g, column group of CG full-plane columns).
+// Trotter from a full (non-Hermitian) stack Gt[b][c][q], c < N: each full
+// column is its own output (test seam for imported full-plane stacks).
 // ------------------------------------------------------------------------
 template <int N, int CG, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
-k_trotter_from_g(const float2* __restrict__ Gt, int gcols, const BfInfo* __restrict__ info, GeomPtrs gp,
-                 QuadChi m, Optics o, float2* __restrict__ S_part) {
+k_trotter_full(const float2* __restrict__ Gt, const BfInfo* __restrict__ info, GeomPtrs gp,
+               const float2* __restrict__ rowph_g, QuadChi m, Optics o, int groups, float2* __restrict__ S_part) {
   constexpr int NCG = N / CG;
   constexpr int P = CG * N / BLOCK;
-  __shared__ float2 rowph[N];
   __shared__ float2 colph[CG];
-  __shared__ float qxs[N];
-  __shared__ float qxsl[N];
-  __shared__ float qyS[CG];
-  __shared__ float qySl[CG];
+  __shared__ float qyS[CG], qySl[CG];
   const int tid = threadIdx.x;
-  for (int i = tid; i < N; i += BLOCK) {
-    qxs[i] = gp.qx[i];
-    qxsl[i] = gp.qxl[i];
-  }
   const int nbf = info->nbf;
-  const int ngroups = (nbf + BG - 1) / BG;
-  const long long items = (long long)ngroups * NCG;
+  const long long items = (long long)groups * NCG;
   for (long long item = blockIdx.x; item < items; item += gridDim.x) {
     const int g = (int)(item / NCG);
     const int cg = (int)(item - (long long)g * NCG);
-    const int b0 = g * BG;
-    const int b1 = min(nbf, b0 + BG);
+    int b0, b1;
+    group_range(g, groups, nbf, b0, b1);
     __syncthreads();
     if (tid < CG) {
       qyS[tid] = gp.qy[cg * CG + tid];
@@ -212,42 +350,31 @@ k_trotter_from_g(const float2* __restrict__ Gt, int gcols, const BfInfo* __restr
       const int pidx = tid + i * BLOCK;
       const int row = pidx % N, cl = pidx / N;
       float sn, cs;
-      sincosf(chi_quad(m, qxs[row], qyS[cl]), &sn, &cs);
+      sincosf(chi_quad(m, gp.qx[row], qyS[cl]), &sn, &cs);
       echi[i] = make_float2(cs, sn);
       ar[i].init();
       ai[i].init();
     }
     for (int b = b0; b < b1; ++b) {
       const Coord kc{gp.kx[b], gp.kxl[b], gp.ky[b], gp.kyl[b]};
-      const float kx = kc.x, ky = kc.y, ak = gp.ak[b];
-      const float ta = 2.0f * (m.mxx * kx + m.mxy * ky);
-      const float tb = 2.0f * (m.mxy * kx + m.myy * ky);
+      const float tb = 2.0f * (m.mxy * kc.x + m.myy * kc.y);
       __syncthreads();
-      for (int r = tid; r < N; r += BLOCK) {
-        float sn, cs;
-        sincosf(qxs[r] * ta, &sn, &cs);
-        rowph[r] = make_float2(cs, sn);
-      }
       if (tid < CG) {
         float sn, cs;
         sincosf(qyS[tid] * tb, &sn, &cs);
         colph[tid] = make_float2(cs, sn);
       }
       __syncthreads();
-      const float2* gb = Gt + (long long)b * gcols * N;
+      const float2* gb = Gt + (long long)b * N * N;
 #pragma unroll
       for (int i = 0; i < P; ++i) {
         const int pidx = tid + i * BLOCK;
         const int row = pidx % N, cl = pidx / N;
         const int col = cg * CG + cl;
-        float2 G;
-        if (col < gcols) {
-          G = gb[(long long)col * N + row];
-        } else {
-          G = cconj(gb[(long long)(N - col) * N + ((N - row) & (N - 1))]);
-        }
-        const Coord qc{qxs[row], qxsl[row], qyS[cl], qySl[cl]};
-        const float2 t = trotter_term(G, qc, kc, ak, cmul(rowph[row], colph[cl]), echi[i], o);
+        const float2 G = gb[(long long)col * N + row];
+        const Coord qc{gp.qx[row], gp.qxl[row], qyS[cl], qySl[cl]};
+        const float2 eth = cmul(rowph_g[(long long)b * N + row], colph[cl]);
+        const float2 t = trotter_term(G, qc, kc, gp.ak[b], eth, echi[i], o);
         ar[i].add(t.x);
         ai[i].add(t.y);
       }
@@ -266,16 +393,15 @@ k_trotter_from_g(const float2* __restrict__ Gt, int gcols, const BfInfo* __restr
 // Finish: reduce partial planes, inverse FFT, angle.
 // ------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
-k_reduce_partials(const float2* __restrict__ S_part, const BfInfo* __restrict__ info, int NN,
+k_reduce_partials(const float2* __restrict__ S_part, const BfInfo* __restrict__ info, int NN, int groups,
                   float2* __restrict__ S) {
   const int idx = blockIdx.x * 256 + threadIdx.x;
   if (idx >= NN) return;
   const int nbf = info->nbf;
-  const int ngroups = (nbf + BG - 1) / BG;
   Kahan kr, ki;
   kr.init();
   ki.init();
-  for (int g = 0; g < ngroups; ++g) {
+  for (int g = 0; g < groups; ++g) {
     const float2 v = S_part[(long long)g * NN + idx];
     kr.add(v.x);
     ki.add(v.y);
@@ -348,26 +474,89 @@ static unsigned grid_items(long long items, long long cap) {
   return (unsigned)items;
 }
 
-int colfft_group_size(int N) { return (N == 512) ? 8 : 16; }
+ApIso iso_aperture(const Optics& o, df& rout2_df) {
+  // Rout = (semi + ang/2) / lambda; the isotropic form needs ang_y == ang_x
+  const double lam = (double)o.lam.hi + (double)o.lam.lo;
+  const double semi = (double)o.semi.hi + (double)o.semi.lo;
+  const double ang = (double)o.angy.hi + (double)o.angy.lo;
+  const double rout = (semi + 0.5 * ang) / lam;
+  const double rin = (semi - 0.5 * ang) / lam;
+  const double r2 = rout * rout;
+  const float hi = (float)r2;
+  rout2_df = df{hi, (float)(r2 - (double)hi)};
+  ApIso a;
+  a.rout2 = hi;
+  a.rout = (float)rout;
+  a.scale = (float)(lam / ang);
+  // exact 0/1 outside the band, with a relative margin above FP32 noise
+  a.d_in = (rin > 0.0) ? (float)((r2 - rin * rin) * (1.0 + 1e-5)) : INFINITY;
+  a.d_out = (float)(-r2 * 1e-5);
+  return a;
+}
 
-hipError_t launch_colfft_trotter(int N, const float2* H, const BfInfo* info, int nbf_hint, const GeomPtrs& gp,
-                                 QuadChi m, const Optics& o, const float2* tw, float2* S_part, int max_groups,
-                                 float2* Gt, bool do_trotter, hipStream_t s) {
-  (void)max_groups;
-  const long long ng = (nbf_hint + BG - 1) / BG;
+bool optics_isotropic(const Optics& o) { return o.angy.hi == o.angx.hi && o.angy.lo == o.angx.lo; }
+
+hipError_t launch_phasors(int N, const BfInfo* info, int nbf_hint, const GeomPtrs& gp, QuadChi m, float2* rowph,
+                          float2* colph, float2* echi, hipStream_t s) {
+  long long total = (long long)(nbf_hint > 0 ? nbf_hint : 1) * (N + N / 2 + 1);
+  if (total < (long long)N * N) total = (long long)N * N;
+  hipLaunchKernelGGL(k_phasors, dim3(grid_items((total + 255) / 256, 4096)), dim3(256), 0, s, info, gp, m, N, rowph,
+                     colph, echi);
+  return hipGetLastError();
+}
+
+hipError_t launch_trotter_slots(int N, bool from_h, const float2* src, const BfInfo* info, const GeomPtrs& gp,
+                                const float2* rowph, const float2* colph, const float2* echi, const Optics& o,
+                                const float2* tw, float2* S_part, float2* Gt, bool do_trotter, hipStream_t s) {
+  df rout2;
+  const ApIso ap = iso_aperture(o, rout2);
+  const int groups = object_groups(N);
   const int dt = do_trotter ? 1 : 0;
+  const bool iso = optics_isotropic(o);
+#define QSSB_SLOTS_ONE(NN, SGV, BL, FH, IS)                                                                 \
+  hipLaunchKernelGGL((k_trotter_slots<NN, SGV, BL, FH, IS>), dim3(grid), dim3(BL), 0, s, src, info, gp, rowph, \
+                     colph, echi, ap, rout2, o, tw, groups, S_part, Gt, dt)
+#define QSSB_SLOTS(NN, SGV, BL)                                                                \
+  do {                                                                                        \
+    const unsigned grid = grid_items((long long)groups * ((NN / 2) / SGV), 65536);            \
+    if (from_h && iso) QSSB_SLOTS_ONE(NN, SGV, BL, true, true);                               \
+    else if (from_h) QSSB_SLOTS_ONE(NN, SGV, BL, true, false);                                \
+    else if (iso) QSSB_SLOTS_ONE(NN, SGV, BL, false, true);                                   \
+    else QSSB_SLOTS_ONE(NN, SGV, BL, false, false);                                           \
+  } while (0)
   switch (N) {
     case 128:
-      hipLaunchKernelGGL((k_colfft_trotter<128, 16, 512>), dim3(grid_items(ng * 4, 8192)), dim3(512), 0, s, H,
-                         info, gp, m, o, tw, S_part, Gt, dt);
+      QSSB_SLOTS(128, 16, 1024);
       break;
     case 256:
-      hipLaunchKernelGGL((k_colfft_trotter<256, 16, 512>), dim3(grid_items(ng * 8, 8192)), dim3(512), 0, s, H,
-                         info, gp, m, o, tw, S_part, Gt, dt);
+      QSSB_SLOTS(256, 16, 1024);
       break;
     case 512:
-      hipLaunchKernelGGL((k_colfft_trotter<512, 8, 512>), dim3(grid_items(ng * 32, 8192)), dim3(512), 0, s, H,
-                         info, gp, m, o, tw, S_part, Gt, dt);
+      QSSB_SLOTS(512, 8, 1024);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef QSSB_SLOTS
+#undef QSSB_SLOTS_ONE
+  return hipGetLastError();
+}
+
+hipError_t launch_trotter_full(int N, const float2* Gt, const BfInfo* info, const GeomPtrs& gp, const float2* rowph,
+                               QuadChi m, const Optics& o, float2* S_part, hipStream_t s) {
+  const int groups = object_groups(N);
+  switch (N) {
+    case 128:
+      hipLaunchKernelGGL((k_trotter_full<128, 16, 256>), dim3(grid_items(groups * 8LL, 65536)), dim3(256), 0, s, Gt,
+                         info, gp, rowph, m, o, groups, S_part);
+      break;
+    case 256:
+      hipLaunchKernelGGL((k_trotter_full<256, 8, 256>), dim3(grid_items(groups * 32LL, 65536)), dim3(256), 0, s, Gt,
+                         info, gp, rowph, m, o, groups, S_part);
+      break;
+    case 512:
+      hipLaunchKernelGGL((k_trotter_full<512, 4, 256>), dim3(grid_items(groups * 128LL, 65536)), dim3(256), 0, s,
+                         Gt, info, gp, rowph, m, o, groups, S_part);
       break;
     default:
       return hipErrorInvalidValue;
@@ -375,36 +564,11 @@ hipError_t launch_colfft_trotter(int N, const float2* H, const BfInfo* info, int
   return hipGetLastError();
 }
 
-hipError_t launch_trotter_from_g(int N, int gcols, const float2* Gt, const BfInfo* info, int nbf_hint,
-                                 const GeomPtrs& gp, QuadChi m, const Optics& o, float2* S_part, int max_groups,
-                                 hipStream_t s) {
-  (void)max_groups;
-  const long long ng = (nbf_hint + BG - 1) / BG;
-  switch (N) {
-    case 128:
-      hipLaunchKernelGGL((k_trotter_from_g<128, 16, 256>), dim3(grid_items(ng * 8, 8192)), dim3(256), 0, s, Gt,
-                         gcols, info, gp, m, o, S_part);
-      break;
-    case 256:
-      hipLaunchKernelGGL((k_trotter_from_g<256, 8, 256>), dim3(grid_items(ng * 32, 8192)), dim3(256), 0, s, Gt,
-                         gcols, info, gp, m, o, S_part);
-      break;
-    case 512:
-      hipLaunchKernelGGL((k_trotter_from_g<512, 4, 256>), dim3(grid_items(ng * 128, 8192)), dim3(256), 0, s, Gt,
-                         gcols, info, gp, m, o, S_part);
-      break;
-    default:
-      return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
-}
-
-hipError_t launch_finish_object(int N, const float2* S_part, const BfInfo* info, int nbf_hint, int max_groups,
-                                const float2* tw, float2* S, float2* obj, float* phase, hipStream_t s) {
-  (void)nbf_hint;
-  (void)max_groups;
+hipError_t launch_finish_object(int N, const float2* S_part, const BfInfo* info, const float2* tw, float2* S,
+                                float2* obj, float* phase, hipStream_t s) {
   const int NN = N * N;
-  hipLaunchKernelGGL(k_reduce_partials, dim3((NN + 255) / 256), dim3(256), 0, s, S_part, info, NN, S);
+  hipLaunchKernelGGL(k_reduce_partials, dim3((NN + 255) / 256), dim3(256), 0, s, S_part, info, NN,
+                     object_groups(N), S);
   switch (N) {
     case 128:
       hipLaunchKernelGGL((k_ifft_cols<128, 8>), dim3(128 / 8), dim3(256), 0, s, S, tw);

@@ -420,13 +420,18 @@ k_gather_rowfft(const uint16_t* __restrict__ data, int D, int det_cols,
 // ------------------------------------------------------------------------
 // Launch wrappers
 // ------------------------------------------------------------------------
-hipError_t launch_mean_dp(const uint16_t* data, long long frames, int D, uint32_t* partial,
-                          int max_ranges, unsigned long long* sums, float* mean, hipStream_t s) {
-  int ranges = (int)((frames + 255) / 256);
+static void range_split(long long frames, int max_ranges, int& ranges, int& fpr) {
+  ranges = (int)((frames + 255) / 256);
   if (ranges > max_ranges) ranges = max_ranges;
   if (ranges < 1) ranges = 1;
-  int fpr = (int)((frames + ranges - 1) / ranges);
+  fpr = (int)((frames + ranges - 1) / ranges);
   ranges = (int)((frames + fpr - 1) / fpr);
+}
+
+hipError_t launch_mean_dp_stream(const uint16_t* data, long long frames, int D, uint32_t* partial, int max_ranges,
+                                 hipStream_t s) {
+  int ranges, fpr;
+  range_split(frames, max_ranges, ranges, fpr);
   const bool vec = (D % 8 == 0) && ((reinterpret_cast<uintptr_t>(data) & 15u) == 0);
   if (vec) {
     dim3 grid((D / 8 + 255) / 256, ranges);
@@ -435,9 +440,22 @@ hipError_t launch_mean_dp(const uint16_t* data, long long frames, int D, uint32_
     dim3 grid((D + 255) / 256, ranges);
     hipLaunchKernelGGL(k_dp_partial<1>, grid, dim3(256), 0, s, data, frames, D, fpr, partial);
   }
-  hipLaunchKernelGGL(k_dp_reduce, dim3((D + 255) / 256), dim3(256), 0, s, partial, ranges, D, frames,
-                     sums, mean);
   return hipGetLastError();
+}
+
+hipError_t launch_mean_dp_reduce(long long frames, int D, const uint32_t* partial, int max_ranges,
+                                 unsigned long long* sums, float* mean, hipStream_t s) {
+  int ranges, fpr;
+  range_split(frames, max_ranges, ranges, fpr);
+  hipLaunchKernelGGL(k_dp_reduce, dim3((D + 255) / 256), dim3(256), 0, s, partial, ranges, D, frames, sums, mean);
+  return hipGetLastError();
+}
+
+hipError_t launch_mean_dp(const uint16_t* data, long long frames, int D, uint32_t* partial, int max_ranges,
+                          unsigned long long* sums, float* mean, hipStream_t s) {
+  hipError_t e = launch_mean_dp_stream(data, frames, D, partial, max_ranges, s);
+  if (e != hipSuccess) return e;
+  return launch_mean_dp_reduce(frames, D, partial, max_ranges, sums, mean, s);
 }
 
 hipError_t launch_select_bf(const float* mean, const unsigned long long* sums, int det_rows, int det_cols,

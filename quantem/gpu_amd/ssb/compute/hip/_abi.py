@@ -51,6 +51,9 @@ SIGNATURES: dict[str, list] = {
     "ssb_object": [_vp, _c_double, _c_double, _c_double, _vp, _vp, _vp],
     "ssb_recon_u16": [_vp, _vp, _c_double, _c_double, _c_double, _c_double, _c_float, _c_float, _vp, _vp,
                       _vp],
+    "ssb_stage_count": [],
+    "ssb_timing_begin": [_vp, _c_int],
+    "ssb_timing_end": [_vp, _fp, _c_int, ctypes.POINTER(_c_int)],
     "ssb_phase_loss": [_vp, _vp, _c_int, _vp, _vp, _vp],
     "ssb_batch_eval": [_vp, _c_int, _vp, _c_int, _c_int, _vp, _vp],
 }
@@ -82,6 +85,8 @@ def lib() -> ctypes.CDLL:
         fn.restype = _c_int
     handle.ssb_last_error.argtypes = []
     handle.ssb_last_error.restype = ctypes.c_char_p
+    handle.ssb_stage_name.argtypes = [_c_int]
+    handle.ssb_stage_name.restype = ctypes.c_char_p
     _lib = handle
     return handle
 

@@ -268,3 +268,20 @@ class HipSSBEngine:
                   _abi.ptr(obj_out), _abi.ptr(phase_out), self._stream())
         self._rotation = float(rotation_rad)
         self._has_g = False
+
+    # ------------------------------------------------------------ timing
+    def timing_begin(self, max_calls: int) -> None:
+        """Record per-stage hipEvents for the next ``max_calls`` recon calls."""
+        _abi.call("ssb_timing_begin", self.plan.handle, int(max_calls))
+
+    def timing_end(self) -> dict:
+        """Summed per-stage milliseconds of the recorded recon calls (synchronises)."""
+        lib = _abi.lib()
+        n = int(lib.ssb_stage_count())
+        out = np.zeros(n, dtype=np.float32)
+        calls = ctypes.c_int(0)
+        _abi.call("ssb_timing_end", self.plan.handle, out.ctypes.data_as(_abi._fp), n, ctypes.byref(calls))
+        names = [lib.ssb_stage_name(i).decode() for i in range(n)]
+        res = {name: float(v) for name, v in zip(names, out)}
+        res["calls"] = int(calls.value)
+        return res
