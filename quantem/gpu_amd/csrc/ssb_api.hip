@@ -175,6 +175,8 @@ int ssb_plan_create(int scan_n, int det_rows, int det_cols, int max_bf, double w
                                           " is not supported; use 128, 256 or 512");
   if (det_rows < 1 || det_cols < 1 || max_bf < 1)
     return fail(QSSB_ERR_ARG, "detector shape and max_bf must be positive");
+  if ((long long)det_rows * det_cols > 65536)
+    return fail(QSSB_ERR_UNSUPPORTED, "detectors above 65536 pixels are not supported; bin the detector first");
   if (!(wavelength_A > 0) || !(semiangle_rad > 0) || !(ang_row_rad > 0) || !(ang_col_rad > 0) ||
       !(scan_sampling_row_A > 0) || !(scan_sampling_col_A > 0))
     return fail(QSSB_ERR_ARG, "optics and sampling must be positive");

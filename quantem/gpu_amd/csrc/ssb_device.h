@@ -113,6 +113,96 @@ __device__ __forceinline__ void fft_lds(float2* __restrict__ buf, const float2* 
   fft_stages<N, 1, COUNT, BLOCK, STRIDE, INV>(buf, tw);
 }
 
+// Same Stockham FFT for COUNT sequences stored interleaved, element (s, i)
+// at buf[i * COUNT + s] -- the layout of COUNT adjacent columns of a
+// row-major array copied row by row.  Work items run s fastest, so a wave
+// touches contiguous LDS.  The last stage writes to `out` in the padded
+// per-sequence layout out[s * OSTRIDE + i] (a separate buffer).
+template <int N, int R, int NS, int COUNT, int BLOCK, bool INV, int OSTRIDE>
+__device__ __forceinline__ void fft_stage_il(float2* __restrict__ buf, float2* __restrict__ out,
+                                             const float2* __restrict__ tw) {
+  constexpr int ITEMS = COUNT * (N / R);
+  constexpr int PER = (ITEMS + BLOCK - 1) / BLOCK;
+  constexpr bool LAST = (NS * R == N);
+  float2 v[PER][R];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int it = tid + p * BLOCK;
+    if (ITEMS % BLOCK == 0 || it < ITEMS) {
+      const int s = it % COUNT;
+      const int j = it / COUNT;
+      const int k = j % NS;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        float2 x = buf[(j + r * (N / R)) * COUNT + s];
+        if (NS > 1 && r > 0) {
+          float2 w = tw[(r * k * (N / (NS * R))) & (N - 1)];
+          if (INV) w.y = -w.y;
+          x = cmul(x, w);
+        }
+        v[p][r] = x;
+      }
+    }
+  }
+  if (!LAST) __syncthreads();
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int it = tid + p * BLOCK;
+    if (ITEMS % BLOCK == 0 || it < ITEMS) {
+      const int s = it % COUNT;
+      const int j = it / COUNT;
+      const int k = j % NS;
+      const int base = (j / NS) * NS * R + k;
+      float2 y[R];
+      if constexpr (R == 4) {
+        float2 a = v[p][0], b = v[p][1], c = v[p][2], d = v[p][3];
+        float2 apc = cadd(a, c), amc = csub(a, c);
+        float2 bpd = cadd(b, d), bmd = cmul_mi<INV>(csub(b, d));
+        y[0] = cadd(apc, bpd);
+        y[1] = cadd(amc, bmd);
+        y[2] = csub(apc, bpd);
+        y[3] = csub(amc, bmd);
+      } else {
+        y[0] = cadd(v[p][0], v[p][1]);
+        y[1] = csub(v[p][0], v[p][1]);
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (LAST) out[s * OSTRIDE + base + r * NS] = y[r];
+        else buf[(base + r * NS) * COUNT + s] = y[r];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+template <int N, int NS, int COUNT, int BLOCK, bool INV, int OSTRIDE>
+__device__ __forceinline__ void fft_stages_il(float2* __restrict__ buf, float2* __restrict__ out,
+                                              const float2* __restrict__ tw) {
+  if constexpr (NS < N) {
+    constexpr int R = (N / NS >= 4) ? 4 : 2;
+    fft_stage_il<N, R, NS, COUNT, BLOCK, INV, OSTRIDE>(buf, out, tw);
+    fft_stages_il<N, NS * R, COUNT, BLOCK, INV, OSTRIDE>(buf, out, tw);
+  }
+}
+
+// FFT of interleaved sequences in `buf` (destroyed) with the result in `out`
+// (padded per-sequence layout).  Caller synchronises before; ends with a barrier.
+template <int N, int COUNT, int BLOCK, int OSTRIDE, bool INV>
+__device__ __forceinline__ void fft_lds_il(float2* __restrict__ buf, float2* __restrict__ out,
+                                           const float2* __restrict__ tw) {
+  fft_stages_il<N, 1, COUNT, BLOCK, INV, OSTRIDE>(buf, out, tw);
+}
+
+// Asynchronous global -> LDS copy of 16 bytes per lane (gfx950
+// global_load_lds_dwordx4): lane l of the wave writes lds_wave_base + 16*l.
+// `lds_wave_base` must be wave-uniform.  Completion: wait_async_copies().
+__device__ __forceinline__ void copy_async16(const void* gptr, void* lds_wave_base) {
+  __builtin_amdgcn_global_load_lds(gptr, (__attribute__((address_space(3))) void*)lds_wave_base, 16, 0, 0);
+}
+__device__ __forceinline__ void wait_async_copies() { __builtin_amdgcn_s_waitcnt(0); }
+
 // ---------------------------------------------------------------- double-float
 // Unevaluated sums hi + lo of two floats (~44-bit significand), built from
 // error-free transformations (Dekker / Knuth two-sum, FMA two-product).

@@ -114,16 +114,23 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
                 const float2* __restrict__ echi_g, ApIso ap, df rout2, Optics o,
                 const float2* __restrict__ tw_g, int groups, float2* __restrict__ S_part, float2* __restrict__ Gt,
                 int do_trotter) {
-  constexpr int STRIDE = N + 1;
   constexpr int HALF = N / 2;
   constexpr int NSG = HALF / SG;
   constexpr int P = SG * N / BLOCK;      // points per thread
   constexpr int SSTEP = BLOCK / N;       // slot step between a thread's points
   constexpr int GCOLS = HALF + 1;
-  constexpr int LOADS = (N * SG) / BLOCK;
-  __shared__ float2 tile[SG * STRIDE];
-  __shared__ float2 tw[N];
-  __shared__ float2 colph[SG + 1];       // e^{i qy[j] b_b} per slot, plus column N/2
+  constexpr int TILE = N * SG;           // complex values per BF tile
+  constexpr int COPIES = (TILE + 2 * BLOCK - 1) / (2 * BLOCK);
+  constexpr int TSTRIDE = N + 1;
+  // slot groups sharing one 128-byte line of H
+  constexpr int LG = (FROM_H && SG * 8 < 128) ? 128 / (SG * 8) : 1;
+  static_assert(TILE % 128 == 0, "tile must split into whole waves of 16-byte copies");
+  // Async landing buffers (double-buffered).  FROM_H: interleaved rows of H,
+  // element (s, r) at [r * SG + s]; otherwise Gt columns, (s, q) at [s * N + q].
+  __shared__ __attribute__((aligned(16))) float2 abuf[2][TILE];
+  // FROM_H: transformed columns, (s, q) at [s * TSTRIDE + q]
+  __shared__ float2 tileT[FROM_H ? SG * TSTRIDE : 1];
+  __shared__ float2 tw[FROM_H ? N : 1];
   __shared__ float qyS[SG + 1], qySl[SG + 1];
   // output-B constants where Q_B != -Q_A: column N/2 (slot 0) per row, and
   // the Nyquist row q = N/2 per slot
@@ -141,13 +148,37 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
   const int qm = (N - q) & (N - 1);
   const df qxA{gp.qx[q], gp.qxl[q]};
   const df qxB0{gp.qx[qm], gp.qxl[qm]};   // row of output B when j >= 1
+  const int wave_base = tid & ~63;
 
   for (long long item = blockIdx.x; item < items; item += gridDim.x) {
-    const int g = (int)(item / NSG);
-    const int sg = (int)(item - (long long)g * NSG);
+    // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs,
+    // so items i, i+8, ..., i+8(LG-1) run together on one XCD; give them the
+    // LG slot groups that share each 128-byte line of H (read once per L2).
+    const long long sl = (item >> 3) % LG;
+    const long long t = (item / (8 * LG)) * 8 + (item & 7);
+    const int g = (int)(t / (NSG / LG));
+    const int sg = (int)((t % (NSG / LG)) * LG + sl);
     int b0, b1;
     group_range(g, groups, nbf, b0, b1);
+    // copy the source tile of BF pixel b into landing buffer `slot`
+    auto issue = [&](int b, int slot) {
+#pragma unroll
+      for (int c = 0; c < COPIES; ++c) {
+        const int e0 = (c * BLOCK + wave_base) * 2;         // wave's first element
+        if (e0 >= TILE) continue;                           // wave-uniform
+        const int el = e0 + (tid & 63) * 2;                 // this lane's first element
+        const float2* gsrc;
+        if (FROM_H) {
+          const int r = el / SG, sl = el % SG;
+          gsrc = src + ((long long)b * N + r) * HALF + sg * SG + sl;
+        } else {
+          gsrc = src + ((long long)b * GCOLS + sg * SG) * N + el;
+        }
+        copy_async16(gsrc, &abuf[slot][e0]);
+      }
+    };
     __syncthreads();
+    if (b0 < b1) issue(b0, 0);
     if (tid <= SG) {
       const int col = (tid < SG) ? sg * SG + tid : HALF;
       qyS[tid] = gp.qy[col];
@@ -183,30 +214,30 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
     }
 
     for (int b = b0; b < b1; ++b) {
+      const int cur = (b - b0) & 1;
+      wait_async_copies();
       __syncthreads();
-#pragma unroll
-      for (int l = 0; l < LOADS; ++l) {
-        const int idx = tid + l * BLOCK;
-        if (FROM_H) {
-          const int s = idx % SG, r = idx / SG;
-          tile[s * STRIDE + r] = src[((long long)b * N + r) * HALF + sg * SG + s];
-        } else {
-          const int s = idx / N, r = idx % N;
-          tile[s * STRIDE + r] = src[((long long)b * GCOLS + sg * SG + s) * N + r];
-        }
-      }
+      // per-BF operands are loaded before the next tile's copies are queued,
+      // so waiting on them never waits on the prefetch
       const float kxh = gp.kx[b], kyh = gp.ky[b];
       const df kx{kxh, gp.kxl[b]}, ky{kyh, gp.kyl[b]};
       const float ak = gp.ak[b];
-      if (do_trotter && tid <= SG) {
-        const int col = (tid < SG) ? sg * SG + tid : HALF;
-        colph[tid] = colph_g[(long long)b * GCOLS + col];
+      float2 rowA = make_float2(1.f, 0.f);
+      float2 colp[P];
+      float2 colHalf = make_float2(1.f, 0.f);
+      if (do_trotter) {
+        rowA = rowph_g[(long long)b * N + q];
+#pragma unroll
+        for (int i = 0; i < P; ++i) colp[i] = colph_g[(long long)b * GCOLS + sg * SG + s0 + i * SSTEP];
+        colHalf = colph_g[(long long)b * GCOLS + HALF];
       }
-      __syncthreads();
-      if (FROM_H) fft_lds<N, SG, BLOCK, STRIDE, false>(tile, tw);
-      // Gt[b] column N/2 (needed only for the resident, non-FFT source)
       float2 gHalf = make_float2(0.f, 0.f);
       if (!FROM_H && sg == 0) gHalf = src[((long long)b * GCOLS + HALF) * N + q];
+      asm volatile("" ::: "memory");
+      if (b + 1 < b1) issue(b + 1, cur ^ 1);
+      if (FROM_H) fft_lds_il<N, SG, BLOCK, TSTRIDE, false>(abuf[cur], tileT, tw);
+      const float2* tile = FROM_H ? tileT : abuf[cur];
+      constexpr int TS = FROM_H ? TSTRIDE : N;
       if (!do_trotter) {
         if (Gt) {
           float2* gb = Gt + (long long)b * GCOLS * N;
@@ -214,9 +245,9 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
           for (int i = 0; i < P; ++i) {
             const int s = s0 + i * SSTEP;
             const int j = sg * SG + s;
-            const float2 z = tile[s * STRIDE + q];
+            const float2 z = tile[s * TS + q];
             if (j == 0) {
-              const float2 zm = tile[s * STRIDE + qm];
+              const float2 zm = tile[s * TS + qm];
               gb[q] = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
               gb[(long long)HALF * N + q] = make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x));
             } else {
@@ -228,16 +259,15 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
       }
       const df kk = df_add(df_mul(kx, kx), df_mul(ky, ky));
       const df cb = df_sub(rout2, kk);           // Rout^2 - |k|^2
-      const float2 rowA = rowph_g[(long long)b * N + q];
 #pragma unroll
       for (int i = 0; i < P; ++i) {
         const int s = s0 + i * SSTEP;
         const int j = sg * SG + s;
-        const float2 z = tile[s * STRIDE + q];
+        const float2 z = tile[s * TS + q];
         float2 GA, GB;
         if (j == 0) {
           if (FROM_H) {
-            const float2 zm = tile[s * STRIDE + qm];
+            const float2 zm = tile[s * TS + qm];
             GA = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
             GB = make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x));
           } else {
@@ -251,7 +281,7 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
         const df qy{qyS[s], qySl[s]};
         float am, app;
         aperture_pair<ISO>(qxA, qy, kx, ky, cb, q2A[i], ap, o, am, app);
-        const float2 eA = cmul(rowA, colph[s]);
+        const float2 eA = cmul(rowA, colp[i]);
         const bool mirror = (j != 0) && (q != HALF);
         float bm, bp;
         float2 eB, echiB;
@@ -267,9 +297,8 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
           const df q2B = (j == 0) ? q2halfc[q] : q2A[i];
           echiB = (j == 0) ? ehalfc[q] : enyqr[s];
           aperture_pair<ISO>(qxB, qyB, kx, ky, cb, q2B, ap, o, bm, bp);
-          const float2 rowB = (j == 0) ? rowA : rowph_g[(long long)b * N + qm];
-          const float2 colB = (j == 0) ? colph[SG] : cconj(colph[s]);
-          eB = cmul(rowB, colB);
+          // rows q and -q coincide at q == N/2, so the row phasor is rowA
+          eB = cmul(rowA, (j == 0) ? colHalf : cconj(colp[i]));
         }
         if (am != 0.0f || app != 0.0f) {
           const float wr = (am - app) * echiA[i].x;
@@ -300,6 +329,9 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
           gb[(long long)j * N + q] = GA;
           if (j == 0) gb[(long long)HALF * N + q] = GB;
         }
+        // one point at a time: keeps the scheduler from overlapping the
+        // points' temporaries, which would spill the accumulators
+        __builtin_amdgcn_sched_barrier(0);
       }
     }
     if (do_trotter) {
@@ -526,13 +558,13 @@ hipError_t launch_trotter_slots(int N, bool from_h, const float2* src, const BfI
   } while (0)
   switch (N) {
     case 128:
-      QSSB_SLOTS(128, 16, 1024);
+      QSSB_SLOTS(128, 8, 1024);
       break;
     case 256:
-      QSSB_SLOTS(256, 16, 1024);
+      QSSB_SLOTS(256, 8, 1024);
       break;
     case 512:
-      QSSB_SLOTS(512, 8, 1024);
+      QSSB_SLOTS(512, 4, 1024);
       break;
     default:
       return hipErrorInvalidValue;

@@ -71,74 +71,127 @@ k_dp_partial(const uint16_t* __restrict__ data, long long frames, int D, int fra
 }
 
 // Pass 2: fold the frame ranges into exact uint64 sums; mean = f32(sum)/f32(N)
-// with one correctly rounded division (backend.py:438-441).
+// with one correctly rounded division (backend.py:438-441).  A workgroup
+// owns 64 pixels; its four waves fold interleaved quarters of the ranges
+// (each wave reads 256 contiguous bytes per range) and combine in LDS.
 __global__ void __launch_bounds__(256)
 k_dp_reduce(const uint32_t* __restrict__ partial, int ranges, int D, long long frames,
             unsigned long long* __restrict__ sums, float* __restrict__ mean) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= D) return;
+  __shared__ unsigned long long part[4][64];
+  const int lane = threadIdx.x & 63;
+  const int quarter = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + lane;
   unsigned long long s = 0ull;
-  for (int r = 0; r < ranges; ++r) s += partial[(long long)r * D + i];
-  sums[i] = s;
-  mean[i] = (float)s / (float)frames;
+  if (i < D) {
+    int r = quarter;
+    for (; r + 12 < ranges; r += 16) {
+      const uint32_t a = partial[(long long)r * D + i];
+      const uint32_t b = partial[(long long)(r + 4) * D + i];
+      const uint32_t c = partial[(long long)(r + 8) * D + i];
+      const uint32_t d = partial[(long long)(r + 12) * D + i];
+      s += (unsigned long long)a + b + c + d;
+    }
+    for (; r < ranges; r += 4) s += partial[(long long)r * D + i];
+  }
+  part[quarter][lane] = s;
+  __syncthreads();
+  if (quarter == 0 && i < D) {
+    const unsigned long long t = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
+    sums[i] = t;
+    mean[i] = (float)t / (float)frames;
+  }
 }
 
 // ------------------------------------------------------------------------
-// Bright-field selection in one workgroup (1024 threads).
+// Bright-field selection in one workgroup (1024 threads = 16 waves).
 // backend.py:448-518, see select_bf() in the oracle for the float32 rules.
+// Reductions run within each wave through cross-lane shuffles, then across
+// the 16 wave results in LDS (two barriers per reduction).
 // ------------------------------------------------------------------------
 #define MASK_THREADS 1024
+#define MASK_WAVES (MASK_THREADS / 64)
+#define MASK_PER 64   // max pixels per thread (D <= 65536)
 
-__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
+__device__ __forceinline__ void two_sum_f(float a, float b, float& s, float& e) {
   s = a + b;
   float bb = s - a;
   e = (a - (s - bb)) + (b - bb);
 }
 
-// Block reduction of a compensated (s, c) pair.
-__device__ float block_sum_comp(float s, float c, float* sh_s, float* sh_c) {
-  const int tid = threadIdx.x;
-  sh_s[tid] = s;
-  sh_c[tid] = c;
-  __syncthreads();
-  for (int off = MASK_THREADS / 2; off > 0; off >>= 1) {
-    if (tid < off) {
-      float hs, he;
-      two_sum(sh_s[tid], sh_s[tid + off], hs, he);
-      sh_s[tid] = hs;
-      sh_c[tid] = sh_c[tid] + sh_c[tid + off] + he;
-    }
-    __syncthreads();
+// compensated (s, c) pair summed over the wave, then over the block
+__device__ float block_sum_comp(float s, float c, float* sh) {
+#pragma clang fp contract(off)
+  for (int off = 32; off > 0; off >>= 1) {
+    const float os = __shfl_xor(s, off, 64);
+    const float oc = __shfl_xor(c, off, 64);
+    float hs, he;
+    two_sum_f(s, os, hs, he);
+    s = hs;
+    c = c + oc + he;
   }
-  float r = sh_s[0] + sh_c[0];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) {
+    sh[w] = s;
+    sh[MASK_WAVES + w] = c;
+  }
   __syncthreads();
-  return r;
+  float ts = (lane < MASK_WAVES) ? sh[lane] : 0.0f;
+  float tc = (lane < MASK_WAVES) ? sh[MASK_WAVES + lane] : 0.0f;
+  for (int off = MASK_WAVES / 2; off > 0; off >>= 1) {
+    const float os = __shfl_xor(ts, off, 64);
+    const float oc = __shfl_xor(tc, off, 64);
+    float hs, he;
+    two_sum_f(ts, os, hs, he);
+    ts = hs;
+    tc = tc + oc + he;
+  }
+  __syncthreads();
+  // lanes 0..15 hold the total; broadcast lane 0 to the whole wave
+  return __shfl(ts + tc, 0, 64);
 }
 
 __device__ float block_max(float v, float* sh) {
-  const int tid = threadIdx.x;
-  sh[tid] = v;
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) sh[w] = v;
   __syncthreads();
-  for (int off = MASK_THREADS / 2; off > 0; off >>= 1) {
-    if (tid < off) sh[tid] = fmaxf(sh[tid], sh[tid + off]);
-    __syncthreads();
-  }
-  float r = sh[0];
+  float t = (lane < MASK_WAVES) ? sh[lane] : -INFINITY;
+  for (int off = MASK_WAVES / 2; off > 0; off >>= 1) t = fmaxf(t, __shfl_xor(t, off, 64));
   __syncthreads();
-  return r;
+  return __shfl(t, 0, 64);
 }
 
-__device__ long long block_sum_i64(long long v, long long* sh) {
-  const int tid = threadIdx.x;
-  sh[tid] = v;
-  __syncthreads();
-  for (int off = MASK_THREADS / 2; off > 0; off >>= 1) {
-    if (tid < off) sh[tid] += sh[tid + off];
-    __syncthreads();
+__device__ __forceinline__ long long shfl_xor_i64(long long v, int off) {
+  int lo = (int)(v & 0xffffffffLL), hi = (int)(v >> 32);
+  lo = __shfl_xor(lo, off, 64);
+  hi = __shfl_xor(hi, off, 64);
+  return ((long long)hi << 32) | (unsigned int)lo;
+}
+
+// three int64 sums at once
+__device__ void block_sum3_i64(long long& a, long long& b, long long& c, long long* sh) {
+  for (int off = 32; off > 0; off >>= 1) {
+    a += shfl_xor_i64(a, off);
+    b += shfl_xor_i64(b, off);
+    c += shfl_xor_i64(c, off);
   }
-  long long r = sh[0];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) {
+    sh[w] = a;
+    sh[MASK_WAVES + w] = b;
+    sh[2 * MASK_WAVES + w] = c;
+  }
   __syncthreads();
-  return r;
+  long long ta = 0, tb = 0, tc = 0;
+  for (int k = 0; k < MASK_WAVES; ++k) {
+    ta += sh[k];
+    tb += sh[MASK_WAVES + k];
+    tc += sh[2 * MASK_WAVES + k];
+  }
+  __syncthreads();
+  a = ta;
+  b = tb;
+  c = tc;
 }
 
 // float32 squared distance without contraction into FMA (matches the
@@ -165,49 +218,47 @@ __global__ void __launch_bounds__(MASK_THREADS)
 k_select_bf(const float* __restrict__ mean, const unsigned long long* __restrict__ sums,
             int det_rows, int det_cols, float threshold, float bf_radius, int max_bf,
             int* __restrict__ bf_rows, int* __restrict__ bf_cols, BfInfo* __restrict__ info) {
-  __shared__ float sh_a[MASK_THREADS];
-  __shared__ float sh_b[MASK_THREADS];
-  __shared__ long long sh_i[MASK_THREADS];
+  __shared__ float sh_f[2 * MASK_WAVES];
+  __shared__ long long sh_i[3 * MASK_WAVES];
+  __shared__ int sh_scan[MASK_WAVES];
   const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
   const int D = det_rows * det_cols;
-  const int per = (D + MASK_THREADS - 1) / MASK_THREADS;
+  // each thread owns the contiguous pixels [p0, p0 + per)
+  const int per = (D + MASK_THREADS - 1) / MASK_THREADS;   // <= MASK_PER
   const int p0 = tid * per;
   const int p1 = min(D, p0 + per);
-
+#define FOR_PIX for (int i = p0; i < p1; ++i)
   float lmax = -INFINITY;
-  for (int i = p0; i < p1; ++i) lmax = fmaxf(lmax, mean[i]);
-  const float mmax = block_max(lmax, sh_a);
+  FOR_PIX lmax = fmaxf(lmax, mean[i]);
+  const float mmax = block_max(lmax, sh_f);
   const float cut = mmax * threshold;
 
   float center_r, center_c, r2;
-  int status = 0;
   if (bf_radius <= 0.0f) {
-    // mean + std threshold (population std), compensated sums
-    Kahan k;
-    k.init();
-    for (int i = p0; i < p1; ++i) k.add(mean[i]);
-    const float total = block_sum_comp(k.s, -k.c, sh_a, sh_b);
+    Kahan k1;
+    k1.init();
+    FOR_PIX k1.add(mean[i]);
+    const float total = block_sum_comp(k1.s, -k1.c, sh_f);
     const float mu = total / (float)D;
-    Kahan q;
-    q.init();
-    for (int i = p0; i < p1; ++i) {
-      float d = mean[i] - mu;
-      q.add(d * d);
+    Kahan k2;
+    k2.init();
+    FOR_PIX {
+      const float d = mean[i] - mu;
+      k2.add(d * d);
     }
-    const float ss = block_sum_comp(q.s, -q.c, sh_a, sh_b);
+    const float ss = block_sum_comp(k2.s, -k2.c, sh_f);
     const float sd = sqrtf(ss / (float)D);
     const float thr = mu + sd;
     long long cnt = 0, sr = 0, sc = 0;
-    for (int i = p0; i < p1; ++i) {
+    FOR_PIX {
       if (mean[i] > thr) {
         cnt += 1;
         sr += i / det_cols;
         sc += i % det_cols;
       }
     }
-    cnt = block_sum_i64(cnt, sh_i);
-    sr = block_sum_i64(sr, sh_i);
-    sc = block_sum_i64(sc, sh_i);
+    block_sum3_i64(cnt, sr, sc, sh_i);
     if (cnt > 0) {
       center_r = (float)sr / (float)cnt;
       center_c = (float)sc / (float)cnt;
@@ -224,24 +275,22 @@ k_select_bf(const float* __restrict__ mean, const unsigned long long* __restrict
     kr.init();
     kc.init();
     long long n = 0, sr = 0, sc = 0;
-    for (int i = p0; i < p1; ++i) {
-      const float w = mean[i];
-      if (w > cut) {
+    FOR_PIX {
+      const float wv = mean[i];
+      if (wv > cut) {
         const int r = i / det_cols, c = i % det_cols;
-        kw.add(w);
-        kr.add((float)r * w);
-        kc.add((float)c * w);
+        kw.add(wv);
+        kr.add((float)r * wv);
+        kc.add((float)c * wv);
         n += 1;
         sr += r;
         sc += c;
       }
     }
-    const float wsum = block_sum_comp(kw.s, -kw.c, sh_a, sh_b);
-    const float rsum = block_sum_comp(kr.s, -kr.c, sh_a, sh_b);
-    const float csum = block_sum_comp(kc.s, -kc.c, sh_a, sh_b);
-    n = block_sum_i64(n, sh_i);
-    sr = block_sum_i64(sr, sh_i);
-    sc = block_sum_i64(sc, sh_i);
+    const float wsum = block_sum_comp(kw.s, -kw.c, sh_f);
+    const float rsum = block_sum_comp(kr.s, -kr.c, sh_f);
+    const float csum = block_sum_comp(kc.s, -kc.c, sh_f);
+    block_sum3_i64(n, sr, sc, sh_i);
     if (wsum > 0.0f) {
       center_r = rsum / wsum;
       center_c = csum / wsum;
@@ -255,47 +304,54 @@ k_select_bf(const float* __restrict__ mean, const unsigned long long* __restrict
     r2 = bf_radius * bf_radius;
   }
 
-  // Keep candidate pixels inside the disk, compacted in raster order.
+  // keep candidates inside the disk, compacted in raster order; also count
+  // candidates and sum the exact integer columns of the kept pixels (dc)
+  unsigned long long mask_bits = 0ull;
+  long long any_cand = 0, dcs = 0;
   int local = 0;
-  long long any_cand = 0;
-  for (int i = p0; i < p1; ++i) {
+  FOR_PIX {
     if (mean[i] > cut) {
       any_cand += 1;
-      if (dist2_f32((float)(i / det_cols), (float)(i % det_cols), center_r, center_c) <= r2) local += 1;
+      if (dist2_f32((float)(i / det_cols), (float)(i % det_cols), center_r, center_c) <= r2) {
+        mask_bits |= 1ull << (i - p0);
+        local += 1;
+        dcs += (long long)sums[i];
+      }
     }
   }
-  any_cand = block_sum_i64(any_cand, sh_i);
-  // exclusive scan of `local`
-  sh_i[tid] = local;
-  __syncthreads();
-  for (int off = 1; off < MASK_THREADS; off <<= 1) {
-    long long v = (tid >= off) ? sh_i[tid - off] : 0;
-    __syncthreads();
-    sh_i[tid] += v;
-    __syncthreads();
+  long long dummy = 0;
+  block_sum3_i64(any_cand, dcs, dummy, sh_i);
+  // exclusive scan of `local` over the block: wave scan + wave offsets
+  int incl = local;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += t;
   }
-  const long long incl = sh_i[tid];
-  const long long nbf = sh_i[MASK_THREADS - 1];
+  if (lane == 63) sh_scan[w] = incl;
   __syncthreads();
-  long long pos = incl - local;
-  unsigned long long dcs = 0ull;
-  for (int i = p0; i < p1; ++i) {
-    if (mean[i] > cut &&
-        dist2_f32((float)(i / det_cols), (float)(i % det_cols), center_r, center_c) <= r2) {
+  int wave_off = 0, nbf = 0;
+  for (int k = 0; k < MASK_WAVES; ++k) {
+    const int t = sh_scan[k];
+    if (k < w) wave_off += t;
+    nbf += t;
+  }
+  int pos = wave_off + incl - local;
+  FOR_PIX {
+    if (mask_bits & (1ull << (i - p0))) {
       if (pos < max_bf) {
         bf_rows[pos] = i / det_cols;
         bf_cols[pos] = i % det_cols;
       }
-      dcs += sums[i];
       ++pos;
     }
   }
-  const long long dc_total = block_sum_i64((long long)dcs, sh_i);
+#undef FOR_PIX
   if (tid == 0) {
+    int status = 0;
     if (any_cand == 0) status = QSSB_DEV_NO_CANDIDATES;
     else if (nbf == 0) status = QSSB_DEV_EMPTY_DISK;
     else if (nbf > max_bf) status = QSSB_DEV_CAPACITY;
-    const int nb = (status == 0) ? (int)nbf : 0;
+    const int nb = (status == 0) ? nbf : 0;
     info->nbf = nb;
     info->status = status;
     info->center_r = center_r;
@@ -304,7 +360,7 @@ k_select_bf(const float* __restrict__ mean, const unsigned long long* __restrict
     float dc = 0.0f;
     if (nb > 0) {
       // G_b(0,0) is the exact integer column sum; dc = mean_b G_b(0,0).
-      const unsigned long long tot = (unsigned long long)dc_total;
+      const unsigned long long tot = (unsigned long long)dcs;
       const unsigned long long qd = tot / (unsigned long long)nb;
       const unsigned long long rm = tot % (unsigned long long)nb;
       dc = (float)qd + (float)rm / (float)nb;
@@ -374,17 +430,23 @@ k_gather_rowfft(const uint16_t* __restrict__ data, int D, int det_cols,
       offs[tid] = (b < nbf) ? bf_rows[b] * det_cols + bf_cols[b] : -1;
     }
     __syncthreads();
-    // gather: lane -> BF pixel within the chunk, lane group -> frame
+    // gather: lane -> BF pixel within the chunk, lane group -> frame; all
+    // of a thread's loads are issued before any is consumed
     {
       constexpr int FR = 256 / CH;
+      constexpr int NL = N / FR;
       const int j = tid % CH;
       const int off = offs[j];
+      const int c0 = tid / CH;
       float* zf = reinterpret_cast<float*>(z);
-      const long long frame0 = (long long)rr * N;
-      for (int c = tid / CH; c < N; c += FR) {
-        float v = 0.0f;
-        if (off >= 0) v = (float)data[(frame0 + c) * (long long)D + off];
-        zf[2 * ((j >> 1) * STRIDE + c) + (j & 1)] = v;
+      const uint16_t* base = data + (long long)rr * N * D + (off >= 0 ? off : 0);
+      unsigned short raw[NL];
+#pragma unroll
+      for (int m = 0; m < NL; ++m) raw[m] = base[(long long)(c0 + m * FR) * D];
+#pragma unroll
+      for (int m = 0; m < NL; ++m) {
+        const int c = c0 + m * FR;
+        zf[2 * ((j >> 1) * STRIDE + c) + (j & 1)] = (off >= 0) ? (float)raw[m] : 0.0f;
       }
     }
     __syncthreads();
@@ -447,7 +509,7 @@ hipError_t launch_mean_dp_reduce(long long frames, int D, const uint32_t* partia
                                  unsigned long long* sums, float* mean, hipStream_t s) {
   int ranges, fpr;
   range_split(frames, max_ranges, ranges, fpr);
-  hipLaunchKernelGGL(k_dp_reduce, dim3((D + 255) / 256), dim3(256), 0, s, partial, ranges, D, frames, sums, mean);
+  hipLaunchKernelGGL(k_dp_reduce, dim3((D + 63) / 64), dim3(256), 0, s, partial, ranges, D, frames, sums, mean);
   return hipGetLastError();
 }
 
