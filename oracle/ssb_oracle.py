@@ -244,6 +244,13 @@ def expand_hermitian(half: np.ndarray) -> np.ndarray:
     return full
 
 
+def _full_plane(g: np.ndarray) -> np.ndarray:
+    """A full ``[b, N, N]`` stack as is; a Hermitian half-plane expanded.
+
+    The reference engine accepts both storages (``cuda/engine.py:497-502``)."""
+    return g if g.shape[2] == g.shape[1] else expand_hermitian(g)
+
+
 def _workers() -> int:
     import os
     try:
@@ -281,7 +288,7 @@ class Geometry:
 
 def detector_geometry(rows, cols, center, det_shape, det_sampling_mrad,
                       wavelength, semiangle_mrad, rotation_rad, scan_n,
-                      scan_sampling, dtype=np.float64) -> Geometry:
+                      scan_sampling, dtype=np.float64, recip=None) -> Geometry:
     """Rotation-dependent BF geometry, ``cuda/engine.py:784-848``.
 
     Detector sampling (``cuda/backend.py:316-327``): reciprocal sampling is
@@ -290,16 +297,22 @@ def detector_geometry(rows, cols, center, det_shape, det_sampling_mrad,
     ``ky = (col - center_col)*recip_x``; passive rotation by ``-rotation``.
     Aperture: ``clip((semiangle - alpha)/denom + 0.5, 0, 1)`` with
     ``denom = sqrt((cos(phi)*ang_y)^2 + (sin(phi)*ang_x)^2)``.
+
+    ``recip`` overrides the reciprocal sampling, as an engine built directly
+    from ``sampling``/``gpts`` does (``cuda/engine.py:792-793``).
     """
     dt = dtype
     ang_y, ang_x = det_sampling_mrad
-    recip_y = ang_y * 1e-3 / wavelength
-    recip_x = ang_x * 1e-3 / wavelength
-    # the engine recomputes recip from sampling = 1/(recip*gpts); same value
-    sampling_y = 1.0 / (recip_y * det_shape[0])
-    sampling_x = 1.0 / (recip_x * det_shape[1])
-    recip_y = 1.0 / (sampling_y * det_shape[0])
-    recip_x = 1.0 / (sampling_x * det_shape[1])
+    if recip is None:
+        recip_y = ang_y * 1e-3 / wavelength
+        recip_x = ang_x * 1e-3 / wavelength
+        # the engine recomputes recip from sampling = 1/(recip*gpts); same value
+        sampling_y = 1.0 / (recip_y * det_shape[0])
+        sampling_x = 1.0 / (recip_x * det_shape[1])
+        recip_y = 1.0 / (sampling_y * det_shape[0])
+        recip_x = 1.0 / (sampling_x * det_shape[1])
+    else:
+        recip_y, recip_x = float(recip[0]), float(recip[1])
     iy = rows.astype(dt) - dt(center[0])
     ix = cols.astype(dt) - dt(center[1])
     kxa = iy * dt(recip_y)
@@ -533,7 +546,7 @@ def corrected_fourier_mean(G_half, dc, g: Geometry, c10, c12, phi12,
     for b0 in range(0, nb, step):
         b1 = min(nb, b0 + step)
         gh = gamma_fn(b0, b1) if gamma_fn else gamma_hat_block(g, b0, b1, c10, c12, phi12)
-        full = expand_hermitian(G_half[b0:b1].astype(cdt))
+        full = _full_plane(G_half[b0:b1].astype(cdt))
         acc += (full * np.conj(gh)).sum(axis=0)
     acc /= nb
     acc[0, 0] = dc
@@ -561,7 +574,7 @@ def _per_bf_moments(G_half, dc, g: Geometry, gamma_fn):
     for b0 in range(0, nb, step):
         b1 = min(nb, b0 + step)
         gh = gamma_fn(b0, b1)
-        corr = expand_hermitian(G_half[b0:b1].astype(cdt)) * np.conj(gh)
+        corr = _full_plane(G_half[b0:b1].astype(cdt)) * np.conj(gh)
         corr[:, 0, 0] = dc
         ob = sfft.ifft2(corr, axes=(-2, -1), workers=_workers())
         ph = np.angle(ob).astype(dt)

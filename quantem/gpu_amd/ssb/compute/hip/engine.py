@@ -94,6 +94,7 @@ class HipSSBEngine:
         self._bf: BrightFieldState | None = None
         self._rotation: float | None = None
         self._has_g = False
+        self._device_bf = False
 
     # ------------------------------------------------------------------ utils
     @property
@@ -179,7 +180,9 @@ class HipSSBEngine:
     @property
     def num_bf(self) -> int:
         if self._bf is None:
-            raise ValueError("no bright-field selection")
+            if not self._device_bf:
+                raise ValueError("no bright-field selection")
+            self.bf_state()
         return int(self._bf.rows.size)
 
     def set_detector_sampling(self, recip_row: float, recip_col: float) -> None:
@@ -268,6 +271,8 @@ class HipSSBEngine:
                   _abi.ptr(obj_out), _abi.ptr(phase_out), self._stream())
         self._rotation = float(rotation_rad)
         self._has_g = False
+        self._bf = None            # selected on the device; read back on demand
+        self._device_bf = True
 
     # ------------------------------------------------------------ timing
     def timing_begin(self, max_calls: int) -> None:

@@ -1,0 +1,326 @@
+"""GPU parity: the HIP path, called through the C ABI, against the CPU oracle.
+
+Integer stages (mean DP, bright-field mask) must be bit-exact.  Fourier
+stacks and objects are compared with the FP64 oracle; phases must stay
+within 1e-4 rad RMS of it (the accuracy bar of the hot path) and within
+the reference's own percentile bars (p99.9 < 3e-4 rad,
+``tests/test_ssb_cuda_128.py:367-377``; Fourier-sum objects p99.9 abs
+< 5e-9 and rel < 1e-4, ``:466-479``).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import ssb_oracle as O
+from tests import _cases as C
+
+pytestmark = pytest.mark.gpu
+
+PHASE_RMS = 1e-4          # rad, vs the FP64 oracle
+PHASE_P999 = 3e-4         # rad
+
+
+def _lib_is_in_tree():
+    from quantem.gpu_amd.ssb.compute.hip import _abi
+    assert _abi.lib()._name == str(_abi.LIB_PATH)
+
+
+def make_engine(torch, n, det, max_bf, wavelength, semiangle_mrad, ang_mrad, step):
+    from quantem.gpu_amd.ssb.compute.hip.engine import HipSSBEngine, PlanSpec, SSBPlan
+    spec = PlanSpec(n, det[0], det[1], max_bf, wavelength, semiangle_mrad * 1e-3,
+                    ang_mrad[0] * 1e-3, ang_mrad[1] * 1e-3, step, step)
+    return HipSSBEngine(SSBPlan(spec, device=torch.device("cuda", 0)))
+
+
+def engine_for_case(torch, case: C.EngineCase, g=None):
+    """The reference ``_make_engine`` setup on the HIP plan (G imported)."""
+    g = case.G if g is None else g
+    eng = make_engine(torch, case.n, case.det, len(case.rows), case.wavelength,
+                      case.semiangle_mrad, case.ang_mrad, case.q_step)
+    if case.recip is not None:
+        eng.set_detector_sampling(*case.recip)
+    eng.set_bf(case.rows, case.cols, case.center, case.dc)
+    eng.import_g(torch.from_numpy(np.ascontiguousarray(g)))
+    eng.cache_rotation(0.0)
+    return eng
+
+
+def object_errors(obj, ref):
+    err = np.abs(obj.astype(np.complex128) - ref)
+    rel = err / np.maximum(np.abs(ref), 1e-6)
+    return float(np.percentile(err, 99.9)), float(np.percentile(rel, 99.9))
+
+
+def scaled_error(obj, ref):
+    """p99.9 of |obj - ref| over the RMS of |ref|.
+
+    The reference's absolute bars (5e-9, 1e-7) compare two storages through
+    identical float32 arithmetic; across different summation orders the
+    float32 bar scales with the object, so it is stated relative to its RMS:
+    2e-6 is 34 float32 ulps, and at the _make_engine scale (RMS ~4e-3) it is
+    the reference's 5e-9."""
+    ref = np.asarray(ref, np.complex128)
+    err = np.abs(np.asarray(obj, np.complex128) - ref)
+    return float(np.percentile(err, 99.9) / np.sqrt(np.mean(np.abs(ref) ** 2)))
+
+
+OBJ_SCALED = 2e-6
+
+
+def engine_for_data(torch, data_np, *, semiangle_mrad, det_sampling, step, voltage_kV=300,
+                    max_bf=4096):
+    n, det = data_np.shape[0], data_np.shape[2:]
+    lam = O.wavelength_angstrom(voltage_kV)
+    return make_engine(torch, n, det, max_bf, lam, semiangle_mrad, (det_sampling, det_sampling), step)
+
+
+def synthetic(scan, det, aberrations, rotation_deg=0.0, dose=5e4, seed=2026):
+    from quantem.gpu_amd.ssb.synthetic import SyntheticConfig, generate
+    cfg = SyntheticConfig(scan=scan, det=det, aberrations=aberrations, rotation_deg=rotation_deg,
+                          dose=dose, seed=seed)
+    return cfg, generate(cfg, device="cuda").cpu().numpy()
+
+
+# ------------------------------------------------------------ integer work
+@pytest.mark.parametrize("scan,det,seed", [(128, 16, 0), (128, 64, 1), (256, 32, 2), (512, 8, 3)])
+def test_mean_dp_bit_exact(cuda, scan, det, seed):
+    _lib_is_in_tree()
+    rng = np.random.default_rng(seed)
+    data = rng.integers(0, 65536, size=(scan, scan, det, det), dtype=np.uint16)
+    data[:, :, 0, 0] = 65535
+    eng = make_engine(cuda, scan, (det, det), 16, 0.0197, 20.0, (1.0, 1.0), 0.25)
+    sums, mean = eng.mean_dp(cuda.from_numpy(data).cuda())
+    rs, rm = O.mean_dp(data)
+    np.testing.assert_array_equal(sums.cpu().numpy().astype(np.uint64), rs)
+    np.testing.assert_array_equal(mean.cpu().numpy().view(np.uint32), rm.view(np.uint32))
+
+
+@pytest.mark.parametrize("kind", ["synthetic-auto", "poisson-radius3", "poisson-threshold"])
+def test_bf_mask_matches_oracle(cuda, kind):
+    if kind == "synthetic-auto":
+        cfg, data = synthetic(128, 64, {"C10": 200.0})
+        thr, radius, semi, ds = 0.0, None, 20.0, 1.25
+    elif kind == "poisson-radius3":
+        data, thr, radius, semi, ds = C.poisson_data(123), 0.0, 3.0, 21.4, 1.0
+    else:
+        data, thr, radius, semi, ds = C.poisson_data(124), 0.5, 3.5, 21.4, 1.0
+    _, mean = O.mean_dp(data)
+    ref = O.select_bf(mean, thr, radius)
+    if ref.near_tie:
+        pytest.skip("a pixel sits within float noise of a threshold (reduction order decides)")
+    eng = engine_for_data(cuda, data, semiangle_mrad=semi, det_sampling=ds, step=0.5)
+    eng.mean_dp(cuda.from_numpy(data).cuda())
+    bf = eng.select_bf(thr, radius)
+    np.testing.assert_array_equal(bf.rows, ref.rows)
+    np.testing.assert_array_equal(bf.cols, ref.cols)
+    assert bf.center == pytest.approx(ref.center, abs=1e-5)
+
+
+def test_bf_mask_errors(cuda):
+    data = np.zeros((128, 128, 8, 8), np.uint16)
+    eng = engine_for_data(cuda, data, semiangle_mrad=20.0, det_sampling=1.0, step=0.5)
+    eng.mean_dp(cuda.from_numpy(data).cuda())
+    with pytest.raises(ValueError, match="No bright-field pixels found"):
+        eng.select_bf(0.0, None)
+
+
+# ---------------------------------------------------------- Fourier stack
+def test_fourier_stack_of_ramp_closed_form(cuda, golden):
+    gv = golden["scan_ramp_128"]
+    data = C.ramp_data(128, 8)
+    eng = engine_for_data(cuda, data, semiangle_mrad=20.0, det_sampling=1.0, step=0.5)
+    eng.set_bf(gv["rows"], gv["cols"], (4.0, 4.0), 0j)
+    eng.prepare_g(cuda.from_numpy(data).cuda())
+    g = eng.G_qk.cpu().numpy().astype(np.complex128)
+    assert g.shape == (5, 128, 65)
+    scale = float(np.abs(g).max())
+    np.testing.assert_allclose(g[:, 0, 0], [complex(*v) for v in gv["g00"]], rtol=1e-6)
+    row0 = np.array(gv["g_row0_minus_dc_real"]) + 1j * np.array(gv["g_row0_minus_dc_imag"])
+    col0 = np.array(gv["g_col0_minus_dc_real"]) + 1j * np.array(gv["g_col0_minus_dc_imag"])
+    for b in range(5):
+        np.testing.assert_allclose(g[b, 0, 1:], row0, atol=2e-6 * scale)
+        np.testing.assert_allclose(g[b, 1:, 0], col0, atol=2e-6 * scale)
+        assert np.abs(g[b, 1:, 1:]).max() < 2e-6 * scale
+
+
+def test_fourier_stack_matches_fp64_oracle(cuda):
+    cfg, data = synthetic(128, 64, {"C10": 200.0})
+    p = O.prepare(data, voltage_kV=300, semiangle_mrad=20.0, scan_sampling=cfg.step_A,
+                  det_sampling=1.25)
+    eng = engine_for_data(cuda, data, semiangle_mrad=20.0, det_sampling=1.25, step=cfg.step_A)
+    d = cuda.from_numpy(data).cuda()
+    eng.mean_dp(d)
+    bf = eng.select_bf(0.0, None)
+    eng.prepare_g(d)
+    g = eng.G_qk.cpu().numpy().astype(np.complex128)
+    rel = np.linalg.norm(g - p.G) / np.linalg.norm(p.G)
+    assert rel < 2e-7
+    assert abs(bf.dc - p.dc) <= 1e-6 * abs(p.dc)
+
+
+# --------------------------------------------------------------- objects
+@pytest.mark.parametrize("size,num_bf", [(128, 7), (256, 5), (512, 5)])
+def test_make_engine_object_matches_oracle(cuda, size, num_bf):
+    case = C.make_engine_case(size, num_bf)
+    eng = engine_for_case(cuda, case)
+    c10, c12, phi12 = C.MAKE_ENGINE_ABERRATIONS
+    phase = cuda.empty((size, size), dtype=cuda.float32, device="cuda")
+    obj = eng.reconstruct_object(c10, c12, phi12, phase_out=phase).cpu().numpy()
+    ref = O.object_wave(case.G, case.dc, case.geometry(), c10, c12, phi12)
+    abs_p, rel_p = object_errors(obj, ref)
+    assert abs_p < 5e-9 and rel_p < 1e-4
+    rms, p999, _ = C.phase_errors(phase.cpu().numpy(), np.angle(ref))
+    assert rms < PHASE_RMS and p999 < PHASE_P999
+
+
+@pytest.mark.parametrize("size,num_bf", [(128, 7), (256, 5), (512, 5)])
+def test_hermitian_import_equals_full(cuda, size, num_bf):
+    full = C.hermitian_stack(size, num_bf)
+    half = np.ascontiguousarray(full[:, :, : size // 2 + 1])
+    sym_full = O.expand_hermitian(half)
+    c10, c12, phi12 = C.MAKE_ENGINE_ABERRATIONS
+    objs = []
+    for g in (sym_full, half):
+        case = C.make_engine_case(size, num_bf, g_qk=g)
+        objs.append(engine_for_case(cuda, case).reconstruct_object(c10, c12, phi12).cpu().numpy())
+    ref = O.object_wave(half, case.dc, case.geometry(), c10, c12, phi12)
+    for obj in objs:
+        assert scaled_error(obj, ref) < OBJ_SCALED
+        assert object_errors(obj, ref)[1] < 1e-4
+    assert scaled_error(objs[1], objs[0]) < OBJ_SCALED
+    assert object_errors(objs[1], objs[0].astype(np.complex128))[1] < 1e-4
+    rms, p999, _ = C.phase_errors(np.angle(objs[1]), np.angle(objs[0]))
+    assert p999 < PHASE_P999
+
+
+@pytest.mark.parametrize("size", [128, 512])
+def test_zero_input_known_answer(cuda, size, golden):
+    za = golden["zero_input"]
+    case = C.make_engine_case(size, 1, g_qk=np.zeros((1, size, size // 2 + 1), np.complex64))
+    case.dc = complex(*za["dc"])
+    eng = engine_for_case(cuda, case)
+    phase = cuda.empty((size, size), dtype=cuda.float32, device="cuda")
+    obj = eng.reconstruct_object(0.0, 0.0, 0.0, phase_out=phase).cpu().numpy()
+    np.testing.assert_allclose(obj.real, 1.0 / size ** 2, atol=2e-7)
+    np.testing.assert_allclose(obj.imag, 0.0, atol=2e-7)
+    assert float(np.abs(phase.cpu().numpy()).max()) < 1e-6
+
+
+def test_rotation_geometry_matches_oracle(cuda):
+    case = C.make_engine_case(128, 7)
+    eng = engine_for_case(cuda, case)
+    for th in (0.0, math.radians(3.0), math.radians(-40.0)):
+        eng.cache_rotation(th)
+        kx, ky, ap = (t.cpu().numpy() for t in eng.geometry())
+        ref = case.geometry(th)
+        np.testing.assert_allclose(kx, ref.kx, atol=1e-7)
+        np.testing.assert_allclose(ky, ref.ky, atol=1e-7)
+        np.testing.assert_allclose(ap, ref.aperture, atol=1e-6)
+
+
+@pytest.mark.parametrize("seed", [123, 124])
+def test_poisson_fixture_end_to_end(cuda, seed, golden):
+    fx = golden["fixtures"]["poisson"]
+    data = C.poisson_data(seed)
+    c10, c12, phi12 = C.MAKE_ENGINE_ABERRATIONS
+    p = O.prepare(data, bf_radius=fx["bf_radius"], **C.POISSON_KW)
+    ref = O.object_wave(p.G, p.dc, p.geom, c10, c12, phi12)
+    eng = engine_for_data(cuda, data, semiangle_mrad=21.4, det_sampling=1.0, step=0.5)
+    d = cuda.from_numpy(data).cuda()
+    phase = cuda.empty((128, 128), dtype=cuda.float32, device="cuda")
+    obj = cuda.empty((128, 128), dtype=cuda.complex64, device="cuda")
+    eng.recon_u16(d, c10, c12, phi12, 0.0, 0.0, fx["bf_radius"], obj_out=obj, phase_out=phase)
+    bf = eng.bf_state()
+    np.testing.assert_array_equal(bf.rows, p.bf.rows)
+    np.testing.assert_array_equal(bf.cols, p.bf.cols)
+    rms, p999, _ = C.phase_errors(phase.cpu().numpy(), np.angle(ref))
+    assert rms < PHASE_RMS and p999 < PHASE_P999
+    # Hermitian (resident) storage vs the same stack expanded to the full plane
+    eng.prepare_g(d)
+    herm_obj = eng.reconstruct_object(c10, c12, phi12).cpu().numpy()
+    full = O.expand_hermitian(eng.G_qk.cpu().numpy())
+    case = C.EngineCase(full, bf.rows, bf.cols, bf.center, bf.dc, 128, (16, 16), None,
+                        p.geom.wavelength, 21.4, (1.0, 1.0), 0.5)
+    full_obj = engine_for_case(cuda, case).reconstruct_object(c10, c12, phi12).cpu().numpy()
+    for o in (herm_obj, full_obj, obj.cpu().numpy()):
+        assert scaled_error(o, ref) < OBJ_SCALED
+        assert object_errors(o, ref)[1] < fx["object_rel_p999"]
+    assert scaled_error(herm_obj, full_obj) < OBJ_SCALED
+    # the end-to-end path (fused column FFT) and the staged path (resident G)
+    assert scaled_error(obj.cpu().numpy(), herm_obj) < 1e-6
+
+
+@pytest.mark.parametrize("scan,det,ab,rot", [
+    (128, 64, {"C10": 200.0}, 0.0),
+    (128, 64, {"C10": 0.0}, 0.0),
+    (128, 64, {"C10": -120.0, "C12": 55.0, "phi12": math.radians(17.0)}, 3.0),
+    (256, 64, {"C10": 150.0}, 0.0),
+    (128, 128, {"C10": 200.0}, -2.0),
+])
+def test_synthetic_recon_matches_fp64_oracle(cuda, scan, det, ab, rot):
+    cfg, data = synthetic(scan, det, ab, rotation_deg=rot)
+    c10, c12, phi12 = ab.get("C10", 0.0), ab.get("C12", 0.0), ab.get("phi12", 0.0)
+    th = math.radians(rot)
+    p = O.prepare(data, voltage_kV=300, semiangle_mrad=20.0, scan_sampling=cfg.step_A,
+                  det_sampling=1.25, rotation_rad=th)
+    ref = np.angle(O.object_wave(p.G, p.dc, p.geom, c10, c12, phi12))
+    eng = engine_for_data(cuda, data, semiangle_mrad=20.0, det_sampling=1.25, step=cfg.step_A)
+    d = cuda.from_numpy(data).cuda()
+    phase = cuda.empty((scan, scan), dtype=cuda.float32, device="cuda")
+    eng.recon_u16(d, c10, c12, phi12, th, 0.0, None, phase_out=phase)
+    got = phase.cpu().numpy()
+    rms, p999, _ = C.phase_errors(got, ref)
+    assert rms < PHASE_RMS and p999 < PHASE_P999
+    # the staged path (resident G) gives the same image
+    eng.mean_dp(d)
+    eng.select_bf(0.0, None)
+    eng.prepare_g(d)
+    eng.cache_rotation(th)
+    staged = cuda.empty((scan, scan), dtype=cuda.float32, device="cuda")
+    eng.reconstruct_object(c10, c12, phi12, phase_out=staged)
+    rms2, _, _ = C.phase_errors(staged.cpu().numpy(), got)
+    assert rms2 < 1e-6
+
+
+def test_full_size_benchmark_config(cuda):
+    """Scan 256x256, detector 128x128, u16, C10 = 200: bit-exact mean DP, phase vs FP64,
+    and a bit-identical repeat (size-independent determinism)."""
+    from quantem.gpu_amd.ssb.synthetic import SyntheticConfig, generate
+    cfg = SyntheticConfig(scan=256, det=128, aberrations={"C10": 200.0}, dose=5e4, seed=2026)
+    d = generate(cfg, device="cuda")
+    host = d.cpu().numpy()
+    eng = engine_for_data(cuda, host, semiangle_mrad=20.0, det_sampling=1.25, step=cfg.step_A)
+    sums, _ = eng.mean_dp(d)
+    rs, _ = O.mean_dp(host)
+    np.testing.assert_array_equal(sums.cpu().numpy().astype(np.uint64), rs)
+    assert int(rs.sum()) == int(d.to(cuda.int64).sum().item())
+    phase = cuda.empty((256, 256), dtype=cuda.float32, device="cuda")
+    again = cuda.empty_like(phase)
+    eng.recon_u16(d, 200.0, 0.0, 0.0, 0.0, 0.0, None, phase_out=phase)
+    eng.recon_u16(d, 200.0, 0.0, 0.0, 0.0, 0.0, None, phase_out=again)
+    assert cuda.equal(phase, again)
+    p = O.prepare(host, voltage_kV=300, semiangle_mrad=20.0, scan_sampling=cfg.step_A,
+                  det_sampling=1.25)
+    assert eng.num_bf == p.bf.rows.size
+    ref = np.angle(O.object_wave(p.G, p.dc, p.geom, 200.0, 0.0, 0.0))
+    rms, p999, _ = C.phase_errors(phase.cpu().numpy(), ref)
+    assert rms < PHASE_RMS and p999 < PHASE_P999
+
+
+# ------------------------------------------------------------- contract
+def test_errors_follow_reference_types(cuda):
+    eng = make_engine(cuda, 128, (16, 16), 8, 0.0197, 20.0, (1.0, 1.0), 0.5)
+    with pytest.raises(ValueError):
+        eng.mean_dp(cuda.zeros((128, 128, 16, 16), dtype=cuda.int32, device="cuda"))
+    with pytest.raises(ValueError):
+        eng.mean_dp(cuda.zeros((128, 128, 8, 8), dtype=cuda.uint16, device="cuda"))
+    with pytest.raises(ValueError):
+        eng.reconstruct_object(0.0, 0.0, 0.0)
+    eng.set_bf([7, 8], [8, 8], (7.5, 7.5), 1.0)
+    with pytest.raises(ValueError, match="G_qk first dimension"):
+        eng.import_g(cuda.zeros((3, 128, 65), dtype=cuda.complex64))
+    with pytest.raises(NotImplementedError):
+        make_engine(cuda, 64, (16, 16), 8, 0.0197, 20.0, (1.0, 1.0), 0.5)
