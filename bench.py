@@ -67,6 +67,20 @@ def cpu_baseline(data_host, cfg) -> dict:
                       f"({dt:.2f} s; FP32 NumPy/SciPy restatement, scipy.fft workers={cores})"}
 
 
+def copy_probe() -> int:
+    """Five launches of an elementwise copy reading and writing 1 GiB each: the
+    reference kernel of known traffic for the FETCH_SIZE/WRITE_SIZE counters."""
+    import torch
+    n = 1 << 28
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,7 +88,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true",
                     help="skip the CPU baseline (used for profiler runs)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB float32 copy kernel (calibrates HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
