@@ -32,6 +32,34 @@ extern "C" {
 
 typedef struct ssb_plan ssb_plan;
 
+/*
+ * Probe disk centre and radius (pixels) from the plan's last mean DP
+ * (ssb_mean_dp_u16): detect_bf_radius of qg/detector/compute/cuda/probe.py:12-123
+ * (threshold_ratio 0.1 there).  The SSB backend derives
+ * det_sampling = 2*semiangle/radius from it when none is given
+ * (cuda/backend.py:271-284).  Synchronises.
+ */
+int ssb_detect_bf_radius(ssb_plan* plan, double threshold_ratio, int* center_row, int* center_col, int* radius,
+                         void* stream);
+
+/*
+ * Supported square scan size for an H x W scan: 128, 256, 512 or 1024, the
+ * next one at or above the longest side (cuda/backend.py:238-246); 0 for an
+ * empty scan.
+ */
+int ssb_scan_target(int H, int W);
+
+/*
+ * Scan normalisation (cuda/backend.py:229-269): centre-crop axes longer than
+ * T (= ssb_scan_target(H, W)) and pad shorter ones with the floor of the
+ * cropped scan's mean frame, at pad_top = (T-h)/2, pad_left = (T-w)/2.
+ * src is [H][W][det_rows][det_cols], dst [T][T][det_rows][det_cols] uint16.
+ * Integer-exact.  Synchronises the stream (it allocates and frees a small
+ * workspace).
+ */
+int ssb_normalize_scan_u16(const uint16_t* src, int H, int W, int det_rows, int det_cols, uint16_t* dst, int T,
+                           void* stream);
+
 /* Library version (major*10000 + minor*100 + patch). */
 int ssb_version(void);
 
@@ -138,6 +166,18 @@ int ssb_import_g(ssb_plan* plan, const void* g_in, int nbf, int gcols, void* str
  */
 int ssb_object(ssb_plan* plan, double c10, double c12, double phi12, void* obj_out, float* phase_out,
                void* stream);
+
+/*
+ * Exact per-BF phase path from the resident Hermitian stack at the cached
+ * rotation: for every BF pixel O_b = IFFT2(G_b conj(Gamma^_b)) with the DC bin
+ * set to dc, phi_b = angle(O_b); phase_out = mean_b phi_b ([N][N] float32)
+ * and loss_out = mean_R(mean_b phi_b^2 - phase^2) (one float32, device).
+ * SSBEngine.reconstruct / reconstruct_with_loss (cuda/engine.py:1153-1191,
+ * 1464-1514; kernels cuda/kernels/common.py:768-816, cuda/engine.py:21-73).
+ * Either output may be NULL.  Deterministic: fixed reduction order.
+ */
+int ssb_phase_loss(ssb_plan* plan, double c10, double c12, double phi12, float* phase_out, float* loss_out,
+                   void* stream);
 
 /*
  * End-to-end plain SSB from resident raw data without host round trips:

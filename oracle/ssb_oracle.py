@@ -204,6 +204,63 @@ def select_bf(mean: np.ndarray, threshold: float = 0.0,
     return BrightField(rows, cols, (center_row, center_col), radius, near_tie)
 
 
+def detect_bf_radius(mean: np.ndarray, threshold_ratio: float = 0.1) -> tuple[tuple[int, int], int]:
+    """Probe disk centre and radius, ``qg/detector/compute/cuda/probe.py:12-123``.
+
+    Centre of mass of ``mean > ratio*max`` (float32 sums of integer
+    coordinates, exact; rounded half-to-even), radial profile over
+    ``rint(|r - centre|)`` bins below the distance to the nearest edge, a
+    13-tap Gaussian (sigma 2) with edge padding, and the first bin under half
+    of the mean of the first five smoothed bins.  Profile sums and the
+    smoothing run in FP64 here.
+    """
+    dp = np.asarray(mean, dtype=np.float32)
+    if dp.ndim != 2:
+        raise ValueError(f"Expected 2D diffraction pattern, got {dp.ndim}D with shape {dp.shape}")
+    R, C = dp.shape
+    if R == 0 or C == 0:
+        raise ValueError(f"Diffraction pattern has zero-size dimension: shape {dp.shape}")
+    with np.errstate(all="ignore"):
+        dmax = float(np.nanmax(dp))
+    if not np.isfinite(dmax) or dmax <= 0:
+        raise ValueError("Diffraction pattern has no positive finite values - cannot detect BF disk. "
+                         "Check that your data is loaded correctly.")
+    mask = dp > np.float32(threshold_ratio * dmax)
+    if not mask.any():
+        raise ValueError(f"No pixels above threshold ({threshold_ratio:.0%} of max intensity). "
+                         "The diffraction pattern may be too noisy or empty.")
+    rr, cc = np.nonzero(mask)
+    total = np.float32(rr.size)
+    rcf = float(np.float32(np.float32(rr.sum()) / total))
+    ccf = float(np.float32(np.float32(cc.sum()) / total))
+    rc = max(0, min(int(round(rcf)), R - 1))
+    col_c = max(0, min(int(round(ccf)), C - 1))
+    max_r = min(rc, col_c, R - rc, C - col_c)
+    if max_r < 2:
+        return (rc, col_c), max(1, min(R, C) // 4)
+    dr = np.arange(R, dtype=np.float32)[:, None] - np.float32(rc)
+    dc = np.arange(C, dtype=np.float32)[None, :] - np.float32(col_c)
+    rint = np.rint(np.sqrt(dr * dr + dc * dc)).astype(np.int64).ravel()
+    flat = dp.astype(np.float64).ravel()
+    valid = rint < max_r
+    profile = np.bincount(rint[valid], weights=flat[valid], minlength=max_r)[:max_r]
+    counts = np.bincount(rint[valid], minlength=max_r)[:max_r]
+    nz = counts > 0
+    profile[nz] /= counts[nz]
+    if profile.size > 5:
+        x = np.arange(13, dtype=np.float64) - 6
+        kern = np.exp(-0.5 * (x / 2.0) ** 2)
+        kern /= kern.sum()
+        padded = np.pad(profile, 6, mode="edge")
+        smooth = np.convolve(padded, kern, mode="valid")[:profile.size]
+        half = smooth[:5].mean() * 0.5
+        below = np.nonzero(smooth < half)[0]
+        radius = int(below[0]) if below.size else int(profile.size) // 2
+    else:
+        radius = min(R, C) // 4
+    return (rc, col_c), max(1, radius)
+
+
 # --------------------------------------------------------------------------
 # Scan Fourier stack (cuda/backend.py:520-589)
 # --------------------------------------------------------------------------

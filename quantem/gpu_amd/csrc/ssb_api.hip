@@ -51,6 +51,12 @@ struct ssb_plan {
   float2* echi = nullptr;
   float2* Gt = nullptr;
   size_t Gt_bytes = 0;
+  // exact per-BF phase path (allocated on first use)
+  float2* T = nullptr;        // column-transformed planes of one BF chunk
+  float2* M1 = nullptr;       // phase moments per BF split, (sum, compensation)
+  float2* M2 = nullptr;
+  float* var_part = nullptr;  // per-block variance sums
+  float* loss = nullptr;      // scalar loss when the caller passes none
   // stage timing
   std::vector<hipEvent_t> events;   // (NSTAGES + 1) per recorded call
   int timing_calls = 0;
@@ -318,6 +324,11 @@ int ssb_plan_destroy(ssb_plan* p) {
   dfree(p->colph);
   dfree(p->echi);
   dfree(p->Gt);
+  dfree(p->T);
+  dfree(p->M1);
+  dfree(p->M2);
+  dfree(p->var_part);
+  dfree(p->loss);
   release_events(p);
   delete p;
   return QSSB_OK;
@@ -524,6 +535,106 @@ int ssb_object(ssb_plan* p, double c10, double c12, double phi12, void* obj_out,
   HIP_TRY(launch_finish_object(p->n, p->S_part, p->info, p->tw, p->S,
                                reinterpret_cast<float2*>(obj_out), phase_out, s),
           "finish_object");
+  return QSSB_OK;
+}
+
+static int alloc_loss(ssb_plan* p) {
+  if (p->T) return QSSB_OK;
+  const size_t nn = (size_t)p->n * p->n;
+  hipError_t e = dmalloc(&p->T, (size_t)loss_chunk_bf(p->n) * nn);
+  if (e == hipSuccess) e = dmalloc(&p->M1, (size_t)loss_splits(p->n) * nn);
+  if (e == hipSuccess) e = dmalloc(&p->M2, (size_t)loss_splits(p->n) * nn);
+  if (e == hipSuccess) e = dmalloc(&p->var_part, (nn + 255) / 256);
+  if (e == hipSuccess) e = dmalloc(&p->loss, 1);
+  if (e != hipSuccess) {
+    dfree(p->T);
+    dfree(p->M1);
+    dfree(p->M2);
+    dfree(p->var_part);
+    dfree(p->loss);
+    p->T = p->M1 = p->M2 = nullptr;
+    p->var_part = p->loss = nullptr;
+    return hip_fail(e, "exact phase workspace allocation");
+  }
+  return QSSB_OK;
+}
+
+int ssb_phase_loss(ssb_plan* p, double c10, double c12, double phi12, float* phase_out, float* loss_out,
+                   void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!p->gcols) return fail(QSSB_ERR_STATE, "no resident Fourier stack");
+  hipStream_t s = as_stream(stream);
+  if (!p->geometry_valid) {
+    rc = ensure_geometry(p, s);
+    if (rc) return rc;
+  }
+  rc = alloc_loss(p);
+  if (rc) return rc;
+  const QuadChi m = quad_chi(p, c10, c12, phi12);
+  HIP_TRY(launch_phasors(p->n, p->info, p->nbf_hint, geom_ptrs(p), m, p->rowph, p->colph, p->echi, s), "phasors");
+  HIP_TRY(launch_phase_moments(p->n, p->Gt, p->gcols, p->info, p->g_nbf, geom_ptrs(p), p->rowph, p->colph, p->echi,
+                               p->optics, p->tw, p->T, p->M1, p->M2, p->var_part, phase_out,
+                               loss_out ? loss_out : p->loss, s),
+          "phase_moments");
+  return QSSB_OK;
+}
+
+int ssb_detect_bf_radius(ssb_plan* p, double threshold_ratio, int* center_row, int* center_col, int* radius,
+                         void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!center_row || !center_col || !radius) return fail(QSSB_ERR_ARG, "output pointers must not be NULL");
+  if (p->det_rows < 1 || p->det_cols < 1 || p->det_rows > 256 || p->det_cols > 256)
+    return fail(QSSB_ERR_UNSUPPORTED, "probe detection supports detectors up to 256 x 256");
+  hipStream_t s = as_stream(stream);
+  int* dout = nullptr;
+  HIP_TRY(dmalloc(&dout, 4), "detect_bf_radius allocation");
+  int host[4] = {0, 0, 0, 0};
+  hipError_t e = launch_detect_bf_radius(p->dp_mean, p->det_rows, p->det_cols, to_df(threshold_ratio), dout, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(host, dout, sizeof(host), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  dfree(dout);
+  if (e != hipSuccess) return hip_fail(e, "detect_bf_radius");
+  if (host[0] == 1)
+    return fail(QSSB_ERR_ARG, "Diffraction pattern has no positive finite values - cannot detect BF disk. "
+                              "Check that your data is loaded correctly.");
+  if (host[0] == 2)
+    return fail(QSSB_ERR_ARG, "No pixels above threshold. The diffraction pattern may be too noisy or empty.");
+  *center_row = host[1];
+  *center_col = host[2];
+  *radius = host[3];
+  return QSSB_OK;
+}
+
+int ssb_scan_target(int H, int W) {
+  if (H < 1 || W < 1) return 0;
+  const int longest = H > W ? H : W;
+  return longest <= 128 ? 128 : (longest <= 256 ? 256 : (longest <= 512 ? 512 : 1024));
+}
+
+int ssb_normalize_scan_u16(const uint16_t* src, int H, int W, int det_rows, int det_cols, uint16_t* dst, int T,
+                           void* stream) {
+  g_last_error.clear();
+  if (!src || !dst) return fail(QSSB_ERR_ARG, "source and destination must not be NULL");
+  if (H < 1 || W < 1 || det_rows < 1 || det_cols < 1) return fail(QSSB_ERR_ARG, "shapes must be positive");
+  if (T != ssb_scan_target(H, W))
+    return fail(QSSB_ERR_ARG, "target scan size " + std::to_string(T) + " is not the supported size for a " +
+                                  std::to_string(H) + " x " + std::to_string(W) + " scan (" +
+                                  std::to_string(ssb_scan_target(H, W)) + ")");
+  const long long D = (long long)det_rows * det_cols;
+  if (D > (1 << 24)) return fail(QSSB_ERR_UNSUPPORTED, "detector too large");
+  hipStream_t s = as_stream(stream);
+  uint32_t* partial = nullptr;
+  uint16_t* mean = nullptr;
+  hipError_t e = dmalloc(&partial, (size_t)normalize_partial_groups(H, T) * D);
+  if (e == hipSuccess) e = dmalloc(&mean, (size_t)D + 8);
+  if (e == hipSuccess) e = launch_normalize_scan(src, H, W, (int)D, dst, T, partial, mean, s);
+  hipError_t e2 = hipStreamSynchronize(s);
+  dfree(partial);
+  dfree(mean);
+  if (e != hipSuccess) return hip_fail(e, "normalize_scan");
+  if (e2 != hipSuccess) return hip_fail(e2, "normalize_scan sync");
   return QSSB_OK;
 }
 

@@ -61,12 +61,19 @@ hipError_t launch_mean_dp_reduce(long long frames, int D, const uint32_t* partia
 hipError_t launch_select_bf(const float* mean, const unsigned long long* sums, int det_rows, int det_cols,
                             float threshold, float bf_radius, int max_bf, int* rows, int* cols, BfInfo* info,
                             hipStream_t s);
+// Probe disk centre and radius of a float32 mean DP; out[4] = {status, row, col, radius}.
+hipError_t launch_detect_bf_radius(const float* mean, int R, int C, df ratio, int* out, hipStream_t s);
 hipError_t launch_bf_geometry(const int* rows, const int* cols, const BfInfo* info, int max_bf, df recip_y,
                               df recip_x, df cos_a, df sin_a, const Optics& o, float* kx, float* kxl, float* ky,
                               float* kyl, float* ak, hipStream_t s);
 hipError_t launch_gather_rowfft(int N, const uint16_t* data, int D, int det_cols, const int* rows,
                                 const int* cols, const BfInfo* info, int nbf_hint, const float2* tw, float2* H,
                                 hipStream_t s);
+// Scan normalisation: crop/pad the H x W scan to T x T (partial holds
+// normalize_partial_groups(H, T) * D uint32, mean D uint16).
+hipError_t launch_normalize_scan(const uint16_t* src, int H, int W, int D, uint16_t* dst, int T,
+                                 uint32_t* partial, uint16_t* mean, hipStream_t s);
+int normalize_partial_groups(int H, int T);
 
 // Constants of the isotropic aperture evaluated from d = Rout^2 - |u|^2.
 struct ApIso {
@@ -102,5 +109,17 @@ hipError_t launch_trotter_full(int N, const float2* Gt, const BfInfo* info, cons
 hipError_t launch_finish_object(int N, const float2* S_part, const BfInfo* info, const float2* tw, float2* S,
                                 float2* obj, float* phase, hipStream_t s);
 
+// Exact per-BF phase path: BF pixels per chunk of column-transformed planes,
+// BF splits of the moment planes.
+int loss_chunk_bf(int N);
+int loss_splits(int N);
+// Mean of per-BF phases and the phase-variance loss from the resident
+// stack Gt[b][c][q] (gcols = N/2+1 Hermitian, or N full-plane); T holds loss_chunk_bf(N) planes, M1/M2
+// loss_splits(N) planes, var_part N*N/256 floats.  phase may be NULL; loss
+// is a device scalar.
+hipError_t launch_phase_moments(int N, const float2* Gt, int gcols, const BfInfo* info, int nbf_hint, const GeomPtrs& gp,
+                                const float2* rowph, const float2* colph, const float2* echi, const Optics& o,
+                                const float2* tw, float2* T, float2* M1, float2* M2, float* var_part,
+                                float* phase, float* loss, hipStream_t s);
 
 }  // namespace qssb

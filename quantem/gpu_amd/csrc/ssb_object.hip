@@ -9,35 +9,11 @@
 //   object + IFFT         src/quantem/gpu/ssb/compute/cuda/engine.py:1047-1107
 //   phase = angle(object) src/quantem/gpu/ssb/results.py:117-122
 #include "ssb_internal.h"
+#include "ssb_overlap.h"
 
 namespace qssb {
 
 int object_groups(int N) { return N == 128 ? 64 : (N == 256 ? 32 : 8); }
-
-// Balanced split of [0, nbf) into `groups` contiguous ranges.
-__device__ __forceinline__ void group_range(int g, int groups, int nbf, int& b0, int& b1) {
-  b0 = (int)(((long long)g * nbf) / groups);
-  b1 = (int)(((long long)(g + 1) * nbf) / groups);
-}
-
-__device__ __forceinline__ float chi_quad(const QuadChi& m, float x, float y) {
-  return m.mxx * x * x + 2.0f * m.mxy * x * y + m.myy * y * y;
-}
-
-// Aperture from d = Rout^2 - |u|^2 (double-float), where Rout is the radius
-// at which the soft edge reaches 0.  For isotropic detector sampling the
-// reference edge (semi - lambda|u|)/ang + 1/2 equals (Rout - |u|)*lambda/ang
-// = d * (lambda/ang) / (Rout + |u|): the cancellation sits entirely in d,
-// which is exact to ~1e-13 here, so ties (d == 0, A(Q-k) == A(Q+k)) resolve
-// as exact arithmetic resolves them.
-__device__ __forceinline__ float aperture_from_d(df d, const ApIso& a) {
-  const float dh = d.hi + d.lo;
-  if (dh >= a.d_in) return 1.0f;
-  if (dh <= a.d_out) return 0.0f;
-  const float r = sqrtf(fmaxf(a.rout2 - dh, 0.0f));
-  const float e = dh * a.scale / (a.rout + r);
-  return fminf(fmaxf(e, 0.0f), 1.0f);
-}
 
 // Phasor tables of the factorised overlap e^{i theta} = e^{i qx a_b} e^{i qy b_b},
 // (a_b, b_b) = 2 M k_b: rowph[b][q] for all rows, colph[b][c] for columns
@@ -74,23 +50,6 @@ k_phasors(const BfInfo* __restrict__ info, GeomPtrs gp, QuadChi m, int N, float2
     }
   }
 }
-
-// Aperture pair A(Q - k), A(Q + k).  ISO: from d = (Rout^2 - |k|^2) - |Q|^2
-// +/- 2 Q.k; otherwise the general double-float soft-edge expression.
-template <bool ISO>
-__device__ __forceinline__ void aperture_pair(df qx, df qy, df kx, df ky, df cb, df q2, const ApIso& ap,
-                                              const Optics& o, float& am, float& app) {
-  if constexpr (ISO) {
-    const df qk2 = df_mul(df_add(df_mul(qx, kx), df_mul(qy, ky)), df{2.0f, 0.0f});
-    const df base = df_sub(cb, q2);
-    am = aperture_from_d(df_add(base, qk2), ap);
-    app = aperture_from_d(df_sub(base, qk2), ap);
-  } else {
-    am = aperture_at<false>(qx.hi, qx.lo, qy.hi, qy.lo, kx.hi, kx.lo, ky.hi, ky.lo, o);
-    app = aperture_at<true>(qx.hi, qx.lo, qy.hi, qy.lo, kx.hi, kx.lo, ky.hi, ky.lo, o);
-  }
-}
-
 
 // ------------------------------------------------------------------------
 // Trotter over Hermitian slot columns.  Work item = (BF group g, slot group

@@ -520,6 +520,145 @@ hipError_t launch_mean_dp(const uint16_t* data, long long frames, int D, uint32_
   return launch_mean_dp_reduce(frames, D, partial, max_ranges, sums, mean, s);
 }
 
+// ------------------------------------------------------------------------
+// Probe disk detection (qg/detector/compute/cuda/probe.py:12-123): centre of
+// mass of `mean > ratio*max`, rounded half-to-even; radial profile of
+// rint(|r - centre|) bins up to the nearest edge; Gaussian smoothing
+// (sigma 2, 13 taps, edge padding); radius = first bin below half of the
+// mean of the first five smoothed bins.  One workgroup; every sum runs in a
+// fixed order.  out = {status, centre row, centre col, radius}; status 1: no
+// positive finite maximum, 2: nothing above the threshold.
+// ------------------------------------------------------------------------
+#define RADIUS_BINS 128
+#define RADIUS_CHUNKS (MASK_THREADS / RADIUS_BINS)
+
+__global__ void __launch_bounds__(MASK_THREADS)
+k_detect_bf_radius(const float* __restrict__ mean, int R, int C, float ratio_hi, float ratio_lo,
+                   int* __restrict__ out) {
+  __shared__ float sh_f[2 * MASK_WAVES];
+  __shared__ long long sh_i[3 * MASK_WAVES];
+  __shared__ float part_s[RADIUS_BINS * RADIUS_CHUNKS];
+  __shared__ int part_n[RADIUS_BINS * RADIUS_CHUNKS];
+  __shared__ float prof[RADIUS_BINS];
+  __shared__ float smooth[RADIUS_BINS];
+  const int tid = threadIdx.x;
+  const int D = R * C;
+  const int per = (D + MASK_THREADS - 1) / MASK_THREADS;
+  const int p0 = tid * per;
+  const int p1 = min(D, p0 + per);
+  float lmax = -INFINITY;
+  for (int i = p0; i < p1; ++i) lmax = fmaxf(lmax, mean[i]);
+  const float mmax = block_max(lmax, sh_f);
+  if (!(mmax > 0.0f) || isinf(mmax)) {
+    if (tid == 0) out[0] = 1;
+    return;
+  }
+  // float32 of (ratio * max) with ratio given as a double-float pair
+  const df pr = two_prod(ratio_hi, mmax);
+  const float thr = pr.hi + (pr.lo + ratio_lo * mmax);
+  long long n = 0, sr = 0, sc = 0;
+  for (int i = p0; i < p1; ++i) {
+    if (mean[i] > thr) {
+      n += 1;
+      sr += i / C;
+      sc += i % C;
+    }
+  }
+  block_sum3_i64(n, sr, sc, sh_i);
+  if (n == 0) {
+    if (tid == 0) out[0] = 2;
+    return;
+  }
+  // float32 sums of integer coordinates are exact here (< 2^24)
+  const float rcf = (float)sr / (float)n;
+  const float ccf = (float)sc / (float)n;
+  const int rc = min(max((int)rintf(rcf), 0), R - 1);
+  const int cc = min(max((int)rintf(ccf), 0), C - 1);
+  const int max_r = min(min(rc, cc), min(R - rc, C - cc));
+  int radius;
+  if (max_r < 2) {
+    radius = max(1, min(R, C) / 4);
+  } else {
+    const int bins = min(max_r, RADIUS_BINS);
+    {
+      // thread = (bin, chunk of rows); raster order within the chunk
+      const int b = tid % RADIUS_BINS, k = tid / RADIUS_BINS;
+      const int ra = (int)(((long long)k * R) / RADIUS_CHUNKS), rb = (int)(((long long)(k + 1) * R) / RADIUS_CHUNKS);
+      float s = 0.0f;
+      int cnt = 0;
+      if (b < bins) {
+        for (int r = ra; r < rb; ++r) {
+          const float dr = (float)(r - rc);
+          for (int c = 0; c < C; ++c) {
+            const float dc = (float)(c - cc);
+            const int ri = (int)rintf(sqrtf(dr * dr + dc * dc));
+            if (ri == b) {
+              s += mean[r * C + c];
+              cnt += 1;
+            }
+          }
+        }
+      }
+      part_s[b * RADIUS_CHUNKS + k] = s;
+      part_n[b * RADIUS_CHUNKS + k] = cnt;
+    }
+    __syncthreads();
+    if (tid < bins) {
+      float s = 0.0f;
+      int cnt = 0;
+      for (int k = 0; k < RADIUS_CHUNKS; ++k) {
+        s += part_s[tid * RADIUS_CHUNKS + k];
+        cnt += part_n[tid * RADIUS_CHUNKS + k];
+      }
+      prof[tid] = cnt > 0 ? s / (float)cnt : s;
+    }
+    __syncthreads();
+    if (bins > 5) {
+      if (tid < bins) {
+        float wsum = 0.0f, acc = 0.0f;
+        for (int t = 0; t < 13; ++t) {
+          const float x = (float)(t - 6) / 2.0f;
+          wsum += expf(-0.5f * x * x);
+        }
+        for (int t = 0; t < 13; ++t) {
+          const float x = (float)(t - 6) / 2.0f;
+          const int src = min(max(tid + t - 6, 0), bins - 1);
+          acc += prof[src] * (expf(-0.5f * x * x) / wsum);
+        }
+        smooth[tid] = acc;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        const float centre = (smooth[0] + smooth[1] + smooth[2] + smooth[3] + smooth[4]) / 5.0f;
+        const float half = centre * 0.5f;
+        int r = -1;
+        for (int i = 0; i < bins; ++i) {
+          if (smooth[i] < half) {
+            r = i;
+            break;
+          }
+        }
+        sh_i[0] = (r >= 0) ? r : bins / 2;
+      }
+      __syncthreads();
+      radius = (int)sh_i[0];
+    } else {
+      radius = min(R, C) / 4;
+    }
+  }
+  if (tid == 0) {
+    out[0] = 0;
+    out[1] = rc;
+    out[2] = cc;
+    out[3] = max(1, radius);
+  }
+}
+
+hipError_t launch_detect_bf_radius(const float* mean, int R, int C, df ratio, int* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_detect_bf_radius, dim3(1), dim3(MASK_THREADS), 0, s, mean, R, C, ratio.hi, ratio.lo, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_select_bf(const float* mean, const unsigned long long* sums, int det_rows, int det_cols,
                             float threshold, float bf_radius, int max_bf, int* rows, int* cols,
                             BfInfo* info, hipStream_t s) {
@@ -562,6 +701,95 @@ hipError_t launch_gather_rowfft(int N, const uint16_t* data, int D, int det_cols
       return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
+// Scan normalisation (cuda/backend.py:229-269): centre-crop oversize axes to
+// the window (r0, c0, h, w) of the source, pad to T x T with the window's
+// mean frame floor(sum / (h*w)) -- the reference's int64 sum, float64 divide
+// and truncating cast give exactly the integer floor for these ranges --
+// placed at pad_top = (T - h)/2, pad_left = (T - w)/2.
+// ------------------------------------------------------------------------
+constexpr int kWindowRows = 64;   // window rows per partial sum: 64*1024*65535 < 2^32
+
+__global__ void __launch_bounds__(256)
+k_window_partial(const uint16_t* __restrict__ src, int W, int D, int r0, int c0, int h, int w,
+                 uint32_t* __restrict__ partial) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= D) return;
+  const int ra = blockIdx.y * kWindowRows;
+  const int rb = min(h, ra + kWindowRows);
+  uint32_t acc = 0u;
+  for (int r = ra; r < rb; ++r) {
+    const uint16_t* row = src + ((long long)(r0 + r) * W + c0) * D + p;
+    for (int c = 0; c < w; ++c) acc += row[(long long)c * D];
+  }
+  partial[(long long)blockIdx.y * D + p] = acc;
+}
+
+__global__ void __launch_bounds__(256)
+k_window_mean(const uint32_t* __restrict__ partial, int groups, int D, unsigned long long count,
+              uint16_t* __restrict__ mean) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= D) return;
+  unsigned long long s = 0ull;
+  for (int g = 0; g < groups; ++g) s += partial[(long long)g * D + p];
+  mean[p] = (uint16_t)(s / count);
+}
+
+template <int VEC>
+__global__ void __launch_bounds__(256)
+k_pad_copy(const uint16_t* __restrict__ src, int W, int D, int r0, int c0, int h, int w, int T, int pad_top,
+           int pad_left, const uint16_t* __restrict__ mean, uint16_t* __restrict__ dst) {
+  const long long frame = blockIdx.y + (long long)blockIdx.z * 65535;
+  if (frame >= (long long)T * T) return;
+  const int i = (int)(frame / T), j = (int)(frame % T);
+  const int si = i - pad_top, sj = j - pad_left;
+  const bool inside = si >= 0 && si < h && sj >= 0 && sj < w;
+  const uint16_t* from = inside ? src + ((long long)(r0 + si) * W + (c0 + sj)) * D : mean;
+  uint16_t* to = dst + frame * D;
+  const int e = (blockIdx.x * 256 + threadIdx.x) * VEC;
+  if (e >= D) return;
+  if constexpr (VEC == 8) {
+    *reinterpret_cast<u32x4*>(to + e) = *reinterpret_cast<const u32x4*>(from + e);
+  } else {
+    to[e] = from[e];
+  }
+}
+
+hipError_t launch_normalize_scan(const uint16_t* src, int H, int W, int D, uint16_t* dst, int T,
+                                 uint32_t* partial, uint16_t* mean, hipStream_t s) {
+  const int r0 = H > T ? (H - T) / 2 : 0;
+  const int c0 = W > T ? (W - T) / 2 : 0;
+  const int h = H < T ? H : T;
+  const int w = W < T ? W : T;
+  const int pad_top = (T - h) / 2, pad_left = (T - w) / 2;
+  if (h < T || w < T) {
+    const int groups = (h + kWindowRows - 1) / kWindowRows;
+    hipLaunchKernelGGL(k_window_partial, dim3((D + 255) / 256, groups), dim3(256), 0, s, src, W, D, r0, c0, h, w,
+                       partial);
+    hipLaunchKernelGGL(k_window_mean, dim3((D + 255) / 256), dim3(256), 0, s, partial, groups, D,
+                       (unsigned long long)h * (unsigned long long)w, mean);
+  }
+  const long long frames = (long long)T * T;
+  const dim3 grid_f(1, (unsigned)(frames < 65535 ? frames : 65535), (unsigned)((frames + 65534) / 65535));
+  const bool vec = (D % 8 == 0) && ((reinterpret_cast<uintptr_t>(src) & 15u) == 0) &&
+                   ((reinterpret_cast<uintptr_t>(dst) & 15u) == 0) && ((reinterpret_cast<uintptr_t>(mean) & 15u) == 0);
+  if (vec) {
+    dim3 g = grid_f;
+    g.x = (D / 8 + 255) / 256;
+    hipLaunchKernelGGL(k_pad_copy<8>, g, dim3(256), 0, s, src, W, D, r0, c0, h, w, T, pad_top, pad_left, mean, dst);
+  } else {
+    dim3 g = grid_f;
+    g.x = (D + 255) / 256;
+    hipLaunchKernelGGL(k_pad_copy<1>, g, dim3(256), 0, s, src, W, D, r0, c0, h, w, T, pad_top, pad_left, mean, dst);
+  }
+  return hipGetLastError();
+}
+
+int normalize_partial_groups(int H, int T) {
+  const int h = H < T ? H : T;
+  return (h + kWindowRows - 1) / kWindowRows;
 }
 
 }  // namespace qssb

@@ -54,8 +54,11 @@ SIGNATURES: dict[str, list] = {
     "ssb_stage_count": [],
     "ssb_timing_begin": [_vp, _c_int],
     "ssb_timing_end": [_vp, _fp, _c_int, ctypes.POINTER(_c_int)],
-    "ssb_phase_loss": [_vp, _vp, _c_int, _vp, _vp, _vp],
-    "ssb_batch_eval": [_vp, _c_int, _vp, _c_int, _c_int, _vp, _vp],
+    "ssb_phase_loss": [_vp, _c_double, _c_double, _c_double, _vp, _vp, _vp],
+    "ssb_scan_target": [_c_int, _c_int],
+    "ssb_detect_bf_radius": [_vp, _c_double, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int),
+                             ctypes.POINTER(_c_int), _vp],
+    "ssb_normalize_scan_u16": [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _vp],
 }
 
 _lib: ctypes.CDLL | None = None

@@ -1,0 +1,456 @@
+"""``HipSSBBackend``: the MI355X SSB session behind the reference's backend
+contract.
+
+Mirrors ``CudaSSBBackend`` (``cuda/backend.py:88-1420``): the same
+constructor arguments, properties and methods, the same defaults, units
+(aberrations in the reference's numeric convention, ``phi12`` in radians,
+public rotation in degrees) and error types.  All computation runs in the
+native library through ``HipSSBEngine``; this class only sequences calls,
+keeps the session state and builds results.  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import contextlib
+import math
+import pathlib
+import time
+from typing import Literal
+
+import numpy as np
+
+from ...types import BrightfieldDisk, SSBExportState, SSBPrecision, SSBResult
+from . import optimizer as _opt
+from .engine import HipSSBEngine, PlanSpec, SSBPlan, normalize_scan, scan_target
+
+
+def _wavelength_angstrom(energy_eV: float) -> float:
+    from ....optics import electron_wavelength_angstrom
+    return electron_wavelength_angstrom(energy_eV)
+
+
+class HipSSBBackend:
+    """SSB reconstruction on one MI355X (see module docstring)."""
+
+    _DEFAULT_OPTIMIZE_RANGES = dict(_opt.DEFAULT_RANGES)
+    backend = "hip"
+    precision = SSBPrecision()
+
+    def __init__(self, data, semiangle: float, scan_sampling: float | tuple[float, float],
+                 det_sampling: float | tuple[float, float] | None = None, *, voltage_kV: float | None = None,
+                 energy: float | None = None, scan_shape: tuple[int, int] | None = None,
+                 bf_intensity_threshold: float = 0.0, bf_radius: float | None = None,
+                 aberrations: dict[str, float] | None = None, rotation_angle_deg: float = 0.0,
+                 detector_gain=None, gqk_storage: Literal["herm"] = "herm", device=None):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("The hip SSB backend requires a ROCm GPU; no CPU fallback exists.")
+        if voltage_kV is not None and energy is not None:
+            raise ValueError("Specify voltage_kV or energy, not both.")
+        if voltage_kV is not None:
+            energy = voltage_kV * 1e3
+        elif energy is None:
+            raise ValueError("Specify voltage_kV (kV) or energy (eV).")
+        if gqk_storage != "herm":
+            raise ValueError(
+                "gqk_storage='full' was removed from the SSB runtime path. Use the default exact "
+                "Hermitian storage; low-level tests may construct canonical full-plane references directly.")
+        if detector_gain is not None:
+            raise NotImplementedError("detector_gain is not supported by the hip SSB backend yet.")
+
+        dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        if dev.index is None:
+            dev = torch.device("cuda", torch.cuda.current_device())
+        data = self._as_device_u16(data, dev)
+        if data.ndim == 3:
+            if scan_shape is None:
+                n = int(data.shape[0])
+                side = int(n ** 0.5)
+                if side * side != n:
+                    raise ValueError(f"scan_shape is required: {n} frames is not a perfect square. "
+                                     f"Pass scan_shape=(rows, cols).")
+                scan_shape = (side, side)
+            if scan_shape[0] * scan_shape[1] != data.shape[0]:
+                raise ValueError("scan_shape does not match number of frames.")
+            data = data.reshape(scan_shape[0], scan_shape[1], data.shape[1], data.shape[2])
+        elif data.ndim != 4:
+            raise ValueError("data must be 3D or 4D.")
+        data = normalize_scan(data.contiguous())
+
+        if isinstance(scan_sampling, (int, float)):
+            scan_sampling = (float(scan_sampling), float(scan_sampling))
+        wavelength = _wavelength_angstrom(energy)
+        det_gpts = (int(data.shape[2]), int(data.shape[3]))
+        scan_gpts = (int(data.shape[0]), int(data.shape[1]))
+        if scan_gpts[0] not in (128, 256, 512):
+            raise NotImplementedError(f"scan size {scan_gpts[0]} is not supported by the hip SSB backend; "
+                                      "use 128, 256 or 512")
+        self._detected_bf_radius: float | None = None
+        if det_sampling is None:
+            radius = self._detect_radius(data, dev, det_gpts, scan_gpts, wavelength)
+            self._detected_bf_radius = float(radius)
+            det_sampling = (2 * semiangle) / radius
+            det_sampling = (det_sampling, det_sampling)
+        elif isinstance(det_sampling, (int, float)):
+            det_sampling = (float(det_sampling), float(det_sampling))
+        if aberrations is None:
+            aberrations = {"C10": 0.0, "C12": 0.0, "phi12": 0.0}
+
+        self.energy = energy
+        self.voltage_kV = voltage_kV
+        self.semiangle_mrad = semiangle
+        self.semiangle_cutoff = semiangle
+        self.scan_sampling = tuple(scan_sampling)
+        self.angular_sampling = tuple(det_sampling)
+        self.bf_intensity_threshold = float(bf_intensity_threshold)
+        self.aberrations = {"C10": 0.0, "C12": 0.0, "phi12": 0.0, **aberrations}
+        self._rotation_angle_rad = math.radians(rotation_angle_deg)
+        self.gqk_storage = gqk_storage
+        self.gpts = det_gpts
+        self.wavelength = wavelength
+        recip = (det_sampling[0] * 1e-3 / wavelength, det_sampling[1] * 1e-3 / wavelength)
+        self.sampling = (1.0 / (recip[0] * det_gpts[0]), 1.0 / (recip[1] * det_gpts[1]))
+        self._scan_shape = scan_gpts
+
+        spec = PlanSpec(scan_gpts[0], det_gpts[0], det_gpts[1], det_gpts[0] * det_gpts[1], wavelength,
+                        semiangle * 1e-3, det_sampling[0] * 1e-3, det_sampling[1] * 1e-3,
+                        self.scan_sampling[0], self.scan_sampling[1])
+        self._plan = SSBPlan(spec, device=dev)
+        self._engine = HipSSBEngine(self._plan)
+        self._engine.mean_dp(data)
+        bf = self._engine.select_bf(self.bf_intensity_threshold, bf_radius)
+        self._engine.prepare_g(data)
+        self._engine.cache_rotation(self._rotation_angle_rad)
+        del data
+        self.bf_inds_row = bf.rows
+        self.bf_inds_col = bf.cols
+        self.bf_center = bf.center
+        self.dc_value = bf.dc
+        self._bf_radius_arg = bf_radius
+
+        self._best_loss = float("inf")
+        self._elapsed_optimize = 0.0
+        self._elapsed_grid = 0.0
+        self._elapsed_refine = 0.0
+        self._refine_method: str | None = None
+        self._refine_nfev: int | None = None
+        self._n_trials: int | None = None
+        self._optuna_trials: list[dict] = []
+        self._optimizer_objective_mode = "exact"
+
+    # ------------------------------------------------------------- helpers
+    @staticmethod
+    def _as_device_u16(data, dev):
+        import torch
+        if isinstance(data, torch.Tensor):
+            if data.dtype != torch.uint16:
+                raise NotImplementedError("the hip SSB backend reads uint16 detector counts; "
+                                          f"got {data.dtype}")
+            return data.to(dev)
+        arr = np.asarray(data)
+        if arr.dtype != np.uint16:
+            raise NotImplementedError(f"the hip SSB backend reads uint16 detector counts; got {arr.dtype}")
+        return torch.from_numpy(np.ascontiguousarray(arr)).to(dev)
+
+    @staticmethod
+    def _detect_radius(data, dev, det_gpts, scan_gpts, wavelength) -> int:
+        spec = PlanSpec(scan_gpts[0], det_gpts[0], det_gpts[1], 1, wavelength, 1e-3, 1e-3, 1e-3, 1.0, 1.0)
+        eng = HipSSBEngine(SSBPlan(spec, device=dev))
+        eng.mean_dp(data)
+        (_, _), radius = eng.detect_bf_radius()
+        eng.plan.close()
+        return radius
+
+    def _resolve_coefs(self, C10, C12, phi12):
+        if C10 is None:
+            C10 = self.aberrations.get("C10", 0.0)
+        if C12 is None:
+            C12 = self.aberrations.get("C12", 0.0)
+        if phi12 is None:
+            phi12 = self.aberrations.get("phi12", 0.0)
+        return C10, C12, phi12
+
+    def _prepare(self, rotation_angle_rad=None) -> HipSSBEngine:
+        if self._engine is None:
+            raise ValueError("this SSB session is closed")
+        if rotation_angle_rad is None:
+            rotation_angle_rad = self._rotation_angle_rad
+        self._engine.cache_rotation(float(rotation_angle_rad))
+        return self._engine
+
+    # ---------------------------------------------------------- properties
+    @property
+    def scan_shape(self) -> tuple[int, int]:
+        return tuple(int(v) for v in self._scan_shape)
+
+    @property
+    def detector_shape(self) -> tuple[int, int]:
+        return tuple(int(v) for v in self.gpts)
+
+    @property
+    def num_bf(self) -> int:
+        return int(self.bf_inds_row.size)
+
+    @property
+    def G_qk(self):
+        """Resident Hermitian Fourier stack ``[b, q_row, 0..N/2]`` (complex64, GPU)."""
+        return self._engine.G_qk
+
+    # ------------------------------------------------------ reconstruction
+    def cache_rotation(self, rotation_rad: float) -> None:
+        self._rotation_angle_rad = float(rotation_rad)
+        self._prepare()
+
+    def reconstruct(self, c10: float, c12: float, phi12: float):
+        return self._prepare().reconstruct(c10, c12, phi12)
+
+    def reconstruct_with_loss(self, c10: float, c12: float, phi12: float):
+        return self._prepare().reconstruct_with_loss(c10, c12, phi12)
+
+    def reconstruct_full(self, mags_m, angles_rad):
+        return self._prepare().reconstruct_full(mags_m, angles_rad)
+
+    def reconstruct_full_with_loss(self, mags_m, angles_rad):
+        return self._prepare().reconstruct_full_with_loss(mags_m, angles_rad)
+
+    @staticmethod
+    def phase_to_numpy(phase) -> np.ndarray:
+        import torch
+        if isinstance(phase, torch.Tensor):
+            return phase.detach().cpu().numpy().astype(np.float32, copy=False)
+        return np.asarray(phase, dtype=np.float32)
+
+    def reconstruct_phase(self, C10=None, C12=None, phi12=None, rotation_angle_rad=None):
+        """Mean per-BF phase image (``cuda/backend.py:719-728``)."""
+        C10, C12, phi12 = self._resolve_coefs(C10, C12, phi12)
+        return self._prepare(rotation_angle_rad).reconstruct(C10, C12, phi12)
+
+    def reconstruct_object(self, C10=None, C12=None, phi12=None, rotation_angle_rad=None):
+        """Complex object of the Fourier-sum path (``cuda/backend.py:929-942``)."""
+        C10, C12, phi12 = self._resolve_coefs(C10, C12, phi12)
+        return self._prepare(rotation_angle_rad).reconstruct_object(C10, C12, phi12)
+
+    def _ho_arrays_from_aberrations(self):
+        """(mags[14], angles_rad[14], any_higher_order) of ``cuda/backend.py:730-769``."""
+        a = self.aberrations
+        mags = np.zeros(14, dtype=np.float64)
+        angs = np.zeros(14, dtype=np.float64)
+        mags[0] = float(a.get("C10", 0.0))
+        mags[1] = float(a.get("C12", 0.0))
+        angs[1] = float(a.get("phi12", 0.0))
+        layout = [("C21", 2, True), ("C23", 3, True), ("C30", 4, False), ("C32", 5, True), ("C34", 6, True),
+                  ("C41", 7, True), ("C43", 8, True), ("C45", 9, True), ("C50", 10, False),
+                  ("C52", 11, True), ("C54", 12, True), ("C56", 13, True)]
+        active = False
+        for name, idx, has_angle in layout:
+            if has_angle:
+                mag = float(a.get(f"{name}_mag", 0.0))
+                ang = float(a.get(f"{name}_angle", 0.0))
+            else:
+                mag, ang = float(a.get(name, 0.0)), 0.0
+            if mag != 0.0:
+                active = True
+            mags[idx] = mag
+            angs[idx] = math.radians(ang)
+        return mags, angs, active
+
+    def _compute_loss(self) -> float:
+        return float(self._prepare().reconstruct_with_loss(
+            self.aberrations["C10"], self.aberrations["C12"], self.aberrations["phi12"])[1])
+
+    def result(self, *, compute_loss: bool = True) -> SSBResult:
+        """Reconstruct with the current aberrations (``cuda/backend.py:771-851``)."""
+        import torch
+        mags, angs, ho_active = self._ho_arrays_from_aberrations()
+        try:
+            if ho_active:
+                phase = self._prepare().reconstruct_full(mags, angs)
+                obj = torch.polar(torch.ones_like(phase), phase).to(torch.complex64)
+            else:
+                obj = self.reconstruct_object()
+        except torch.cuda.OutOfMemoryError:
+            free = torch.cuda.mem_get_info(self._plan.device)[0] / 1e9
+            raise MemoryError(
+                f"Out of GPU VRAM during SSB reconstruction ({self.num_bf} BF pixels, {free:.1f} GB free).\n"
+                "Try: SSB(..., bf_radius=<smaller>) to reduce BF pixel count, "
+                "or restart the kernel to free stale GPU memory.") from None
+        loss = None
+        if compute_loss and not ho_active:
+            try:
+                loss = self._compute_loss()
+            except (ValueError, ZeroDivisionError, FloatingPointError):
+                loss = None
+        elapsed = self._elapsed_optimize + self._elapsed_grid + self._elapsed_refine
+        bf = self.browser_state().brightfield
+        return SSBResult(
+            object_wave=obj, backend="hip", aberrations=self.aberrations.copy(),
+            rotation_angle_deg=math.degrees(self._rotation_angle_rad), loss=loss,
+            elapsed=elapsed if elapsed > 0 else None, n_trials=self._n_trials, num_bf=self.num_bf,
+            refine_method=self._refine_method, refine_nfev=self._refine_nfev,
+            refine_elapsed=self._elapsed_refine if self._elapsed_refine > 0 else None,
+            voltage_kV=self.voltage_kV, semiangle_mrad=self.semiangle_mrad,
+            scan_sampling_A=self.scan_sampling[0], bf_center=bf.center_row_col, bf_radius=bf.radius_px,
+            detected_bf_radius=bf.detected_radius_px, optuna_trials=self._optuna_trials)
+
+    def reconstruct_result(self, aberrations: dict[str, float], *, compute_loss: bool = True) -> SSBResult:
+        self.aberrations.update(aberrations)
+        return self.result(compute_loss=compute_loss)
+
+    def preview(self, aberrations: dict[str, float], *, compute_loss: bool,
+                higher_order_magnitudes: np.ndarray | None, higher_order_angles: np.ndarray | None):
+        """One transient float32 phase and optional exact loss (``cuda/backend.py:892-922``)."""
+        if higher_order_magnitudes is not None:
+            if compute_loss:
+                phase, loss = self.reconstruct_full_with_loss(higher_order_magnitudes, higher_order_angles)
+            else:
+                phase, loss = self.reconstruct_full(higher_order_magnitudes, higher_order_angles), None
+        elif compute_loss:
+            phase, loss = self.reconstruct_with_loss(aberrations["C10"], aberrations["C12"],
+                                                     aberrations["phi12"])
+        else:
+            phase, loss = self.reconstruct(aberrations["C10"], aberrations["C12"], aberrations["phi12"]), None
+        return self.phase_to_numpy(phase), None if loss is None else float(loss)
+
+    def preview_context(self, num_bf: int):
+        """Reduced-BF previews are not built yet: previews run on the full disk."""
+        return contextlib.nullcontext()
+
+    def browser_state(self) -> SSBExportState:
+        """Compact host state (``cuda/engine.py:529-575``)."""
+        eng = self._prepare()
+        kx, ky, ak = (t.cpu().numpy() for t in eng.geometry())
+        rows = np.asarray(self.bf_inds_row, dtype=np.int32)
+        cols = np.asarray(self.bf_inds_col, dtype=np.int32)
+        center = (float(self.bf_center[0]), float(self.bf_center[1]))
+        dist = np.sqrt((rows - center[0]) ** 2 + (cols - center[1]) ** 2)
+        radius = float(dist.max()) + 1e-3
+        sampling = 0.5 * (float(self.angular_sampling[0]) + float(self.angular_sampling[1]))
+        detected = 2.0 * float(self.semiangle_cutoff) / sampling
+        disk = BrightfieldDisk(rows, cols, center, radius, detected, self.detector_shape)
+        k = np.sqrt(kx.astype(np.float64) ** 2 + ky.astype(np.float64) ** 2)
+        phi = np.arctan2(ky, kx)
+        n = self._scan_shape[0]
+        qx = np.fft.fftfreq(n, self.scan_sampling[0]).astype(np.float32)
+        qy = np.fft.fftfreq(n, self.scan_sampling[1]).astype(np.float32)
+        return SSBExportState(
+            backend="hip", scan_shape=self.scan_shape, brightfield=disk, kx_bf=kx, ky_bf=ky, qx_1d=qx,
+            qy_1d=qy, aperture_k=ak, alpha_k2=((k * self.wavelength) ** 2).astype(np.float32),
+            cos2phi_k=np.cos(2 * phi).astype(np.float32), sin2phi_k=np.sin(2 * phi).astype(np.float32),
+            wavelength_A=float(self.wavelength), semiangle_rad=float(self.semiangle_cutoff) * 1e-3,
+            angular_sampling_rad=(float(self.angular_sampling[0]) * 1e-3, float(self.angular_sampling[1]) * 1e-3),
+            sampling_A=(float(self.sampling[0]), float(self.sampling[1])), dc_value=complex(self.dc_value))
+
+    def export_brightfield(self, data, path_stem):
+        """The detector-major BF companion file is not built by this backend yet."""
+        return None
+
+    # -------------------------------------------------------- optimisation
+    def _evaluate(self, points: np.ndarray) -> np.ndarray:
+        """Exact loss of K (C10, C12, phi12) points, one synchronisation."""
+        import torch
+        eng = self._prepare()
+        pts = np.asarray(points, dtype=np.float64).reshape(-1, 3)
+        out = torch.empty(len(pts), dtype=torch.float32, device=self._plan.device)
+        for i, (c10, c12, phi12) in enumerate(pts):
+            # float32 parameters, as the reference evaluates its candidates
+            eng.phase_loss_into(float(np.float32(c10)), float(np.float32(c12)), float(np.float32(phi12)),
+                                out[i:i + 1])
+        return out.cpu().numpy().astype(np.float64)
+
+    def optimize(self, aberrations=None, rotation_angle_deg=None, n_trials: int = 200, seed: int = 42,
+                 verbose: bool = True, bf_subsample: float | None = None,
+                 objective_mode: Literal["native", "exact"] = "exact"):
+        """Global search (``cuda/backend.py:958-1110``): a seeded batched
+        quasi-random design on the exact objective replaces Optuna TPE."""
+        t0 = time.perf_counter()
+        if objective_mode not in {"native", "exact"}:
+            raise ValueError(f"objective_mode must be 'native' or 'exact'; got {objective_mode!r}.")
+        if isinstance(rotation_angle_deg, tuple):
+            raise NotImplementedError("rotation search is not supported by the hip SSB backend yet.")
+        if bf_subsample is not None and bf_subsample != 1.0:
+            raise NotImplementedError("bf_subsample is not supported by the hip SSB backend yet.")
+        if aberrations is None:
+            aberrations = dict(self._DEFAULT_OPTIMIZE_RANGES)
+        if rotation_angle_deg is not None:
+            self._rotation_angle_rad = math.radians(float(rotation_angle_deg))
+        self._optimizer_objective_mode = objective_mode
+        best_params, best_value, history = _opt.batched_search(self._evaluate, aberrations, self.aberrations,
+                                                               int(n_trials), seed=int(seed))
+        self._optuna_trials = history
+        self._best_loss = best_value
+        for key, abr, conv in (("C10_nm", "C10", None), ("C12_nm", "C12", None),
+                               ("phi12_deg", "phi12", math.radians)):
+            if key in best_params:
+                v = best_params[key]
+            elif key in aberrations and not isinstance(aberrations[key], tuple):
+                v = aberrations[key]
+            else:
+                continue
+            self.aberrations[abr] = conv(v) if conv else v
+        self._elapsed_optimize = time.perf_counter() - t0
+        self._n_trials = int(n_trials)
+        if verbose:
+            self._print_summary("Optimize", self._elapsed_optimize)
+        return self
+
+    def refine(self, verbose: bool = True, xatol: float = 0.1, fatol: float = 1e-8, lock=None,
+               bf_subsample: float | None = None, objective_mode=None):
+        """Nelder-Mead on the exact objective (``cuda/backend.py:1112-1259``)."""
+        t0 = time.perf_counter()
+        if objective_mode is None:
+            objective_mode = self._optimizer_objective_mode
+        if objective_mode not in {"native", "exact"}:
+            raise ValueError(f"objective_mode must be 'native' or 'exact'; got {objective_mode!r}.")
+        if bf_subsample is not None and bf_subsample != 1.0:
+            raise NotImplementedError("bf_subsample is not supported by the hip SSB backend yet.")
+        self._optimizer_objective_mode = objective_mode
+        lock = set(lock or [])
+        keys = ["C10", "C12", "phi12"]
+        free = [k for k in keys if k not in lock]
+        if free != keys:
+            raise ValueError(
+                "GPU-only SSB Nelder-Mead does not support locked refinement yet "
+                f"(locked: {', '.join(sorted(lock)) or 'unknown'}). Use no locked aberrations with "
+                "refinement='nelder-mead', locked SSB reference mode, or add a GPU-batched locked refiner.")
+        x0 = np.array([self.aberrations[k] for k in keys], dtype=np.float64)
+        # the exact objective's tolerances of cuda/backend.py:1218-1230
+        eff_xatol, eff_fatol, max_iter = xatol, fatol, 300
+        if xatol == 0.1 and fatol == 1e-8:
+            eff_xatol, eff_fatol, max_iter = 0.25, 2e-6, 160
+        best_x, best_loss, n_evals = _opt.nelder_mead(self._evaluate, x0, eff_xatol, eff_fatol, max_iter)
+        for i, k in enumerate(keys):
+            self.aberrations[k] = float(best_x[i])
+        self._best_loss = float(best_loss)
+        self._elapsed_refine = time.perf_counter() - t0
+        self._refine_method = "nelder-mead"
+        self._refine_nfev = int(n_evals)
+        if verbose:
+            self._print_summary(f"Refine (nelder-mead, {n_evals} evals)", self._elapsed_refine)
+        return self
+
+    def fit(self, *, trials: int, refinement: str | None, search_ranges, refine_lock, seed: int,
+            verbose: bool) -> SSBResult:
+        """The shared fit contract (``cuda/backend.py:853-879``)."""
+        if trials:
+            self.optimize(aberrations=search_ranges, n_trials=int(trials), seed=int(seed), verbose=verbose,
+                          objective_mode="exact")
+        if refinement == "nelder-mead":
+            self.refine(verbose=verbose, lock=refine_lock, objective_mode="exact")
+        return self.result()
+
+    def _print_summary(self, stage: str, elapsed: float) -> None:
+        a = self.aberrations
+        print(f"  {stage}: loss={self._best_loss:.6f}  C10={a['C10']:.1f} nm  C12={a['C12']:.1f} nm  "
+              f"phi12={math.degrees(a['phi12']):.1f}°  {elapsed:.1f}s")
+
+    # ------------------------------------------------------------ lifetime
+    def free(self) -> None:
+        if getattr(self, "_plan", None) is not None:
+            self._plan.close()
+        self._plan = None
+        self._engine = None
+
+    def close(self) -> None:
+        self.free()
+
+
+__all__ = ["HipSSBBackend", "scan_target"]

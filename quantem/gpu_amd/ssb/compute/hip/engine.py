@@ -25,6 +25,32 @@ def _torch():
     return torch
 
 
+def scan_target(rows: int, cols: int) -> int:
+    """Supported square scan size for a rows x cols scan (``cuda/backend.py:238-246``)."""
+    return int(_abi.lib().ssb_scan_target(int(rows), int(cols)))
+
+
+def normalize_scan(data):
+    """Crop/pad a resident uint16 ``[H, W, k_r, k_c]`` scan to the supported
+    square size with the mean-frame padding of ``cuda/backend.py:229-269``
+    (returns ``data`` itself when it is already supported)."""
+    torch = _torch()
+    if not isinstance(data, torch.Tensor) or data.dtype != torch.uint16 or not data.is_cuda:
+        raise ValueError("data must be a torch.uint16 tensor on the GPU")
+    if data.ndim != 4:
+        raise ValueError("data must be 4D (scan_row, scan_col, k_row, k_col)")
+    H, W, kr, kc = (int(v) for v in data.shape)
+    T = scan_target(H, W)
+    if (H, W) == (T, T):
+        return data
+    src = data.contiguous()
+    out = torch.empty((T, T, kr, kc), dtype=torch.uint16, device=data.device)
+    with torch.cuda.device(data.device):
+        stream = _abi.stream_handle(torch.cuda.current_stream(data.device))
+        _abi.call("ssb_normalize_scan_u16", _abi.ptr(src), H, W, kr, kc, _abi.ptr(out), T, stream)
+    return out
+
+
 @dataclass(frozen=True)
 class PlanSpec:
     scan_n: int
@@ -163,6 +189,14 @@ class HipSSBEngine:
                                     complex(float(info[3]), float(info[4])))
         return self._bf
 
+    def detect_bf_radius(self, threshold_ratio: float = 0.1):
+        """((centre row, centre col), radius) of the probe disk in the last
+        mean DP (``qg/detector/compute/cuda/probe.py:12-123``)."""
+        r, c, rad = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+        _abi.call("ssb_detect_bf_radius", self.plan.handle, float(threshold_ratio), ctypes.byref(r),
+                  ctypes.byref(c), ctypes.byref(rad), self._stream())
+        return (int(r.value), int(c.value)), int(rad.value)
+
     def set_bf(self, rows, cols, center, dc) -> None:
         rows = np.ascontiguousarray(np.asarray(rows, dtype=np.int32).reshape(-1))
         cols = np.ascontiguousarray(np.asarray(cols, dtype=np.int32).reshape(-1))
@@ -260,6 +294,48 @@ class HipSSBEngine:
         _abi.call("ssb_object", self.plan.handle, float(c10), float(c12), float(phi12),
                   _abi.ptr(obj), _abi.ptr(phase_out), self._stream())
         return obj
+
+    def _phase_loss(self, c10, c12, phi12, want_loss: bool):
+        torch = self.torch
+        if not self._has_g:
+            raise ValueError("no resident Fourier stack")
+        if self._rotation is None:
+            self.cache_rotation(0.0)
+        phase = torch.empty((self.n, self.n), dtype=torch.float32, device=self.plan.device)
+        loss = torch.empty(1, dtype=torch.float32, device=self.plan.device) if want_loss else None
+        _abi.call("ssb_phase_loss", self.plan.handle, float(c10), float(c12), float(phi12),
+                  _abi.ptr(phase), _abi.ptr(loss), self._stream())
+        return phase, loss
+
+    def phase_loss_into(self, c10: float, c12: float, phi12: float, loss_out) -> None:
+        """Enqueue one exact-loss evaluation writing a 1-element float32 device tensor."""
+        if not self._has_g:
+            raise ValueError("no resident Fourier stack")
+        if self._rotation is None:
+            self.cache_rotation(0.0)
+        _abi.call("ssb_phase_loss", self.plan.handle, float(c10), float(c12), float(phi12), None,
+                  _abi.ptr(loss_out), self._stream())
+
+    def reconstruct_full(self, mags, angles):
+        """Mean per-BF phase for the 14-coefficient aberration vector."""
+        raise NotImplementedError("higher-order (C_s) reconstruction is not built in the hip backend yet")
+
+    def reconstruct_full_with_loss(self, mags, angles):
+        raise NotImplementedError("higher-order (C_s) reconstruction is not built in the hip backend yet")
+
+    def reconstruct(self, c10: float, c12: float, phi12: float):
+        """Mean of the per-BF phase images (float32 [N, N], on the GPU).
+
+        ``SSBEngine.reconstruct`` (``cuda/engine.py:1153-1191``)."""
+        return self._phase_loss(c10, c12, phi12, False)[0]
+
+    def reconstruct_with_loss(self, c10: float, c12: float, phi12: float):
+        """(mean per-BF phase, exact phase-variance loss as a Python float).
+
+        ``SSBEngine.reconstruct_with_loss`` (``cuda/engine.py:1464-1514``);
+        the loss read-back synchronises the stream, as ``float(loss)`` does."""
+        phase, loss = self._phase_loss(c10, c12, phi12, True)
+        return phase, float(loss.item())
 
     def recon_u16(self, data, c10: float, c12: float, phi12: float, rotation_rad: float = 0.0,
                   threshold: float = 0.0, bf_radius: float | None = None, obj_out=None, phase_out=None):

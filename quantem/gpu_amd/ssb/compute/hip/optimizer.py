@@ -1,0 +1,173 @@
+"""Host drivers of the aberration fit: a batched quasi-random search and the
+Nelder-Mead refinement.
+
+Both take ``evaluate(points) -> losses``: ``points`` is a float64 array
+``[K, 3]`` of (C10, C12, phi12) and the callee evaluates all K on the GPU
+with one synchronisation (the losses stay on the device until the batch is
+done).  Nothing here computes a loss.
+
+* ``nelder_mead`` follows ``batch_nelder_mead`` (``cuda/optimizer.py:237-396``):
+  initial steps max(5% |x_i|, 2.5e-4), coefficients (alpha, gamma, rho, sigma)
+  = (1, 2, 0.5, 0.5), vertices ordered by a stable sort so ties keep the
+  earlier vertex, convergence when the simplex spread is below ``xatol`` and
+  the loss spread below ``fatol``, the 4 initial vertices and the 3 shrunk
+  vertices evaluated as batches.
+* ``batched_search`` replaces the reference's Optuna TPE loop
+  (``cuda/optimizer.py:59-230``), whose trial sequence cannot be reproduced
+  offline: a scrambled Sobol design over the search box (seeded), evaluated
+  in batches, then a second Sobol design in a box shrunk around the best
+  point.  Parameter names and the trial-history format follow the
+  reference (``C10_nm``, ``C12_nm``, ``phi12_deg``).
+"""
+from __future__ import annotations
+
+import math
+from typing import Callable
+
+import numpy as np
+
+Evaluate = Callable[[np.ndarray], np.ndarray]
+
+PARAM_KEYS = ("C10_nm", "C12_nm", "phi12_deg")
+DEFAULT_RANGES = {"C10_nm": (-400.0, 400.0), "C12_nm": (0.0, 100.0), "phi12_deg": (-90.0, 90.0)}
+
+
+def nelder_mead(evaluate: Evaluate, x0, xatol: float = 0.1, fatol: float = 1e-8, max_iter: int = 300,
+                flat_fatol: float | None = None) -> tuple[np.ndarray, float, int]:
+    """Minimise over (C10, C12, phi12) from ``x0``; returns (best_x, best_loss, n_evals)."""
+    x0 = np.asarray(x0, dtype=np.float64)
+    n = x0.size
+    if n != 3:
+        raise ValueError("Expected 3 parameters: C10, C12, phi12")
+    simplex = np.empty((n + 1, n), dtype=np.float64)
+    simplex[0] = x0
+    for i in range(n):
+        simplex[i + 1] = x0
+        simplex[i + 1, i] += max(abs(x0[i]) * 0.05, 0.00025)
+    f = np.asarray(evaluate(simplex), dtype=np.float64).copy()
+    n_evals = n + 1
+    if flat_fatol is not None and np.ptp(f) <= float(flat_fatol):
+        return simplex[0], float(f[0]), n_evals
+    alpha, gamma, rho, sigma = 1.0, 2.0, 0.5, 0.5
+
+    def one(x) -> float:
+        return float(np.asarray(evaluate(x[None, :]), dtype=np.float64)[0])
+
+    for _ in range(max_iter):
+        order = np.argsort(f, kind="stable")
+        simplex = simplex[order]
+        f = f[order]
+        if np.max(np.abs(simplex[-1] - simplex[0])) < xatol and abs(f[-1] - f[0]) < fatol:
+            break
+        centroid = simplex[:-1].mean(axis=0)
+        x_r = centroid + alpha * (centroid - simplex[-1])
+        f_r = one(x_r)
+        n_evals += 1
+        if f[0] <= f_r < f[-2]:
+            simplex[-1], f[-1] = x_r, f_r
+            continue
+        if f_r < f[0]:
+            x_e = centroid + gamma * (x_r - centroid)
+            f_e = one(x_e)
+            n_evals += 1
+            if f_e < f_r:
+                simplex[-1], f[-1] = x_e, f_e
+            else:
+                simplex[-1], f[-1] = x_r, f_r
+            continue
+        if f_r < f[-1]:
+            x_c = centroid + rho * (x_r - centroid)
+            f_c = one(x_c)
+            n_evals += 1
+            if f_c <= f_r:
+                simplex[-1], f[-1] = x_c, f_c
+                continue
+        else:
+            x_c = centroid - rho * (centroid - simplex[-1])
+            f_c = one(x_c)
+            n_evals += 1
+            if f_c < f[-1]:
+                simplex[-1], f[-1] = x_c, f_c
+                continue
+        for i in range(1, n + 1):
+            simplex[i] = simplex[0] + sigma * (simplex[i] - simplex[0])
+        f[1:] = np.asarray(evaluate(simplex[1:]), dtype=np.float64)
+        n_evals += n
+    best = int(np.argsort(f, kind="stable")[0])
+    return simplex[best], float(f[best]), n_evals
+
+
+def _split_specs(aberrations: dict, defaults: dict):
+    """Free (key, lo, hi) ranges and fixed values in engine units."""
+    free = []
+    fixed = {}
+    engine_default = {"C10_nm": float(defaults.get("C10", 0.0)), "C12_nm": float(defaults.get("C12", 0.0)),
+                      "phi12_deg": math.degrees(float(defaults.get("phi12", 0.0)))}
+    for key in PARAM_KEYS:
+        spec = aberrations.get(key, engine_default[key])
+        if isinstance(spec, tuple):
+            lo, hi = float(spec[0]), float(spec[1])
+            if not hi >= lo:
+                raise ValueError(f"search range for {key} must be (low, high); got {spec!r}")
+            free.append((key, lo, hi))
+        else:
+            fixed[key] = float(spec)
+    return free, fixed
+
+
+def _to_points(params: list[dict]) -> np.ndarray:
+    pts = np.empty((len(params), 3), dtype=np.float64)
+    for i, p in enumerate(params):
+        pts[i] = (p["C10_nm"], p["C12_nm"], math.radians(p["phi12_deg"]))
+    return pts
+
+
+def batched_search(evaluate: Evaluate, aberrations: dict | None, defaults: dict, n_trials: int,
+                   seed: int = 42, batch: int = 64, refine_fraction: float = 0.25):
+    """Quasi-random search; returns (best_params, best_value, history)."""
+    from scipy.stats import qmc
+
+    if n_trials < 1:
+        raise ValueError("n_trials must be positive")
+    aberrations = dict(DEFAULT_RANGES if aberrations is None else aberrations)
+    free, fixed = _split_specs(aberrations, defaults)
+    history: list[dict] = []
+
+    def run(box, count, rng_seed):
+        if count <= 0:
+            return
+        if free:
+            sampler = qmc.Sobol(d=len(free), scramble=True, seed=rng_seed)
+            m = int(math.ceil(math.log2(max(count, 1))))
+            unit = sampler.random_base2(m)[:count]
+        else:
+            unit = np.zeros((count, 0))
+        params = []
+        for u in unit:
+            p = dict(fixed)
+            for (key, _, _), (lo, hi), v in zip(free, box, u):
+                p[key] = lo + (hi - lo) * float(v)
+            params.append(p)
+        for s in range(0, count, batch):
+            chunk = params[s:s + batch]
+            losses = np.asarray(evaluate(_to_points(chunk)), dtype=np.float64)
+            for p, loss in zip(chunk, losses):
+                history.append({"params": {k: float(p[k]) for k in PARAM_KEYS if k not in fixed},
+                                "loss": float(loss), "_all": p})
+
+    n_local = int(round(n_trials * refine_fraction)) if free else 0
+    run([(lo, hi) for _, lo, hi in free], n_trials - n_local, seed)
+    best = min(range(len(history)), key=lambda i: (history[i]["loss"], i))
+    if n_local:
+        centre = history[best]["_all"]
+        box = []
+        for key, lo, hi in free:
+            half = 0.1 * (hi - lo)
+            box.append((max(lo, centre[key] - half), min(hi, centre[key] + half)))
+        run(box, n_local, seed + 1)
+        best = min(range(len(history)), key=lambda i: (history[i]["loss"], i))
+    best_all = history[best]["_all"]
+    for h in history:
+        h.pop("_all", None)
+    best_params = {k: float(best_all[k]) for k, _, _ in free}
+    return best_params, float(history[best]["loss"]), history

@@ -76,3 +76,11 @@ def test_status_mapping():
         _abi.check(_abi.QSSB_ERR_OOM, "x")
     with pytest.raises(RuntimeError):
         _abi.check(_abi.QSSB_ERR_LAUNCH, "x")
+
+
+def test_scan_target_matches_reference_rule(lib):
+    # cuda/backend.py:238-246: next of 128/256/512/1024 at or above the longest side
+    cases = {(96, 96): 128, (128, 128): 128, (129, 5): 256, (130, 40): 256, (300, 512): 512,
+             (513, 1): 1024, (1100, 10): 1024, (0, 4): 0}
+    for (h, w), t in cases.items():
+        assert lib.ssb_scan_target(h, w) == t

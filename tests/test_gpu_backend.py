@@ -1,0 +1,97 @@
+"""The ``hip`` backend behind the reference's SSB contract, on the GPU:
+construction (3D/4D input, scan padding, detector-sampling detection),
+``result``/``preview``/``browser_state``, the fit, against the oracle."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import ssb_oracle as O
+from tests import _cases as C
+from tests._gpu import synthetic
+
+pytestmark = pytest.mark.gpu
+
+ABER = dict(zip(("C10", "C12", "phi12"), C.MAKE_ENGINE_ABERRATIONS))
+
+
+def backend(data, **kw):
+    from quantem.gpu_amd.ssb.compute.hip.backend import HipSSBBackend
+    args = dict(voltage_kV=300, semiangle=21.4, scan_sampling=0.5, det_sampling=1.0, bf_radius=3)
+    args.update(kw)
+    return HipSSBBackend(data, **args)
+
+
+def test_protocol_result_and_preview_match_oracle(cuda, golden):
+    from quantem.gpu_amd.ssb.types import SSBProtocol
+    data = C.poisson_data(123)
+    ssb = backend(cuda.from_numpy(data).cuda(), aberrations=dict(ABER))
+    assert isinstance(ssb, SSBProtocol)
+    assert ssb.backend == "hip" and ssb.scan_shape == (128, 128) and ssb.detector_shape == (16, 16)
+    p = O.prepare(data, bf_radius=3, **C.POISSON_KW)
+    assert ssb.num_bf == p.bf.rows.size
+    res = ssb.result()
+    ref_obj = O.object_wave(p.G, p.dc, p.geom, *C.MAKE_ENGINE_ABERRATIONS)
+    rms, p999, _ = C.phase_errors(res.phase.cpu().numpy(), np.angle(ref_obj))
+    assert rms < 1e-4 and p999 < 3e-4
+    _, ref_loss = O.phase_and_loss(p.G, p.dc, p.geom, *C.MAKE_ENGINE_ABERRATIONS)
+    assert res.loss == pytest.approx(ref_loss, rel=1e-5, abs=1e-6)
+    assert res.backend == "hip" and res.num_bf == ssb.num_bf and res.aberrations == ssb.aberrations
+    phase, loss = ssb.preview(ABER, compute_loss=True, higher_order_magnitudes=None, higher_order_angles=None)
+    assert phase.dtype == np.float32 and loss == pytest.approx(res.loss, rel=1e-6)
+    st = ssb.browser_state()
+    assert st.num_bf == ssb.num_bf and st.backend == "hip"
+    np.testing.assert_allclose(st.kx_bf, p.geom.kx, atol=1e-7)
+    np.testing.assert_allclose(st.aperture_k, p.geom.aperture, atol=1e-6)
+    ssb.close()
+
+
+def test_numpy_3d_input_and_roi96_padding(cuda, golden):
+    fx = golden["fixtures"]["roi96"]
+    data = C.poisson_data(fx["seed"], scan=(96, 96), add=fx["disk_add"])
+    flat = data.reshape(96 * 96, 16, 16)
+    ssb = backend(flat, scan_shape=(96, 96))
+    assert ssb.scan_shape == (fx["padded_scan"],) * 2
+    padded = O.normalize_scan(data)
+    p = O.prepare(padded, bf_radius=3, **C.POISSON_KW)
+    np.testing.assert_array_equal(ssb.bf_inds_row, p.bf.rows)
+    np.testing.assert_array_equal(ssb.bf_inds_col, p.bf.cols)
+    with pytest.raises(ValueError, match="perfect square"):
+        backend(flat[:-1])
+
+
+def test_detector_sampling_detection_matches_oracle(cuda):
+    cfg, data = synthetic(128, 64, {"C10": 200.0})
+    _, mean = O.mean_dp(data)
+    (rc, cc), radius = O.detect_bf_radius(mean)
+    from quantem.gpu_amd.ssb.compute.hip.backend import HipSSBBackend
+    ssb = HipSSBBackend(cuda.from_numpy(data).cuda(), semiangle=20.0, scan_sampling=cfg.step_A,
+                        det_sampling=None, voltage_kV=300)
+    assert ssb.angular_sampling[0] == pytest.approx(2 * 20.0 / radius)
+    yy, xx = np.mgrid[:64, :64]
+    for r0 in (10, 16, 23):
+        disk = np.where((yy - 30) ** 2 + (xx - 33) ** 2 <= r0 ** 2, 1000, 1).astype(np.uint16)
+        d4 = np.broadcast_to(disk, (128, 128, 64, 64)).copy()
+        from quantem.gpu_amd.ssb.compute.hip.engine import HipSSBEngine, PlanSpec, SSBPlan
+        eng = HipSSBEngine(SSBPlan(PlanSpec(128, 64, 64, 8, 0.0197, 0.02, 1e-3, 1e-3, 0.5, 0.5)))
+        eng.mean_dp(cuda.from_numpy(d4).cuda())
+        assert eng.detect_bf_radius() == ((30, 33), r0)
+
+
+def test_fit_recovers_defocus(cuda):
+    """Synthetic weak-phase data at C10 = 200 A: search + Nelder-Mead on the
+    exact objective lands near the truth and not above the truth's loss."""
+    cfg, data = synthetic(128, 64, {"C10": 200.0}, dose=1e6)
+    from quantem.gpu_amd.ssb.compute.hip.backend import HipSSBBackend
+    ssb = HipSSBBackend(cuda.from_numpy(data).cuda(), semiangle=20.0, scan_sampling=cfg.step_A,
+                        det_sampling=1.25, voltage_kV=300)
+    res = ssb.fit(trials=64, refinement="nelder-mead",
+                  search_ranges={"C10_nm": (-500.0, 500.0), "C12_nm": 0.0, "phi12_deg": 0.0},
+                  refine_lock=None, seed=42, verbose=False)
+    assert abs(res.aberrations["C10"] - 200.0) < 10.0
+    truth = ssb.reconstruct_with_loss(200.0, 0.0, 0.0)[1]
+    assert res.loss <= truth + 1e-6
+    assert res.n_trials == 64 and res.refine_method == "nelder-mead" and res.refine_nfev > 4
+    assert len(res.optuna_trials) == 64

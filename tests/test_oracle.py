@@ -379,3 +379,17 @@ def test_object_of_weak_phase_tracks_ground_truth():
 
     assert corr(200.0) > 0.8
     assert abs(corr(-200.0)) < 0.2
+
+
+def test_detect_bf_radius_hard_disks():
+    yy, xx = np.mgrid[:64, :64]
+    for r0 in (10, 16, 23):
+        disk = np.where((yy - 30) ** 2 + (xx - 33) ** 2 <= r0 ** 2, 100.0, 1.0).astype(np.float32)
+        assert O.detect_bf_radius(disk) == ((30, 33), r0)
+    small = np.where((yy - 30) ** 2 + (xx - 33) ** 2 <= 25, 100.0, 1.0).astype(np.float32)
+    assert O.detect_bf_radius(small)[1] in (5, 6)
+    edge = np.zeros((64, 64), np.float32)
+    edge[0, 0] = 1.0
+    assert O.detect_bf_radius(edge) == ((0, 0), 16)            # max_r < 2: min(shape) // 4
+    with pytest.raises(ValueError, match="no positive finite"):
+        O.detect_bf_radius(np.zeros((8, 8), np.float32))
