@@ -43,6 +43,24 @@ int ssb_detect_bf_radius(ssb_plan* plan, double threshold_ratio, int* center_row
                          void* stream);
 
 /*
+ * Maximum of `count` uint16 values (device array) into *max_out_host.
+ * Synchronises.  Decides whether the BF companion narrows to uint8
+ * (cuda/engine.py:602-604).
+ */
+int ssb_max_u16(const uint16_t* data, long long count, unsigned int* max_out_host, void* stream);
+
+/*
+ * Exact detector-major BF columns of the plan's selection:
+ * out[b][s] = data[s][bf_rows[bf_offset+b]][bf_cols[bf_offset+b]] for
+ * b < count and the N*N scan positions s, as uint16 (narrow_u8 = 0) or
+ * uint8 (narrow_u8 = 1; only lossless when every count is <= 255).  The
+ * gather of SSBEngine.write_exact_bf_source (cuda/engine.py:577-701); the
+ * caller writes the chunks to the companion file.
+ */
+int ssb_bf_columns_u16(ssb_plan* plan, const uint16_t* data, int bf_offset, int count, int narrow_u8, void* out,
+                       void* stream);
+
+/*
  * Supported square scan size for an H x W scan: 128, 256, 512 or 1024, the
  * next one at or above the longest side (cuda/backend.py:238-246); 0 for an
  * empty scan.
@@ -178,6 +196,31 @@ int ssb_object(ssb_plan* plan, double c10, double c12, double phi12, void* obj_o
  */
 int ssb_phase_loss(ssb_plan* plan, double c10, double c12, double phi12, float* phase_out, float* loss_out,
                    void* stream);
+
+/*
+ * The exact per-BF phase path for the full 14-coefficient aberration
+ * vector (host arrays; order C10, C12, C21, C23, C30, C32, C34, C41, C43,
+ * C45, C50, C52, C54, C56; magnitudes in the reference's numeric convention,
+ * angles in radians): chi = (2 pi/lambda) sum alpha^(n+1)/(n+1) C_nm
+ * cos m(phi - phi_nm).  SSBEngine.reconstruct_full(_with_loss)
+ * (cuda/engine.py:1208-1458; chi_full cuda/kernels/common.py:410-532;
+ * packing cuda/backend.py:730-769).  Phase = mean per-BF phase.
+ */
+int ssb_phase_loss_full(ssb_plan* plan, const double* mags_host, const double* angles_host, float* phase_out,
+                        float* loss_out, void* stream);
+
+/*
+ * Exact losses of n_trials aberration sets in one call, all enqueued on
+ * `stream` with no host synchronisation: trial t uses mags_host[14t..],
+ * angles_host[14t..] (layout of ssb_phase_loss_full) and, if rotation_host
+ * is not NULL, the scan rotation rotation_host[t] (rad).  losses_out is a
+ * device array of n_trials floats.  Trials without higher-order terms use
+ * the quadratic-chi kernels.  Replaces the per-trial loop of the
+ * reference's exact objective (SSBEngine._variance_loss_batch_reconstruct_fallback,
+ * cuda/engine.py:1813-1836, driven from cuda/optimizer.py:59-230).
+ */
+int ssb_batch_phase_loss(ssb_plan* plan, int n_trials, const double* mags_host, const double* angles_host,
+                         const double* rotation_host, float* losses_out, void* stream);
 
 /*
  * End-to-end plain SSB from resident raw data without host round trips:

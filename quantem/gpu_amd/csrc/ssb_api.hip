@@ -574,7 +574,7 @@ int ssb_phase_loss(ssb_plan* p, double c10, double c12, double phi12, float* pha
   const QuadChi m = quad_chi(p, c10, c12, phi12);
   HIP_TRY(launch_phasors(p->n, p->info, p->nbf_hint, geom_ptrs(p), m, p->rowph, p->colph, p->echi, s), "phasors");
   HIP_TRY(launch_phase_moments(p->n, p->Gt, p->gcols, p->info, p->g_nbf, geom_ptrs(p), p->rowph, p->colph, p->echi,
-                               p->optics, p->tw, p->T, p->M1, p->M2, p->var_part, phase_out,
+                               nullptr, p->optics, p->tw, p->T, p->M1, p->M2, p->var_part, phase_out,
                                loss_out ? loss_out : p->loss, s),
           "phase_moments");
   return QSSB_OK;
@@ -607,6 +607,37 @@ int ssb_detect_bf_radius(ssb_plan* p, double threshold_ratio, int* center_row, i
   return QSSB_OK;
 }
 
+int ssb_max_u16(const uint16_t* data, long long count, unsigned int* max_out_host, void* stream) {
+  g_last_error.clear();
+  if (!data || !max_out_host) return fail(QSSB_ERR_ARG, "pointers must not be NULL");
+  if (count < 1) return fail(QSSB_ERR_ARG, "count must be positive");
+  hipStream_t s = as_stream(stream);
+  unsigned int* buf = nullptr;
+  HIP_TRY(dmalloc(&buf, (size_t)max_u16_parts() + 1), "max allocation");
+  hipError_t e = launch_max_u16(data, count, buf, buf + max_u16_parts(), s);
+  if (e == hipSuccess) e = hipMemcpyAsync(max_out_host, buf + max_u16_parts(), sizeof(unsigned int),
+                                          hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  dfree(buf);
+  if (e != hipSuccess) return hip_fail(e, "max_u16");
+  return QSSB_OK;
+}
+
+int ssb_bf_columns_u16(ssb_plan* p, const uint16_t* data, int bf_offset, int count, int narrow_u8, void* out,
+                       void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!data || !out) return fail(QSSB_ERR_ARG, "data and output must not be NULL");
+  if (p->nbf_hint < 1) return fail(QSSB_ERR_STATE, "no bright-field selection");
+  if (bf_offset < 0 || count < 1 || bf_offset + count > p->nbf_hint)
+    return fail(QSSB_ERR_ARG, "BF range [" + std::to_string(bf_offset) + ", " + std::to_string(bf_offset + count) +
+                                  ") is outside the selection of " + std::to_string(p->nbf_hint));
+  HIP_TRY(launch_bf_columns(data, (long long)p->n * p->n, p->D, p->det_cols, p->bf_rows, p->bf_cols, bf_offset,
+                            count, narrow_u8 != 0, out, as_stream(stream)),
+          "bf_columns");
+  return QSSB_OK;
+}
+
 int ssb_scan_target(int H, int W) {
   if (H < 1 || W < 1) return 0;
   const int longest = H > W ? H : W;
@@ -635,6 +666,94 @@ int ssb_normalize_scan_u16(const uint16_t* src, int H, int W, int det_rows, int 
   dfree(mean);
   if (e != hipSuccess) return hip_fail(e, "normalize_scan");
   if (e2 != hipSuccess) return hip_fail(e2, "normalize_scan sync");
+  return QSSB_OK;
+}
+
+static ChiPoly chi_poly(const ssb_plan* p, const double* mags, const double* angles) {
+  static const int kn[14] = {1, 1, 2, 2, 3, 3, 3, 4, 4, 4, 5, 5, 5, 5};
+  static const int km[14] = {0, 2, 1, 3, 0, 2, 4, 1, 3, 5, 0, 2, 4, 6};
+  ChiPoly cp{};
+  const double lam = p->wavelength;
+  for (int i = 0; i < 14; ++i) {
+    const int n = kn[i], m = km[i];
+    cp.K[i] = (float)(2.0 * M_PI / lam * std::pow(lam, n + 1) / (n + 1) * mags[i]);
+    cp.ca[i] = (float)std::cos(m * angles[i]);
+    cp.sa[i] = (float)std::sin(m * angles[i]);
+  }
+  return cp;
+}
+
+int ssb_phase_loss_full(ssb_plan* p, const double* mags_host, const double* angles_host, float* phase_out,
+                        float* loss_out, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!mags_host || !angles_host) return fail(QSSB_ERR_ARG, "aberration vectors must not be NULL");
+  if (!p->gcols) return fail(QSSB_ERR_STATE, "no resident Fourier stack");
+  hipStream_t s = as_stream(stream);
+  if (!p->geometry_valid) {
+    rc = ensure_geometry(p, s);
+    if (rc) return rc;
+  }
+  rc = alloc_loss(p);
+  if (rc) return rc;
+  const ChiPoly cp = chi_poly(p, mags_host, angles_host);
+  HIP_TRY(launch_phase_moments(p->n, p->Gt, p->gcols, p->info, p->g_nbf, geom_ptrs(p), p->rowph, p->colph, p->echi,
+                               &cp, p->optics, p->tw, p->T, p->M1, p->M2, p->var_part, phase_out,
+                               loss_out ? loss_out : p->loss, s),
+          "phase_moments_full");
+  return QSSB_OK;
+}
+
+int ssb_batch_phase_loss(ssb_plan* p, int n_trials, const double* mags_host, const double* angles_host,
+                         const double* rotation_host, float* losses_out, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (n_trials < 0) return fail(QSSB_ERR_ARG, "n_trials must be non-negative");
+  if (n_trials == 0) return QSSB_OK;
+  if (!mags_host || !angles_host || !losses_out) return fail(QSSB_ERR_ARG, "trial arrays must not be NULL");
+  if (!p->gcols) return fail(QSSB_ERR_STATE, "no resident Fourier stack");
+  hipStream_t s = as_stream(stream);
+  rc = alloc_loss(p);
+  if (rc) return rc;
+  const double rot0 = p->rotation;
+  bool have_geometry = p->geometry_valid;
+  for (int t = 0; t < n_trials; ++t) {
+    const double* m = mags_host + (size_t)t * 14;
+    const double* a = angles_host + (size_t)t * 14;
+    if (rotation_host && (!have_geometry || rotation_host[t] != p->rotation)) {
+      p->rotation = rotation_host[t];
+      rc = ensure_geometry(p, s);
+      if (rc) return rc;
+      have_geometry = true;
+    } else if (!have_geometry) {
+      rc = ensure_geometry(p, s);
+      if (rc) return rc;
+      have_geometry = true;
+    }
+    bool higher = false;
+    for (int i = 2; i < 14; ++i) higher = higher || (m[i] != 0.0);
+    float* lt = losses_out + t;
+    if (higher) {
+      const ChiPoly cp = chi_poly(p, m, a);
+      HIP_TRY(launch_phase_moments(p->n, p->Gt, p->gcols, p->info, p->g_nbf, geom_ptrs(p), p->rowph, p->colph,
+                                   p->echi, &cp, p->optics, p->tw, p->T, p->M1, p->M2, p->var_part, nullptr, lt, s),
+              "batch phase_moments_full");
+    } else {
+      const QuadChi qc = quad_chi(p, m[0], m[1], a[1]);
+      HIP_TRY(launch_phasors(p->n, p->info, p->nbf_hint, geom_ptrs(p), qc, p->rowph, p->colph, p->echi, s),
+              "batch phasors");
+      HIP_TRY(launch_phase_moments(p->n, p->Gt, p->gcols, p->info, p->g_nbf, geom_ptrs(p), p->rowph, p->colph,
+                                   p->echi, nullptr, p->optics, p->tw, p->T, p->M1, p->M2, p->var_part, nullptr, lt,
+                                   s),
+              "batch phase_moments");
+    }
+  }
+  // leave the cached rotation as the caller set it
+  if (rotation_host && p->rotation != rot0) {
+    p->rotation = rot0;
+    rc = ensure_geometry(p, s);
+    if (rc) return rc;
+  }
   return QSSB_OK;
 }
 

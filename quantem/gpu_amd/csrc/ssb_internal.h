@@ -45,6 +45,17 @@ struct GeomPtrs {
   const float* qyl;
 };
 
+// Krivanek aberration phase as a polynomial in u = (ux, uy): for each of the
+// 14 coefficients C_nm (order C10, C12, C21, C23, C30, C32, C34, C41, C43,
+// C45, C50, C52, C54, C56), chi += K * |u|^(n+1-m) * (ca Re(z^m) + sa Im(z^m))
+// with z = ux + i uy, K = (2 pi / lambda) lambda^(n+1) C_nm / (n+1),
+// (ca, sa) = (cos m phi_nm, sin m phi_nm).  Odd m are the odd part of chi.
+struct ChiPoly {
+  float K[14];
+  float ca[14];
+  float sa[14];
+};
+
 // Matrix of the quadratic aberration phase chi(u) = u^T M u,
 // M = pi*lambda*[[C10 + C12 cos2p, C12 sin2p], [C12 sin2p, C10 - C12 cos2p]].
 struct QuadChi {
@@ -69,6 +80,15 @@ hipError_t launch_bf_geometry(const int* rows, const int* cols, const BfInfo* in
 hipError_t launch_gather_rowfft(int N, const uint16_t* data, int D, int det_cols, const int* rows,
                                 const int* cols, const BfInfo* info, int nbf_hint, const float2* tw, float2* H,
                                 hipStream_t s);
+// Exact BF columns out[b][s] (uint16, or uint8 when narrow) for BF pixels
+// [bf_offset, bf_offset + count) of the list rows/cols.
+hipError_t launch_bf_columns(const uint16_t* data, long long frames, int D, int det_cols, const int* rows,
+                             const int* cols, int bf_offset, int count, bool narrow, void* out, hipStream_t s);
+// max of n uint16 values into out[0] (partial holds max_u16_parts() uints).
+hipError_t launch_max_u16(const uint16_t* data, long long n, unsigned int* partial, unsigned int* out,
+                          hipStream_t s);
+int max_u16_parts();
+
 // Scan normalisation: crop/pad the H x W scan to T x T (partial holds
 // normalize_partial_groups(H, T) * D uint32, mean D uint16).
 hipError_t launch_normalize_scan(const uint16_t* src, int H, int W, int D, uint16_t* dst, int T,
@@ -114,11 +134,13 @@ hipError_t launch_finish_object(int N, const float2* S_part, const BfInfo* info,
 int loss_chunk_bf(int N);
 int loss_splits(int N);
 // Mean of per-BF phases and the phase-variance loss from the resident
-// stack Gt[b][c][q] (gcols = N/2+1 Hermitian, or N full-plane); T holds loss_chunk_bf(N) planes, M1/M2
+// stack Gt[b][c][q] (gcols = N/2+1 Hermitian, or N full-plane), with the
+// phasor tables of a quadratic chi, or (poly != NULL) the full chi; T holds loss_chunk_bf(N) planes, M1/M2
 // loss_splits(N) planes, var_part N*N/256 floats.  phase may be NULL; loss
 // is a device scalar.
 hipError_t launch_phase_moments(int N, const float2* Gt, int gcols, const BfInfo* info, int nbf_hint, const GeomPtrs& gp,
-                                const float2* rowph, const float2* colph, const float2* echi, const Optics& o,
+                                const float2* rowph, const float2* colph, const float2* echi, const ChiPoly* poly,
+                                const Optics& o,
                                 const float2* tw, float2* T, float2* M1, float2* M2, float* var_part,
                                 float* phase, float* loss, hipStream_t s);
 

@@ -35,12 +35,12 @@ namespace qssb {
 // columns (an imported full-plane stack), so the B column is read, not
 // mirrored from A.
 // ------------------------------------------------------------------------
-template <int N, int SG, int BLOCK, bool ISO, bool FULL>
+template <int N, int SG, int BLOCK, bool ISO, bool FULL, bool POLY>
 __global__ void __launch_bounds__(BLOCK)
 k_corr_colifft(const float2* __restrict__ Gt, const BfInfo* __restrict__ info, GeomPtrs gp,
                const float2* __restrict__ rowph_g, const float2* __restrict__ colph_g,
-               const float2* __restrict__ echi_g, ApIso ap, df rout2, Optics o, const float2* __restrict__ tw_g,
-               int b_begin, int b_count, float2* __restrict__ T) {
+               const float2* __restrict__ echi_g, ChiPoly cp, ApIso ap, df rout2, Optics o,
+               const float2* __restrict__ tw_g, int b_begin, int b_count, float2* __restrict__ T) {
   constexpr int HALF = N / 2;
   constexpr int NSG = HALF / SG;
   constexpr int P = SG * N / BLOCK;
@@ -56,6 +56,8 @@ k_corr_colifft(const float2* __restrict__ Gt, const BfInfo* __restrict__ info, G
   __shared__ float2 ehalfc[N];
   __shared__ df q2halfc[N];
   __shared__ float2 enyqr[SG];
+  // ISO column terms of d(Q -/+ k): (Rout^2 - |k|^2) - qy^2 +/- 2 qy ky
+  __shared__ df colA_s[SG + 1], colB_s[SG + 1];
 
   const int tid = threadIdx.x;
   for (int i = tid; i < N; i += BLOCK) tw[i] = tw_g[i];
@@ -69,6 +71,7 @@ k_corr_colifft(const float2* __restrict__ Gt, const BfInfo* __restrict__ info, G
   const int qm = (N - q) & (N - 1);
   const df qxA{gp.qx[q], gp.qxl[q]};
   const df qxB0{gp.qx[qm], gp.qxl[qm]};
+  const df q2x = df_mul(qxA, qxA);
 
   for (long long item = blockIdx.x; item < items; item += gridDim.x) {
     const int bl = (int)(item / NSG);
@@ -83,20 +86,41 @@ k_corr_colifft(const float2* __restrict__ Gt, const BfInfo* __restrict__ info, G
     if (sg == 0 && tid < N) {
       const df qx{gp.qx[tid], gp.qxl[tid]};
       const df qh{gp.qy[HALF], gp.qyl[HALF]};
-      q2halfc[tid] = df_add(df_mul(qx, qx), df_mul(qh, qh));
-      ehalfc[tid] = echi_g[(long long)HALF * N + tid];
+      if (!ISO) q2halfc[tid] = df_add(df_mul(qx, qx), df_mul(qh, qh));
+      if (!POLY) ehalfc[tid] = echi_g[(long long)HALF * N + tid];
     }
-    if (tid < SG) {
+    if (!POLY && tid < SG) {
       const int j = sg * SG + tid;
       enyqr[tid] = echi_g[(long long)((N - j) & (N - 1)) * N + HALF];
     }
-    __syncthreads();
     const df kx{gp.kx[b], gp.kxl[b]}, ky{gp.ky[b], gp.kyl[b]};
+    if (ISO && tid <= SG) {
+      const int col = (tid < SG) ? sg * SG + tid : HALF;
+      const df qy{gp.qy[col], gp.qyl[col]};
+      const df cbv = df_sub(rout2, df_add(df_mul(kx, kx), df_mul(ky, ky)));
+      const df base = df_sub(cbv, df_mul(qy, qy));
+      const df qk = df_mul(qy, df{2.0f * ky.hi, 2.0f * ky.lo});
+      colA_s[tid] = df_add(base, qk);
+      colB_s[tid] = df_sub(base, qk);
+    }
+    __syncthreads();
     const float ak = gp.ak[b];
-    const float2 rowA = rowph_g[(long long)b * N + q];
-    const float2 colHalf = colph_g[(long long)b * GCOLS + HALF];
-    const df kk = df_add(df_mul(kx, kx), df_mul(ky, ky));
-    const df cb = df_sub(rout2, kk);
+    const float2 rowA = POLY ? make_float2(1.f, 0.f) : rowph_g[(long long)b * N + q];
+    const float2 colHalf = POLY ? make_float2(1.f, 0.f) : colph_g[(long long)b * GCOLS + HALF];
+    float chik = 0.0f;
+    if constexpr (POLY) {
+      float e, od;
+      chi_parts(cp, kx.hi, ky.hi, e, od);
+      chik = e + od;
+    }
+    df rowTA{0.f, 0.f}, rowTB{0.f, 0.f}, cb{0.f, 0.f};
+    if constexpr (ISO) {
+      const df u = df_mul(qxA, df{2.0f * kx.hi, 2.0f * kx.lo});
+      rowTA = df_sub(u, q2x);
+      rowTB = df_neg(df_add(q2x, u));
+    } else {
+      cb = df_sub(rout2, df_add(df_mul(kx, kx), df_mul(ky, ky)));
+    }
     const float2* gb = Gt + (long long)b * SCOLS * N;
 #pragma unroll
     for (int i = 0; i < P; ++i) {
@@ -107,11 +131,16 @@ k_corr_colifft(const float2* __restrict__ Gt, const BfInfo* __restrict__ info, G
       const float2 GB = (j == 0) ? gb[(long long)HALF * N + q]
                                  : (FULL ? gb[(long long)(N - j) * N + qm] : cconj(z));
       const df qy{qyS[s], qySl[s]};
-      const df q2A = df_add(df_mul(qxA, qxA), df_mul(qy, qy));
-      const float2 echiA = echi_g[(long long)j * N + q];
-      const float2 colp = colph_g[(long long)b * GCOLS + j];
+      const df q2A = ISO ? df{0.f, 0.f} : df_add(q2x, df_mul(qy, qy));
+      const float2 echiA = POLY ? make_float2(1.f, 0.f) : echi_g[(long long)j * N + q];
+      const float2 colp = POLY ? make_float2(1.f, 0.f) : colph_g[(long long)b * GCOLS + j];
       float am, app;
-      aperture_pair<ISO>(qxA, qy, kx, ky, cb, q2A, ap, o, am, app);
+      if constexpr (ISO) {
+        am = aperture_from_d(df_add(rowTA, colA_s[s]), ap);
+        app = aperture_from_d(df_add(rowTB, colB_s[s]), ap);
+      } else {
+        aperture_pair<ISO>(qxA, qy, kx, ky, cb, q2A, ap, o, am, app);
+      }
       const float2 eA = cmul(rowA, colp);
       const bool mirror = (j != 0) && (q != HALF);
       float bm, bp;
@@ -122,15 +151,44 @@ k_corr_colifft(const float2* __restrict__ Gt, const BfInfo* __restrict__ info, G
         eB = cconj(eA);
         echiB = echiA;
       } else {
-        const df qyB = (j == 0) ? df{qyS[SG], qySl[SG]} : df{-qy.hi, -qy.lo};
-        const df qxB = (j == 0) ? qxA : qxB0;
-        const df q2B = (j == 0) ? q2halfc[q] : q2A;
-        echiB = (j == 0) ? ehalfc[q] : enyqr[s];
-        aperture_pair<ISO>(qxB, qyB, kx, ky, cb, q2B, ap, o, bm, bp);
+        echiB = POLY ? make_float2(1.f, 0.f) : ((j == 0) ? ehalfc[q] : enyqr[s]);
+        if constexpr (ISO) {
+          const df dA = (j == 0) ? df_add(rowTA, colA_s[SG]) : df_add(rowTA, colB_s[s]);
+          const df dB = (j == 0) ? df_add(rowTB, colB_s[SG]) : df_add(rowTB, colA_s[s]);
+          bm = aperture_from_d(dA, ap);
+          bp = aperture_from_d(dB, ap);
+        } else {
+          const df qyB = (j == 0) ? df{qyS[SG], qySl[SG]} : df{-qy.hi, -qy.lo};
+          const df qxB = (j == 0) ? qxA : qxB0;
+          const df q2B = (j == 0) ? q2halfc[q] : q2A;
+          aperture_pair<ISO>(qxB, qyB, kx, ky, cb, q2B, ap, o, bm, bp);
+        }
         eB = cmul(rowA, (j == 0) ? colHalf : cconj(colp));
       }
-      float2 CA = mul_conj(GA, gamma_hat(ak, eA, am, app, echiA));
-      const float2 CB = mul_conj(GB, gamma_hat(ak, eB, bm, bp, echiB));
+      float2 gA, gB;
+      if constexpr (POLY) {
+        // chi at Q -/+ k split into even and odd parts: chi(-u) = E(u) - O(u)
+        const float qxf = qxA.hi, qyf = qy.hi;
+        float em, om, ep, op;
+        chi_parts(cp, qxf - kx.hi, qyf - ky.hi, em, om);
+        chi_parts(cp, qxf + kx.hi, qyf + ky.hi, ep, op);
+        gA = gamma_hat_chi(ak, am, app, em + om - chik, ep + op - chik);
+        if (mirror) {
+          gB = gamma_hat_chi(ak, bm, bp, ep - op - chik, em - om - chik);
+        } else {
+          const float qxb = (j == 0) ? qxf : qxB0.hi;
+          const float qyb = (j == 0) ? qyS[SG] : -qyf;
+          float e1, o1, e2, o2;
+          chi_parts(cp, qxb - kx.hi, qyb - ky.hi, e1, o1);
+          chi_parts(cp, qxb + kx.hi, qyb + ky.hi, e2, o2);
+          gB = gamma_hat_chi(ak, bm, bp, e1 + o1 - chik, e2 + o2 - chik);
+        }
+      } else {
+        gA = gamma_hat(ak, eA, am, app, echiA);
+        gB = gamma_hat(ak, eB, bm, bp, echiB);
+      }
+      float2 CA = mul_conj(GA, gA);
+      const float2 CB = mul_conj(GB, gB);
       if (j == 0 && q == 0) CA = dcv;
       const int rowB = (j == 0) ? q : qm;
       buf[s * STRIDE + q] = CA;
@@ -292,7 +350,8 @@ int loss_chunk_bf(int N) { return N == 128 ? 512 : (N == 256 ? 128 : 32); }
 int loss_splits(int N) { return N == 128 ? 32 : (N == 256 ? 8 : 2); }
 
 hipError_t launch_phase_moments(int N, const float2* Gt, int gcols, const BfInfo* info, int nbf_hint, const GeomPtrs& gp,
-                                const float2* rowph, const float2* colph, const float2* echi, const Optics& o,
+                                const float2* rowph, const float2* colph, const float2* echi, const ChiPoly* poly,
+                                const Optics& o,
                                 const float2* tw, float2* T, float2* M1, float2* M2, float* var_part,
                                 float* phase, float* loss, hipStream_t s) {
   df rout2;
@@ -304,8 +363,14 @@ hipError_t launch_phase_moments(int N, const float2* Gt, int gcols, const BfInfo
   const int nsplit = loss_splits(N);
   const int nb = nbf_hint > 0 ? nbf_hint : 1;
 #define QSSB_P1_ONE(NN, SGV, BL, IS, FU)                                                                    \
-  hipLaunchKernelGGL((k_corr_colifft<NN, SGV, BL, IS, FU>), dim3(g1), dim3(BL), 0, s, Gt, info, gp, rowph, colph, \
-                     echi, ap, rout2, o, tw, b0, cnt, T)
+  do {                                                                                                       \
+    if (poly)                                                                                                \
+      hipLaunchKernelGGL((k_corr_colifft<NN, SGV, BL, IS, FU, true>), dim3(g1), dim3(BL), 0, s, Gt, info, gp,   \
+                         rowph, colph, echi, *poly, ap, rout2, o, tw, b0, cnt, T);                            \
+    else                                                                                                     \
+      hipLaunchKernelGGL((k_corr_colifft<NN, SGV, BL, IS, FU, false>), dim3(g1), dim3(BL), 0, s, Gt, info, gp,  \
+                         rowph, colph, echi, ChiPoly{}, ap, rout2, o, tw, b0, cnt, T);                        \
+  } while (0)
 #define QSSB_P1(NN, SGV, BL)                                                    \
   do {                                                                         \
     const unsigned g1 = (unsigned)((long long)cnt * ((NN / 2) / SGV));        \

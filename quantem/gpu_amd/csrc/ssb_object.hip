@@ -96,6 +96,10 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
   __shared__ float2 ehalfc[N];
   __shared__ df q2halfc[N];
   __shared__ float2 enyqr[SG];
+  // ISO: column terms of d(Q -/+ k) for the SG slot columns and column N/2,
+  // double-buffered over BF pixels:
+  //   colA = (Rout^2 - |k|^2) - qy^2 + 2 qy ky,  colB = (Rout^2 - |k|^2) - qy^2 - 2 qy ky
+  __shared__ df colA_s[2][SG + 1], colB_s[2][SG + 1];
 
   const int tid = threadIdx.x;
   if (FROM_H)
@@ -107,6 +111,7 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
   const int qm = (N - q) & (N - 1);
   const df qxA{gp.qx[q], gp.qxl[q]};
   const df qxB0{gp.qx[qm], gp.qxl[qm]};   // row of output B when j >= 1
+  const df q2x = df_mul(qxA, qxA);
   const int wave_base = tid & ~63;
 
   for (long long item = blockIdx.x; item < items; item += gridDim.x) {
@@ -136,8 +141,22 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
         copy_async16(gsrc, &abuf[slot][e0]);
       }
     };
+    // column terms of BF pixel b into buffer `slot` (threads 0..SG)
+    auto colterms = [&](int b, int slot) {
+      if (ISO && tid <= SG) {
+        const int col = (tid < SG) ? sg * SG + tid : HALF;
+        const df qy{gp.qy[col], gp.qyl[col]};
+        const df kx{gp.kx[b], gp.kxl[b]}, ky{gp.ky[b], gp.kyl[b]};
+        const df cbv = df_sub(rout2, df_add(df_mul(kx, kx), df_mul(ky, ky)));
+        const df base = df_sub(cbv, df_mul(qy, qy));
+        const df qk = df_mul(qy, df{2.0f * ky.hi, 2.0f * ky.lo});
+        colA_s[slot][tid] = df_add(base, qk);
+        colB_s[slot][tid] = df_sub(base, qk);
+      }
+    };
     __syncthreads();
     if (b0 < b1) issue(b0, 0);
+    if (b0 < b1) colterms(b0, 0);
     if (tid <= SG) {
       const int col = (tid < SG) ? sg * SG + tid : HALF;
       qyS[tid] = gp.qy[col];
@@ -147,7 +166,7 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
       if (sg == 0 && tid < N) {
         const df qx{gp.qx[tid], gp.qxl[tid]};
         const df qh{gp.qy[HALF], gp.qyl[HALF]};   // not qyS: written in this same phase
-        q2halfc[tid] = df_add(df_mul(qx, qx), df_mul(qh, qh));
+        if (!ISO) q2halfc[tid] = df_add(df_mul(qx, qx), df_mul(qh, qh));
         ehalfc[tid] = echi_g[(long long)HALF * N + tid];
       }
       if (tid < SG) {
@@ -164,7 +183,7 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
     for (int i = 0; i < P; ++i) {
       const int s = s0 + i * SSTEP;
       const df qy{qyS[s], qySl[s]};
-      q2A[i] = df_add(df_mul(qxA, qxA), df_mul(qy, qy));
+      q2A[i] = ISO ? df{0.f, 0.f} : df_add(df_mul(qxA, qxA), df_mul(qy, qy));
       echiA[i] = echi_g[(long long)(sg * SG + s) * N + q];
       arA[i].init();
       aiA[i].init();
@@ -194,6 +213,7 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
       if (!FROM_H && sg == 0) gHalf = src[((long long)b * GCOLS + HALF) * N + q];
       asm volatile("" ::: "memory");
       if (b + 1 < b1) issue(b + 1, cur ^ 1);
+      if (b + 1 < b1) colterms(b + 1, cur ^ 1);
       if (FROM_H) fft_lds_il<N, SG, BLOCK, TSTRIDE, false>(abuf[cur], tileT, tw);
       const float2* tile = FROM_H ? tileT : abuf[cur];
       constexpr int TS = FROM_H ? TSTRIDE : N;
@@ -216,8 +236,16 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
         }
         continue;
       }
-      const df kk = df_add(df_mul(kx, kx), df_mul(ky, ky));
-      const df cb = df_sub(rout2, kk);           // Rout^2 - |k|^2
+      // ISO row terms of d(Q -/+ k): -qx^2 +/- 2 qx kx
+      df rowTA{0.f, 0.f}, rowTB{0.f, 0.f};
+      df cb{0.f, 0.f};
+      if constexpr (ISO) {
+        const df u = df_mul(qxA, df{2.0f * kx.hi, 2.0f * kx.lo});
+        rowTA = df_sub(u, q2x);
+        rowTB = df_neg(df_add(q2x, u));
+      } else {
+        cb = df_sub(rout2, df_add(df_mul(kx, kx), df_mul(ky, ky)));
+      }
 #pragma unroll
       for (int i = 0; i < P; ++i) {
         const int s = s0 + i * SSTEP;
@@ -239,7 +267,12 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
         }
         const df qy{qyS[s], qySl[s]};
         float am, app;
-        aperture_pair<ISO>(qxA, qy, kx, ky, cb, q2A[i], ap, o, am, app);
+        if constexpr (ISO) {
+          am = aperture_from_d(df_add(rowTA, colA_s[cur][s]), ap);
+          app = aperture_from_d(df_add(rowTB, colB_s[cur][s]), ap);
+        } else {
+          aperture_pair<ISO>(qxA, qy, kx, ky, cb, q2A[i], ap, o, am, app);
+        }
         const float2 eA = cmul(rowA, colp[i]);
         const bool mirror = (j != 0) && (q != HALF);
         float bm, bp;
@@ -251,11 +284,19 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
           echiB = echiA[i];
         } else {
           // j == 0: row q, column N/2; q == N/2: row N/2, column N-j
-          const df qyB = (j == 0) ? df{qyS[SG], qySl[SG]} : df{-qy.hi, -qy.lo};
-          const df qxB = (j == 0) ? qxA : qxB0;
-          const df q2B = (j == 0) ? q2halfc[q] : q2A[i];
           echiB = (j == 0) ? ehalfc[q] : enyqr[s];
-          aperture_pair<ISO>(qxB, qyB, kx, ky, cb, q2B, ap, o, bm, bp);
+          if constexpr (ISO) {
+            // j == 0: row q, column N/2; q == N/2: row N/2 (this row), column -qy
+            const df dA = (j == 0) ? df_add(rowTA, colA_s[cur][SG]) : df_add(rowTA, colB_s[cur][s]);
+            const df dB = (j == 0) ? df_add(rowTB, colB_s[cur][SG]) : df_add(rowTB, colA_s[cur][s]);
+            bm = aperture_from_d(dA, ap);
+            bp = aperture_from_d(dB, ap);
+          } else {
+            const df qyB = (j == 0) ? df{qyS[SG], qySl[SG]} : df{-qy.hi, -qy.lo};
+            const df qxB = (j == 0) ? qxA : qxB0;
+            const df q2B = (j == 0) ? q2halfc[q] : q2A[i];
+            aperture_pair<ISO>(qxB, qyB, kx, ky, cb, q2B, ap, o, bm, bp);
+          }
           // rows q and -q coincide at q == N/2, so the row phasor is rowA
           eB = cmul(rowA, (j == 0) ? colHalf : cconj(colp[i]));
         }
@@ -517,13 +558,13 @@ hipError_t launch_trotter_slots(int N, bool from_h, const float2* src, const BfI
   } while (0)
   switch (N) {
     case 128:
-      QSSB_SLOTS(128, 8, 1024);
+      QSSB_SLOTS(128, 4, 256);
       break;
     case 256:
-      QSSB_SLOTS(256, 8, 1024);
+      QSSB_SLOTS(256, 2, 256);
       break;
     case 512:
-      QSSB_SLOTS(512, 4, 1024);
+      QSSB_SLOTS(512, 2, 512);
       break;
     default:
       return hipErrorInvalidValue;

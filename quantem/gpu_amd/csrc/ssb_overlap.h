@@ -62,6 +62,39 @@ __device__ __forceinline__ float2 gamma_hat(float ak, float2 eth, float am, floa
   return make_float2(gr * sc, gi * sc);
 }
 
+// chi(u) split into its even part E and odd part O (ChiPoly).
+__device__ __forceinline__ void chi_parts(const ChiPoly& cp, float ux, float uy, float& E, float& O) {
+  const float r2 = ux * ux + uy * uy;
+  float zr[7], zi[7];
+  zr[0] = 1.0f;
+  zi[0] = 0.0f;
+#pragma unroll
+  for (int m = 1; m < 7; ++m) {
+    zr[m] = zr[m - 1] * ux - zi[m - 1] * uy;
+    zi[m] = zr[m - 1] * uy + zi[m - 1] * ux;
+  }
+  const float r4 = r2 * r2, r6 = r4 * r2;
+  auto t = [&](int i, int m) { return cp.ca[i] * zr[m] + cp.sa[i] * zi[m]; };
+  E = cp.K[0] * r2 + cp.K[1] * t(1, 2) + cp.K[4] * r4 + cp.K[5] * r2 * t(5, 2) + cp.K[6] * t(6, 4) +
+      cp.K[10] * r6 + cp.K[11] * r4 * t(11, 2) + cp.K[12] * r2 * t(12, 4) + cp.K[13] * t(13, 6);
+  O = cp.K[2] * r2 * t(2, 1) + cp.K[3] * t(3, 3) + cp.K[7] * r4 * t(7, 1) + cp.K[8] * r2 * t(8, 3) +
+      cp.K[9] * t(9, 5);
+}
+
+// Gamma^ from the phases: Gamma = A(k) [A(Q-k) e^{-i dm} - A(Q+k) e^{i dp}],
+// dm = chi(Q-k) - chi(k), dp = chi(Q+k) - chi(k); same guard as gamma_hat.
+__device__ __forceinline__ float2 gamma_hat_chi(float ak, float am, float app, float dm, float dp) {
+  if (am == 0.0f && app == 0.0f) return make_float2(0.0f, 0.0f);
+  float sm, cm, sp, cpp;
+  sincosf(dm, &sm, &cm);
+  sincosf(dp, &sp, &cpp);
+  float gr = ak * (am * cm - app * cpp);
+  float gi = ak * (-am * sm - app * sp);
+  const float mag2 = gr * gr + gi * gi;
+  const float sc = (mag2 > 1e-16f) ? rsqrtf(mag2) : 1e8f;
+  return make_float2(gr * sc, gi * sc);
+}
+
 // G * conj(g)
 __device__ __forceinline__ float2 mul_conj(float2 G, float2 g) {
   return make_float2(G.x * g.x + G.y * g.y, G.y * g.x - G.x * g.y);

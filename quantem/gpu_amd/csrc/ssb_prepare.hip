@@ -214,21 +214,34 @@ __device__ __forceinline__ float count_over_pi(float n) {
   return p + (e + n * inv_pi_lo);
 }
 
+// CACHE: the mean DP (<= 16384 pixels) is staged in LDS once for the passes.
+#define SELECT_CACHE_PIX 16384
+template <bool CACHE>
 __global__ void __launch_bounds__(MASK_THREADS)
-k_select_bf(const float* __restrict__ mean, const unsigned long long* __restrict__ sums,
+k_select_bf(const float* __restrict__ mean_g, const unsigned long long* __restrict__ sums,
             int det_rows, int det_cols, float threshold, float bf_radius, int max_bf,
             int* __restrict__ bf_rows, int* __restrict__ bf_cols, BfInfo* __restrict__ info) {
   __shared__ float sh_f[2 * MASK_WAVES];
   __shared__ long long sh_i[3 * MASK_WAVES];
   __shared__ int sh_scan[MASK_WAVES];
+  __shared__ float smean[CACHE ? SELECT_CACHE_PIX : 1];
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = tid >> 6;
   const int D = det_rows * det_cols;
+  if (CACHE) {
+    for (int i = tid; i < D; i += MASK_THREADS) smean[i] = mean_g[i];
+    __syncthreads();
+  }
+  const float* mean = CACHE ? smean : mean_g;
   // each thread owns the contiguous pixels [p0, p0 + per)
   const int per = (D + MASK_THREADS - 1) / MASK_THREADS;   // <= MASK_PER
   const int p0 = tid * per;
   const int p1 = min(D, p0 + per);
 #define FOR_PIX for (int i = p0; i < p1; ++i)
+  // (row, col) of pixel i, advanced incrementally from p0 (no divisions)
+  const int r_start = p0 / det_cols, c_start = p0 - (p0 / det_cols) * det_cols;
+#define FOR_PIX_RC \
+  for (int i = p0, r_ = r_start, c_ = c_start; i < p1; ++i, (++c_ == det_cols) ? (c_ = 0, ++r_) : 0)
   float lmax = -INFINITY;
   FOR_PIX lmax = fmaxf(lmax, mean[i]);
   const float mmax = block_max(lmax, sh_f);
@@ -251,11 +264,11 @@ k_select_bf(const float* __restrict__ mean, const unsigned long long* __restrict
     const float sd = sqrtf(ss / (float)D);
     const float thr = mu + sd;
     long long cnt = 0, sr = 0, sc = 0;
-    FOR_PIX {
+    FOR_PIX_RC {
       if (mean[i] > thr) {
         cnt += 1;
-        sr += i / det_cols;
-        sc += i % det_cols;
+        sr += r_;
+        sc += c_;
       }
     }
     block_sum3_i64(cnt, sr, sc, sh_i);
@@ -275,10 +288,10 @@ k_select_bf(const float* __restrict__ mean, const unsigned long long* __restrict
     kr.init();
     kc.init();
     long long n = 0, sr = 0, sc = 0;
-    FOR_PIX {
+    FOR_PIX_RC {
       const float wv = mean[i];
       if (wv > cut) {
-        const int r = i / det_cols, c = i % det_cols;
+        const int r = r_, c = c_;
         kw.add(wv);
         kr.add((float)r * wv);
         kc.add((float)c * wv);
@@ -309,10 +322,10 @@ k_select_bf(const float* __restrict__ mean, const unsigned long long* __restrict
   unsigned long long mask_bits = 0ull;
   long long any_cand = 0, dcs = 0;
   int local = 0;
-  FOR_PIX {
+  FOR_PIX_RC {
     if (mean[i] > cut) {
       any_cand += 1;
-      if (dist2_f32((float)(i / det_cols), (float)(i % det_cols), center_r, center_c) <= r2) {
+      if (dist2_f32((float)r_, (float)c_, center_r, center_c) <= r2) {
         mask_bits |= 1ull << (i - p0);
         local += 1;
         dcs += (long long)sums[i];
@@ -336,16 +349,17 @@ k_select_bf(const float* __restrict__ mean, const unsigned long long* __restrict
     nbf += t;
   }
   int pos = wave_off + incl - local;
-  FOR_PIX {
+  FOR_PIX_RC {
     if (mask_bits & (1ull << (i - p0))) {
       if (pos < max_bf) {
-        bf_rows[pos] = i / det_cols;
-        bf_cols[pos] = i % det_cols;
+        bf_rows[pos] = r_;
+        bf_cols[pos] = c_;
       }
       ++pos;
     }
   }
 #undef FOR_PIX
+#undef FOR_PIX_RC
   if (tid == 0) {
     int status = 0;
     if (any_cand == 0) status = QSSB_DEV_NO_CANDIDATES;
@@ -662,8 +676,12 @@ hipError_t launch_detect_bf_radius(const float* mean, int R, int C, df ratio, in
 hipError_t launch_select_bf(const float* mean, const unsigned long long* sums, int det_rows, int det_cols,
                             float threshold, float bf_radius, int max_bf, int* rows, int* cols,
                             BfInfo* info, hipStream_t s) {
-  hipLaunchKernelGGL(k_select_bf, dim3(1), dim3(MASK_THREADS), 0, s, mean, sums, det_rows, det_cols,
-                     threshold, bf_radius, max_bf, rows, cols, info);
+  if (det_rows * det_cols <= SELECT_CACHE_PIX)
+    hipLaunchKernelGGL(k_select_bf<true>, dim3(1), dim3(MASK_THREADS), 0, s, mean, sums, det_rows, det_cols,
+                       threshold, bf_radius, max_bf, rows, cols, info);
+  else
+    hipLaunchKernelGGL(k_select_bf<false>, dim3(1), dim3(MASK_THREADS), 0, s, mean, sums, det_rows, det_cols,
+                       threshold, bf_radius, max_bf, rows, cols, info);
   return hipGetLastError();
 }
 
@@ -791,5 +809,87 @@ int normalize_partial_groups(int H, int T) {
   const int h = H < T ? H : T;
   return (h + kWindowRows - 1) / kWindowRows;
 }
+
+// ------------------------------------------------------------------------
+// Exact BF columns, detector-major (SSBEngine.write_exact_bf_source,
+// cuda/engine.py:577-701): out[b][s] = data[s][bf_rows[b]][bf_cols[b]] for
+// BF pixels [bf_offset, bf_offset + count), as uint16 or narrowed to uint8
+// (the caller narrows only when the whole block fits).  A 16 x 256 tile of
+// (BF, scan) passes through LDS so both the frame reads and the column
+// writes are contiguous runs.
+// ------------------------------------------------------------------------
+template <typename OUT>
+__global__ void __launch_bounds__(256)
+k_bf_columns(const uint16_t* __restrict__ data, long long frames, int D, int det_cols, const int* __restrict__ rows,
+             const int* __restrict__ cols, int bf_offset, int count, OUT* __restrict__ out) {
+  __shared__ uint16_t tile[16][257];
+  __shared__ int lin[16];
+  const int tid = threadIdx.x;
+  const int b0 = blockIdx.y * 16;
+  const long long s0 = (long long)blockIdx.x * 256;
+  if (tid < 16) {
+    const int b = b0 + tid;
+    lin[tid] = (b < count) ? rows[bf_offset + b] * det_cols + cols[bf_offset + b] : 0;
+  }
+  __syncthreads();
+  // reads: 16 lanes per frame (one per BF pixel), 16 frames per pass
+  const int lb = tid & 15, lf = tid >> 4;
+  for (int f = lf; f < 256; f += 16) {
+    const long long s = s0 + f;
+    if (s < frames && b0 + lb < count) tile[lb][f] = data[s * D + lin[lb]];
+  }
+  __syncthreads();
+  for (int bl = 0; bl < 16; ++bl) {
+    const long long s = s0 + tid;
+    if (b0 + bl < count && s < frames) out[(long long)(b0 + bl) * frames + s] = (OUT)tile[bl][tid];
+  }
+}
+
+// max over n uint16 values: per-block maxima, then one block
+__global__ void __launch_bounds__(256)
+k_max_u16(const uint16_t* __restrict__ data, long long n, unsigned int* __restrict__ partial) {
+  unsigned int m = 0u;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    m = max(m, (unsigned int)data[i]);
+  for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned int)__shfl_xor((int)m, off, 64));
+  __shared__ unsigned int sm[4];
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = max(max(sm[0], sm[1]), max(sm[2], sm[3]));
+}
+
+__global__ void __launch_bounds__(256)
+k_max_final(const unsigned int* __restrict__ partial, int parts, unsigned int* __restrict__ out) {
+  unsigned int m = 0u;
+  for (int i = threadIdx.x; i < parts; i += 256) m = max(m, partial[i]);
+  for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned int)__shfl_xor((int)m, off, 64));
+  __shared__ unsigned int sm[4];
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) out[0] = max(max(sm[0], sm[1]), max(sm[2], sm[3]));
+}
+
+hipError_t launch_bf_columns(const uint16_t* data, long long frames, int D, int det_cols, const int* rows,
+                             const int* cols, int bf_offset, int count, bool narrow, void* out, hipStream_t s) {
+  const dim3 grid((unsigned)((frames + 255) / 256), (unsigned)((count + 15) / 16));
+  if (narrow)
+    hipLaunchKernelGGL(k_bf_columns<uint8_t>, grid, dim3(256), 0, s, data, frames, D, det_cols, rows, cols,
+                       bf_offset, count, reinterpret_cast<uint8_t*>(out));
+  else
+    hipLaunchKernelGGL(k_bf_columns<uint16_t>, grid, dim3(256), 0, s, data, frames, D, det_cols, rows, cols,
+                       bf_offset, count, reinterpret_cast<uint16_t*>(out));
+  return hipGetLastError();
+}
+
+constexpr int kMaxParts = 1024;
+
+hipError_t launch_max_u16(const uint16_t* data, long long n, unsigned int* partial, unsigned int* out,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_max_u16, dim3(kMaxParts), dim3(256), 0, s, data, n, partial);
+  hipLaunchKernelGGL(k_max_final, dim3(1), dim3(256), 0, s, partial, kMaxParts, out);
+  return hipGetLastError();
+}
+
+int max_u16_parts() { return kMaxParts; }
 
 }  // namespace qssb

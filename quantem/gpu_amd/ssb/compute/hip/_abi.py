@@ -29,6 +29,7 @@ _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
 _ip = ctypes.POINTER(ctypes.c_int32)
 _fp = ctypes.POINTER(ctypes.c_float)
+_dp = ctypes.POINTER(ctypes.c_double)
 _pp = ctypes.POINTER(ctypes.c_void_p)
 
 # name -> argtypes; every function returns int
@@ -55,7 +56,11 @@ SIGNATURES: dict[str, list] = {
     "ssb_timing_begin": [_vp, _c_int],
     "ssb_timing_end": [_vp, _fp, _c_int, ctypes.POINTER(_c_int)],
     "ssb_phase_loss": [_vp, _c_double, _c_double, _c_double, _vp, _vp, _vp],
+    "ssb_phase_loss_full": [_vp, _dp, _dp, _vp, _vp, _vp],
+    "ssb_batch_phase_loss": [_vp, _c_int, _dp, _dp, _dp, _vp, _vp],
     "ssb_scan_target": [_c_int, _c_int],
+    "ssb_max_u16": [_vp, _i64, ctypes.POINTER(ctypes.c_uint), _vp],
+    "ssb_bf_columns_u16": [_vp, _vp, _c_int, _c_int, _c_int, _vp, _vp],
     "ssb_detect_bf_radius": [_vp, _c_double, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int),
                              ctypes.POINTER(_c_int), _vp],
     "ssb_normalize_scan_u16": [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _vp],

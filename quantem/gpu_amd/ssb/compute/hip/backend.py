@@ -136,6 +136,9 @@ class HipSSBBackend:
         self._n_trials: int | None = None
         self._optuna_trials: list[dict] = []
         self._optimizer_objective_mode = "exact"
+        self._bf_source_path: pathlib.Path | None = None
+        self._bf_source_dtype: np.dtype | None = None
+        self._bf_source_max_value: int | None = None
 
     # ------------------------------------------------------------- helpers
     @staticmethod
@@ -337,48 +340,112 @@ class HipSSBBackend:
             cos2phi_k=np.cos(2 * phi).astype(np.float32), sin2phi_k=np.sin(2 * phi).astype(np.float32),
             wavelength_A=float(self.wavelength), semiangle_rad=float(self.semiangle_cutoff) * 1e-3,
             angular_sampling_rad=(float(self.angular_sampling[0]) * 1e-3, float(self.angular_sampling[1]) * 1e-3),
-            sampling_A=(float(self.sampling[0]), float(self.sampling[1])), dc_value=complex(self.dc_value))
+            sampling_A=(float(self.sampling[0]), float(self.sampling[1])), dc_value=complex(self.dc_value),
+            bf_source_path=self._bf_source_path, bf_source_dtype=self._bf_source_dtype,
+            bf_source_max_value=self._bf_source_max_value)
 
     def export_brightfield(self, data, path_stem):
-        """The detector-major BF companion file is not built by this backend yet."""
-        return None
+        """Write exact detector counts as detector-major BF columns ``[bf, scan]``
+        (``.u8`` when every count fits, else ``.u16``), as
+        ``SSBEngine.write_exact_bf_source`` (``cuda/engine.py:577-701``);
+        returns (path, seconds)."""
+        import torch
+        t0 = time.perf_counter()
+        eng = self._prepare()
+        if not isinstance(data, torch.Tensor):
+            data = torch.from_numpy(np.ascontiguousarray(np.asarray(data)))
+        if data.dtype not in (torch.uint8, torch.uint16):
+            raise TypeError(f"Exact BF-source export supports uint8 or uint16 counts; got {data.dtype}.")
+        if data.ndim != 4 or tuple(int(v) for v in data.shape[:2]) != self.scan_shape:
+            raise ValueError("BF-source data must match the engine scan shape; "
+                             f"got {tuple(data.shape)}, expected {self.scan_shape} + detector.")
+        if data.dtype == torch.uint8:
+            raise NotImplementedError("uint8 sources are not supported by the hip SSB backend yet.")
+        data = data.to(self._plan.device).contiguous()
+        max_value = eng.max_count(data)
+        narrow = max_value <= 255
+        out_dtype = np.dtype(np.uint8 if narrow else np.uint16)
+        path = pathlib.Path(path_stem).with_suffix(".u8" if narrow else ".u16").expanduser().resolve()
+        path.parent.mkdir(parents=True, exist_ok=True)
+        path.unlink(missing_ok=True)
+        chunk = min(self.num_bf, 256)
+        with path.open("wb") as fh:
+            for b0 in range(0, self.num_bf, chunk):
+                cnt = min(chunk, self.num_bf - b0)
+                cols = eng.bf_columns(data, b0, cnt, narrow)
+                fh.write(cols.cpu().numpy().astype(out_dtype, copy=False).tobytes())
+        self._bf_source_path = path
+        self._bf_source_dtype = out_dtype
+        self._bf_source_max_value = max_value
+        return path, time.perf_counter() - t0
 
     # -------------------------------------------------------- optimisation
     def _evaluate(self, points: np.ndarray) -> np.ndarray:
-        """Exact loss of K (C10, C12, phi12) points, one synchronisation."""
-        import torch
-        eng = self._prepare()
+        """Exact loss of K (C10, C12, phi12) points at the session rotation and
+        higher-order terms, one synchronisation."""
         pts = np.asarray(points, dtype=np.float64).reshape(-1, 3)
-        out = torch.empty(len(pts), dtype=torch.float32, device=self._plan.device)
-        for i, (c10, c12, phi12) in enumerate(pts):
-            # float32 parameters, as the reference evaluates its candidates
-            eng.phase_loss_into(float(np.float32(c10)), float(np.float32(c12)), float(np.float32(phi12)),
-                                out[i:i + 1])
-        return out.cpu().numpy().astype(np.float64)
+        mags, angs, _ = self._ho_arrays_from_aberrations()
+        m = np.repeat(mags[None, :], len(pts), axis=0)
+        a = np.repeat(angs[None, :], len(pts), axis=0)
+        # float32 parameters, as the reference evaluates its candidates
+        m[:, 0] = pts[:, 0].astype(np.float32)
+        m[:, 1] = pts[:, 1].astype(np.float32)
+        a[:, 1] = pts[:, 2].astype(np.float32)
+        losses = self._prepare().batch_phase_loss(m, a)
+        return losses.cpu().numpy().astype(np.float64)
+
+    def _evaluate_params(self, params: list[dict]) -> np.ndarray:
+        """Exact loss of search candidates (``C10_nm``, ``C12_nm``, ``phi12_deg``,
+        ``rotation_angle_deg``, ``C30``), one synchronisation."""
+        mags, angs, _ = self._ho_arrays_from_aberrations()
+        m = np.repeat(mags[None, :], len(params), axis=0)
+        a = np.repeat(angs[None, :], len(params), axis=0)
+        rot = np.empty(len(params), dtype=np.float64)
+        for i, p in enumerate(params):
+            m[i, 0] = np.float32(p["C10_nm"])
+            m[i, 1] = np.float32(p["C12_nm"])
+            a[i, 1] = np.float32(math.radians(p["phi12_deg"]))
+            m[i, 4] = np.float32(p["C30"])
+            rot[i] = math.radians(p["rotation_angle_deg"])
+        losses = self._prepare().batch_phase_loss(m, a, rot)
+        return losses.cpu().numpy().astype(np.float64)
 
     def optimize(self, aberrations=None, rotation_angle_deg=None, n_trials: int = 200, seed: int = 42,
                  verbose: bool = True, bf_subsample: float | None = None,
                  objective_mode: Literal["native", "exact"] = "exact"):
         """Global search (``cuda/backend.py:958-1110``): a seeded batched
-        quasi-random design on the exact objective replaces Optuna TPE."""
+        quasi-random design on the exact objective replaces Optuna TPE.
+
+        Besides the reference's keys (``C10_nm``, ``C12_nm``, ``phi12_deg``
+        and a ``rotation_angle_deg`` range), a ``C30`` range (Angstrom) may be
+        searched; all candidates of a batch are evaluated with one launch
+        sequence and one synchronisation.
+        """
         t0 = time.perf_counter()
         if objective_mode not in {"native", "exact"}:
             raise ValueError(f"objective_mode must be 'native' or 'exact'; got {objective_mode!r}.")
-        if isinstance(rotation_angle_deg, tuple):
-            raise NotImplementedError("rotation search is not supported by the hip SSB backend yet.")
         if bf_subsample is not None and bf_subsample != 1.0:
             raise NotImplementedError("bf_subsample is not supported by the hip SSB backend yet.")
         if aberrations is None:
             aberrations = dict(self._DEFAULT_OPTIMIZE_RANGES)
-        if rotation_angle_deg is not None:
+        spec = dict(aberrations)
+        if isinstance(rotation_angle_deg, tuple):
+            spec["rotation_angle_deg"] = rotation_angle_deg
+        elif rotation_angle_deg is not None:
             self._rotation_angle_rad = math.radians(float(rotation_angle_deg))
         self._optimizer_objective_mode = objective_mode
-        best_params, best_value, history = _opt.batched_search(self._evaluate, aberrations, self.aberrations,
-                                                               int(n_trials), seed=int(seed))
+        defaults = {"C10_nm": float(self.aberrations.get("C10", 0.0)),
+                    "C12_nm": float(self.aberrations.get("C12", 0.0)),
+                    "phi12_deg": math.degrees(float(self.aberrations.get("phi12", 0.0))),
+                    "C30": float(self.aberrations.get("C30", 0.0)),
+                    "rotation_angle_deg": math.degrees(self._rotation_angle_rad)}
+        keys = _opt.PARAM_KEYS + ("C30", "rotation_angle_deg")
+        best_params, best_value, history = _opt.batched_search(self._evaluate_params, spec, defaults,
+                                                               int(n_trials), seed=int(seed), keys=keys)
         self._optuna_trials = history
         self._best_loss = best_value
         for key, abr, conv in (("C10_nm", "C10", None), ("C12_nm", "C12", None),
-                               ("phi12_deg", "phi12", math.radians)):
+                               ("phi12_deg", "phi12", math.radians), ("C30", "C30", None)):
             if key in best_params:
                 v = best_params[key]
             elif key in aberrations and not isinstance(aberrations[key], tuple):
@@ -386,6 +453,9 @@ class HipSSBBackend:
             else:
                 continue
             self.aberrations[abr] = conv(v) if conv else v
+        if "rotation_angle_deg" in best_params:
+            self._rotation_angle_rad = math.radians(best_params["rotation_angle_deg"])
+        self._prepare()
         self._elapsed_optimize = time.perf_counter() - t0
         self._n_trials = int(n_trials)
         if verbose:

@@ -18,6 +18,17 @@ import numpy as np
 from . import _abi
 
 
+def _host(x):
+    """A host copy of a small array given as NumPy, a list or a torch tensor."""
+    try:
+        import torch
+        if isinstance(x, torch.Tensor):
+            return x.detach().cpu().numpy()
+    except ModuleNotFoundError:
+        pass
+    return x
+
+
 def _torch():
     import torch
     if not torch.cuda.is_available():
@@ -197,6 +208,24 @@ class HipSSBEngine:
                   ctypes.byref(c), ctypes.byref(rad), self._stream())
         return (int(r.value), int(c.value)), int(rad.value)
 
+    def max_count(self, data) -> int:
+        """Largest count in resident uint16 data (synchronises)."""
+        self._check_u16(data)
+        out = ctypes.c_uint(0)
+        _abi.call("ssb_max_u16", _abi.ptr(data), int(data.numel()), ctypes.byref(out), self._stream())
+        return int(out.value)
+
+    def bf_columns(self, data, bf_offset: int, count: int, narrow_u8: bool = False, out=None):
+        """Exact detector-major BF columns ``[count, N*N]`` (uint16, or uint8)."""
+        torch = self.torch
+        self._check_u16(data)
+        dt = torch.uint8 if narrow_u8 else torch.uint16
+        if out is None:
+            out = torch.empty((count, self.n * self.n), dtype=dt, device=self.plan.device)
+        _abi.call("ssb_bf_columns_u16", self.plan.handle, _abi.ptr(data), int(bf_offset), int(count),
+                  1 if narrow_u8 else 0, _abi.ptr(out), self._stream())
+        return out
+
     def set_bf(self, rows, cols, center, dc) -> None:
         rows = np.ascontiguousarray(np.asarray(rows, dtype=np.int32).reshape(-1))
         cols = np.ascontiguousarray(np.asarray(cols, dtype=np.int32).reshape(-1))
@@ -316,12 +345,56 @@ class HipSSBEngine:
         _abi.call("ssb_phase_loss", self.plan.handle, float(c10), float(c12), float(phi12), None,
                   _abi.ptr(loss_out), self._stream())
 
+    def _phase_loss_full(self, mags, angles, want_loss: bool):
+        torch = self.torch
+        if not self._has_g:
+            raise ValueError("no resident Fourier stack")
+        if self._rotation is None:
+            self.cache_rotation(0.0)
+        m = np.ascontiguousarray(np.asarray(_host(mags), dtype=np.float64).reshape(-1))
+        a = np.ascontiguousarray(np.asarray(_host(angles), dtype=np.float64).reshape(-1))
+        if m.size != 14 or a.size != 14:
+            raise ValueError("aberration magnitudes and angles must have 14 entries")
+        phase = torch.empty((self.n, self.n), dtype=torch.float32, device=self.plan.device)
+        loss = torch.empty(1, dtype=torch.float32, device=self.plan.device) if want_loss else None
+        _abi.call("ssb_phase_loss_full", self.plan.handle, m.ctypes.data_as(_abi._dp),
+                  a.ctypes.data_as(_abi._dp), _abi.ptr(phase), _abi.ptr(loss), self._stream())
+        return phase, loss
+
+    def batch_phase_loss(self, mags, angles, rotations=None, out=None):
+        """Exact losses of T trials (``mags``/``angles``: [T, 14]; ``rotations``:
+        [T] rad or None) as a float32 device tensor, enqueued without a sync."""
+        torch = self.torch
+        if not self._has_g:
+            raise ValueError("no resident Fourier stack")
+        if self._rotation is None:
+            self.cache_rotation(0.0)
+        m = np.ascontiguousarray(np.asarray(_host(mags), dtype=np.float64).reshape(-1, 14))
+        a = np.ascontiguousarray(np.asarray(_host(angles), dtype=np.float64).reshape(-1, 14))
+        if m.shape != a.shape:
+            raise ValueError("mags and angles must have the same [T, 14] shape")
+        t = m.shape[0]
+        r = None
+        if rotations is not None:
+            r = np.ascontiguousarray(np.asarray(_host(rotations), dtype=np.float64).reshape(-1))
+            if r.size != t:
+                raise ValueError("rotations must have one entry per trial")
+        if out is None:
+            out = torch.empty(t, dtype=torch.float32, device=self.plan.device)
+        _abi.call("ssb_batch_phase_loss", self.plan.handle, int(t), m.ctypes.data_as(_abi._dp),
+                  a.ctypes.data_as(_abi._dp), r.ctypes.data_as(_abi._dp) if r is not None else None,
+                  _abi.ptr(out), self._stream())
+        return out
+
     def reconstruct_full(self, mags, angles):
-        """Mean per-BF phase for the 14-coefficient aberration vector."""
-        raise NotImplementedError("higher-order (C_s) reconstruction is not built in the hip backend yet")
+        """Mean per-BF phase for the 14-coefficient aberration vector
+        (``SSBEngine.reconstruct_full``, ``cuda/engine.py:1208-1296``)."""
+        return self._phase_loss_full(mags, angles, False)[0]
 
     def reconstruct_full_with_loss(self, mags, angles):
-        raise NotImplementedError("higher-order (C_s) reconstruction is not built in the hip backend yet")
+        """(mean per-BF phase, loss) for the 14-coefficient vector."""
+        phase, loss = self._phase_loss_full(mags, angles, True)
+        return phase, float(loss.item())
 
     def reconstruct(self, c10: float, c12: float, phi12: float):
         """Mean of the per-BF phase images (float32 [N, N], on the GPU).

@@ -97,14 +97,12 @@ def nelder_mead(evaluate: Evaluate, x0, xatol: float = 0.1, fatol: float = 1e-8,
     return simplex[best], float(f[best]), n_evals
 
 
-def _split_specs(aberrations: dict, defaults: dict):
-    """Free (key, lo, hi) ranges and fixed values in engine units."""
+def _split_specs(aberrations: dict, defaults: dict, keys):
+    """Free (key, lo, hi) ranges and fixed values, in the search's units."""
     free = []
     fixed = {}
-    engine_default = {"C10_nm": float(defaults.get("C10", 0.0)), "C12_nm": float(defaults.get("C12", 0.0)),
-                      "phi12_deg": math.degrees(float(defaults.get("phi12", 0.0)))}
-    for key in PARAM_KEYS:
-        spec = aberrations.get(key, engine_default[key])
+    for key in keys:
+        spec = aberrations.get(key, defaults.get(key, 0.0))
         if isinstance(spec, tuple):
             lo, hi = float(spec[0]), float(spec[1])
             if not hi >= lo:
@@ -115,23 +113,23 @@ def _split_specs(aberrations: dict, defaults: dict):
     return free, fixed
 
 
-def _to_points(params: list[dict]) -> np.ndarray:
-    pts = np.empty((len(params), 3), dtype=np.float64)
-    for i, p in enumerate(params):
-        pts[i] = (p["C10_nm"], p["C12_nm"], math.radians(p["phi12_deg"]))
-    return pts
+def batched_search(evaluate_params: Callable[[list], np.ndarray], aberrations: dict | None, defaults: dict,
+                   n_trials: int, seed: int = 42, batch: int = 64, refine_fraction: float = 0.25,
+                   keys=PARAM_KEYS):
+    """Quasi-random search over ``keys``; returns (best_params, best_value, history).
 
-
-def batched_search(evaluate: Evaluate, aberrations: dict | None, defaults: dict, n_trials: int,
-                   seed: int = 42, batch: int = 64, refine_fraction: float = 0.25):
-    """Quasi-random search; returns (best_params, best_value, history)."""
+    ``evaluate_params(list_of_dicts)`` returns the losses of a batch of full
+    parameter dicts (every key of ``keys``).  ``defaults`` gives the value of
+    keys absent from ``aberrations``.
+    """
     from scipy.stats import qmc
 
     if n_trials < 1:
         raise ValueError("n_trials must be positive")
     aberrations = dict(DEFAULT_RANGES if aberrations is None else aberrations)
-    free, fixed = _split_specs(aberrations, defaults)
+    free, fixed = _split_specs(aberrations, defaults, keys)
     history: list[dict] = []
+    full_params: list[dict] = []
 
     def run(box, count, rng_seed):
         if count <= 0:
@@ -150,24 +148,21 @@ def batched_search(evaluate: Evaluate, aberrations: dict | None, defaults: dict,
             params.append(p)
         for s in range(0, count, batch):
             chunk = params[s:s + batch]
-            losses = np.asarray(evaluate(_to_points(chunk)), dtype=np.float64)
+            losses = np.asarray(evaluate_params(chunk), dtype=np.float64)
             for p, loss in zip(chunk, losses):
-                history.append({"params": {k: float(p[k]) for k in PARAM_KEYS if k not in fixed},
-                                "loss": float(loss), "_all": p})
+                history.append({"params": {k: float(p[k]) for k, _, _ in free}, "loss": float(loss)})
+                full_params.append(p)
 
     n_local = int(round(n_trials * refine_fraction)) if free else 0
     run([(lo, hi) for _, lo, hi in free], n_trials - n_local, seed)
     best = min(range(len(history)), key=lambda i: (history[i]["loss"], i))
     if n_local:
-        centre = history[best]["_all"]
+        centre = full_params[best]
         box = []
         for key, lo, hi in free:
             half = 0.1 * (hi - lo)
             box.append((max(lo, centre[key] - half), min(hi, centre[key] + half)))
         run(box, n_local, seed + 1)
         best = min(range(len(history)), key=lambda i: (history[i]["loss"], i))
-    best_all = history[best]["_all"]
-    for h in history:
-        h.pop("_all", None)
-    best_params = {k: float(best_all[k]) for k, _, _ in free}
+    best_params = {k: float(full_params[best][k]) for k, _, _ in free}
     return best_params, float(history[best]["loss"]), history

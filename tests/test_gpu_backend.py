@@ -95,3 +95,33 @@ def test_fit_recovers_defocus(cuda):
     assert res.loss <= truth + 1e-6
     assert res.n_trials == 64 and res.refine_method == "nelder-mead" and res.refine_nfev > 4
     assert len(res.optuna_trials) == 64
+
+
+def test_result_routes_higher_order_through_mean_phase(cuda):
+    data = C.poisson_data(124)
+    ab = {"C10": -120.0, "C12": 55.0, "phi12": math.radians(17.0), "C30": 1.5e5}
+    ssb = backend(cuda.from_numpy(data).cuda(), aberrations=dict(ab))
+    res = ssb.result()
+    assert res.loss is None                       # cuda/backend.py:820-826
+    p = O.prepare(data, bf_radius=3, **C.POISSON_KW)
+    ref, definition = O.result_phase(p.G, p.dc, p.geom, ab)
+    assert definition == "mean-bf"
+    rms, p999, _ = C.phase_errors(res.phase.cpu().numpy(), ref)
+    assert rms < 1e-4 and p999 < 3e-4
+    np.testing.assert_allclose(res.amplitude.cpu().numpy(), 1.0, atol=1e-6)
+
+
+@pytest.mark.parametrize("add", [80, 300])
+def test_export_brightfield_is_exact(cuda, tmp_path, add):
+    """Detector-major BF columns equal ``data.reshape(-1, k, k)[:, rows, cols].T``
+    (``tests/test_ssb_cuda_128.py:291-303``); uint8 only when lossless."""
+    data = C.poisson_data(123, add=add)
+    ssb = backend(cuda.from_numpy(data).cuda())
+    path, seconds = ssb.export_brightfield(cuda.from_numpy(data).cuda(), tmp_path / "bf")
+    expect = O.gather_bf_columns(data, ssb.bf_inds_row, ssb.bf_inds_col)
+    dtype = np.uint8 if data.max() <= 255 else np.uint16
+    assert path.suffix == (".u8" if dtype == np.uint8 else ".u16") and seconds >= 0
+    got = np.fromfile(path, dtype=dtype).reshape(ssb.num_bf, 128 * 128)
+    np.testing.assert_array_equal(got, expect.astype(dtype))
+    st = ssb.browser_state()
+    assert st.bf_source_path == path and st.bf_source_max_value == int(data.max())
