@@ -51,6 +51,7 @@ struct ssb_plan {
   float2* echi = nullptr;
   float2* Gt = nullptr;
   size_t Gt_bytes = 0;
+  int H_cap = 0;              // BF planes the row-transformed stack H holds
   // exact per-BF phase path (allocated on first use)
   float2* T = nullptr;        // column-transformed planes of one BF chunk
   float2* M1 = nullptr;       // phase moments per BF split, (sum, compensation)
@@ -176,9 +177,9 @@ int ssb_plan_create(int scan_n, int det_rows, int det_cols, int max_bf, double w
   g_last_error.clear();
   if (!out) return fail(QSSB_ERR_ARG, "plan output pointer is NULL");
   *out = nullptr;
-  if (scan_n != 128 && scan_n != 256 && scan_n != 512)
+  if (scan_n != 128 && scan_n != 256 && scan_n != 512 && scan_n != 1024)
     return fail(QSSB_ERR_UNSUPPORTED, "scan size " + std::to_string(scan_n) +
-                                          " is not supported; use 128, 256 or 512");
+                                          " is not supported; use 128, 256, 512 or 1024");
   if (det_rows < 1 || det_cols < 1 || max_bf < 1)
     return fail(QSSB_ERR_ARG, "detector shape and max_bf must be positive");
   if ((long long)det_rows * det_cols > 65536)
@@ -240,7 +241,6 @@ int ssb_plan_create(int scan_n, int det_rows, int det_cols, int max_bf, double w
   if (e == hipSuccess) e = dmalloc(&p->qy, (size_t)n);
   if (e == hipSuccess) e = dmalloc(&p->qyl, (size_t)n);
   if (e == hipSuccess) e = dmalloc(&p->tw, (size_t)n);
-  if (e == hipSuccess) e = dmalloc(&p->H, (size_t)max_bf * n * (n / 2));
   if (e == hipSuccess) e = dmalloc(&p->S_part, (size_t)p->max_groups * n * n);
   if (e == hipSuccess) e = dmalloc(&p->S, (size_t)n * n);
   if (e == hipSuccess) e = dmalloc(&p->rowph, (size_t)max_bf * n);
@@ -440,6 +440,18 @@ int ssb_get_geometry(ssb_plan* p, float* kx_out, float* ky_out, float* ak_out, i
   return QSSB_OK;
 }
 
+// Row-transformed stack H[b][R_r][slot] for `cap` BF pixels (grown on demand).
+static int alloc_h(ssb_plan* p, int cap) {
+  if (p->H_cap >= cap) return QSSB_OK;
+  dfree(p->H);
+  p->H = nullptr;
+  p->H_cap = 0;
+  hipError_t e = dmalloc(&p->H, (size_t)cap * p->n * (p->n / 2));
+  if (e != hipSuccess) return hip_fail(e, "row-transformed stack allocation");
+  p->H_cap = cap;
+  return QSSB_OK;
+}
+
 static int alloc_gt(ssb_plan* p, int gcols, int nbf) {
   const size_t need = (size_t)nbf * gcols * p->n * sizeof(float2);
   if (p->Gt_bytes < need) {
@@ -466,7 +478,9 @@ int ssb_prepare_g_u16(ssb_plan* p, const uint16_t* data, void* stream) {
     if (info.status != 0 || info.nbf < 1) return fail(QSSB_ERR_STATE, "no valid bright-field selection");
     p->nbf_hint = info.nbf;
   }
-  rc = alloc_gt(p, p->n / 2 + 1, p->max_bf);
+  rc = alloc_gt(p, p->n / 2 + 1, p->nbf_hint);
+  if (rc) return rc;
+  rc = alloc_h(p, p->nbf_hint);
   if (rc) return rc;
   HIP_TRY(launch_gather_rowfft(p->n, data, p->D, p->det_cols, p->bf_rows, p->bf_cols, p->info, p->nbf_hint, p->tw,
                                p->H, s),
@@ -785,6 +799,8 @@ int ssb_recon_u16(ssb_plan* p, const uint16_t* data, double c10, double c12, dou
   rc = ensure_geometry(p, s);
   if (rc) return rc;
   p->gcols = 0;
+  rc = alloc_h(p, p->max_bf);
+  if (rc) return rc;
   mark(4);
   HIP_TRY(launch_gather_rowfft(p->n, data, p->D, p->det_cols, p->bf_rows, p->bf_cols, p->info, p->nbf_hint, p->tw,
                                p->H, s),

@@ -346,8 +346,8 @@ k_loss_final(const float* __restrict__ var_part, int nparts, int NN, float* __re
 // ------------------------------------------------------------------------
 // Launch wrappers
 // ------------------------------------------------------------------------
-int loss_chunk_bf(int N) { return N == 128 ? 512 : (N == 256 ? 128 : 32); }
-int loss_splits(int N) { return N == 128 ? 32 : (N == 256 ? 8 : 2); }
+int loss_chunk_bf(int N) { return N == 128 ? 512 : (N == 256 ? 128 : (N == 512 ? 32 : 8)); }
+int loss_splits(int N) { return N == 128 ? 32 : (N == 256 ? 8 : (N == 512 ? 2 : 1)); }
 
 hipError_t launch_phase_moments(int N, const float2* Gt, int gcols, const BfInfo* info, int nbf_hint, const GeomPtrs& gp,
                                 const float2* rowph, const float2* colph, const float2* echi, const ChiPoly* poly,
@@ -396,6 +396,10 @@ hipError_t launch_phase_moments(int N, const float2* Gt, int gcols, const BfInfo
       case 512:
         QSSB_P1(512, 4, 512);
         QSSB_P2(512, 2);
+        break;
+      case 1024:
+        QSSB_P1(1024, 2, 1024);
+        QSSB_P2(1024, 1);
         break;
       default:
         return hipErrorInvalidValue;

@@ -13,7 +13,7 @@
 
 namespace qssb {
 
-int object_groups(int N) { return N == 128 ? 64 : (N == 256 ? 32 : 8); }
+int object_groups(int N) { return N == 128 ? 64 : (N == 256 ? 32 : (N == 512 ? 8 : 4)); }
 
 // Phasor tables of the factorised overlap e^{i theta} = e^{i qx a_b} e^{i qy b_b},
 // (a_b, b_b) = 2 M k_b: rowph[b][q] for all rows, colph[b][c] for columns
@@ -566,6 +566,9 @@ hipError_t launch_trotter_slots(int N, bool from_h, const float2* src, const BfI
     case 512:
       QSSB_SLOTS(512, 2, 512);
       break;
+    case 1024:
+      QSSB_SLOTS(1024, 1, 1024);
+      break;
     default:
       return hipErrorInvalidValue;
   }
@@ -588,6 +591,10 @@ hipError_t launch_trotter_full(int N, const float2* Gt, const BfInfo* info, cons
       break;
     case 512:
       hipLaunchKernelGGL((k_trotter_full<512, 4, 256>), dim3(grid_items(groups * 128LL, 65536)), dim3(256), 0, s,
+                         Gt, info, gp, rowph, m, o, groups, S_part);
+      break;
+    case 1024:
+      hipLaunchKernelGGL((k_trotter_full<1024, 1, 256>), dim3(grid_items(groups * 1024LL, 65536)), dim3(256), 0, s,
                          Gt, info, gp, rowph, m, o, groups, S_part);
       break;
     default:
@@ -613,6 +620,10 @@ hipError_t launch_finish_object(int N, const float2* S_part, const BfInfo* info,
     case 512:
       hipLaunchKernelGGL((k_ifft_cols<512, 2>), dim3(512 / 2), dim3(256), 0, s, S, tw);
       hipLaunchKernelGGL((k_ifft_rows_phase<512, 2>), dim3(512 / 2), dim3(256), 0, s, S, tw, obj, phase);
+      break;
+    case 1024:
+      hipLaunchKernelGGL((k_ifft_cols<1024, 1>), dim3(1024), dim3(256), 0, s, S, tw);
+      hipLaunchKernelGGL((k_ifft_rows_phase<1024, 1>), dim3(1024), dim3(256), 0, s, S, tw, obj, phase);
       break;
     default:
       return hipErrorInvalidValue;

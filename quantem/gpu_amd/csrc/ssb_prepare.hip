@@ -435,8 +435,13 @@ k_gather_rowfft(const uint16_t* __restrict__ data, int D, int det_cols,
   const int nchunks = (nbf + CH - 1) / CH;
   const long long items = (long long)nchunks * N;
   for (long long item = blockIdx.x; item < items; item += gridDim.x) {
-    const int chunk = (int)(item / N);
-    const int rr = (int)(item - (long long)chunk * N);
+    // XCD-aware order: workgroups go round-robin over the 8 XCDs, so give
+    // XCD x the items (row, chunk) of rows x, x+8, ... with the chunks of one
+    // scan row consecutive -- they read the same frames, and the 128-byte
+    // lines that straddle chunk boundaries then hit in that XCD's L2.
+    const long long xcd = item & 7, t = item >> 3;
+    const int rr = (int)((t / nchunks) * 8 + xcd);
+    const int chunk = (int)(t % nchunks);
     const int b0 = chunk * CH;
     __syncthreads();
     if (tid < CH) {
@@ -713,6 +718,10 @@ hipError_t launch_gather_rowfft(int N, const uint16_t* data, int D, int det_cols
       break;
     case 512:
       hipLaunchKernelGGL((k_gather_rowfft<512, 8>), grid_for(16), dim3(256), 0, s, data, D, det_cols, rows,
+                         cols, info, tw, H);
+      break;
+    case 1024:
+      hipLaunchKernelGGL((k_gather_rowfft<1024, 4>), grid_for(8), dim3(256), 0, s, data, D, det_cols, rows,
                          cols, info, tw, H);
       break;
     default:

@@ -81,9 +81,9 @@ class HipSSBBackend:
         wavelength = _wavelength_angstrom(energy)
         det_gpts = (int(data.shape[2]), int(data.shape[3]))
         scan_gpts = (int(data.shape[0]), int(data.shape[1]))
-        if scan_gpts[0] not in (128, 256, 512):
+        if scan_gpts[0] not in (128, 256, 512, 1024):
             raise NotImplementedError(f"scan size {scan_gpts[0]} is not supported by the hip SSB backend; "
-                                      "use 128, 256 or 512")
+                                      "use 128, 256, 512 or 1024")
         self._detected_bf_radius: float | None = None
         if det_sampling is None:
             radius = self._detect_radius(data, dev, det_gpts, scan_gpts, wavelength)
@@ -495,6 +495,38 @@ class HipSSBBackend:
         self._refine_nfev = int(n_evals)
         if verbose:
             self._print_summary(f"Refine (nelder-mead, {n_evals} evals)", self._elapsed_refine)
+        return self
+
+    _DEFAULT_GRID_HALF_WIDTHS = {"C10_nm": 50, "C12_nm": 20, "phi12_deg": 30}
+    _DEFAULT_GRID_POINTS = {"C10_nm": 21, "C12_nm": 11, "phi12_deg": 13}
+
+    def grid_search(self, window: dict | None = None, n_points: dict | None = None, verbose: bool = True):
+        """Exhaustive (C10, C12, phi12) grid around the current aberrations
+        (``cuda/backend.py:1261-1374``): same default half-widths and point
+        counts, C12 clipped at 0, first minimum (``argmin``) wins.  All points
+        are evaluated in batches of exact losses, one synchronisation each."""
+        from itertools import product
+        t0 = time.perf_counter()
+        window = window or {}
+        n_points = n_points or {}
+        hw = {k: window.get(k, v) for k, v in self._DEFAULT_GRID_HALF_WIDTHS.items()}
+        npt = {k: n_points.get(k, v) for k, v in self._DEFAULT_GRID_POINTS.items()}
+        c10c, c12c = self.aberrations["C10"], self.aberrations["C12"]
+        phic = math.degrees(self.aberrations["phi12"])
+        c10v = np.linspace(c10c - hw["C10_nm"], c10c + hw["C10_nm"], npt["C10_nm"])
+        c12v = np.linspace(max(0, c12c - hw["C12_nm"]), c12c + hw["C12_nm"], npt["C12_nm"])
+        phiv = [math.radians(v) for v in np.linspace(phic - hw["phi12_deg"], phic + hw["phi12_deg"],
+                                                     npt["phi12_deg"])]
+        params = np.array(list(product(c10v, c12v, phiv)), dtype=np.float64)
+        if verbose:
+            print(f"Grid search: {len(params)} combinations ({len(c10v)}x{len(c12v)}x{len(phiv)})")
+        losses = np.concatenate([self._evaluate(params[s:s + 256]) for s in range(0, len(params), 256)])
+        best = int(np.argmin(losses))
+        self._best_loss = float(losses[best])
+        self.aberrations["C10"], self.aberrations["C12"], self.aberrations["phi12"] = (float(v) for v in params[best])
+        self._elapsed_grid = time.perf_counter() - t0
+        if verbose:
+            self._print_summary(f"Grid ({len(params)} evals)", self._elapsed_grid)
         return self
 
     def fit(self, *, trials: int, refinement: str | None, search_ranges, refine_lock, seed: int,

@@ -125,3 +125,16 @@ def test_export_brightfield_is_exact(cuda, tmp_path, add):
     np.testing.assert_array_equal(got, expect.astype(dtype))
     st = ssb.browser_state()
     assert st.bf_source_path == path and st.bf_source_max_value == int(data.max())
+
+
+def test_grid_search_picks_the_grid_minimum(cuda):
+    cfg, data = synthetic(128, 64, {"C10": 200.0}, dose=1e6)
+    from quantem.gpu_amd.ssb.compute.hip.backend import HipSSBBackend
+    ssb = HipSSBBackend(cuda.from_numpy(data).cuda(), semiangle=20.0, scan_sampling=cfg.step_A,
+                        det_sampling=1.25, voltage_kV=300, aberrations={"C10": 180.0, "C12": 0.0, "phi12": 0.0})
+    ssb.grid_search(window={"C10_nm": 40, "C12_nm": 0, "phi12_deg": 0},
+                    n_points={"C10_nm": 9, "C12_nm": 1, "phi12_deg": 1}, verbose=False)
+    grid = np.linspace(140.0, 220.0, 9)
+    losses = [ssb.reconstruct_with_loss(c, 0.0, 0.0)[1] for c in grid]
+    assert ssb.aberrations["C10"] == pytest.approx(grid[int(np.argmin(losses))])
+    assert abs(ssb.aberrations["C10"] - 200.0) <= 10.0

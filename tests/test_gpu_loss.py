@@ -37,7 +37,7 @@ def test_make_engine_128_matches_reference_bars(cuda):
     assert loss == pytest.approx(ref_loss, rel=1e-5, abs=1e-6)
 
 
-@pytest.mark.parametrize("size,num_bf", [(256, 5), (512, 5)])
+@pytest.mark.parametrize("size,num_bf", [(256, 5), (512, 5), (1024, 3)])
 def test_make_engine_large_matches_reference_bars(cuda, size, num_bf):
     case = C.make_engine_case(size, num_bf)
     phase, loss = phase_loss(engine_for_case(cuda, case), *ABER)

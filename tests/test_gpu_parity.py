@@ -118,7 +118,7 @@ def test_fourier_stack_matches_fp64_oracle(cuda):
 
 
 # --------------------------------------------------------------- objects
-@pytest.mark.parametrize("size,num_bf", [(128, 7), (256, 5), (512, 5)])
+@pytest.mark.parametrize("size,num_bf", [(128, 7), (256, 5), (512, 5), (1024, 3)])
 def test_make_engine_object_matches_oracle(cuda, size, num_bf):
     case = C.make_engine_case(size, num_bf)
     eng = engine_for_case(cuda, case)
