@@ -61,6 +61,16 @@ int ssb_bf_columns_u16(ssb_plan* plan, const uint16_t* data, int bf_offset, int 
                        void* stream);
 
 /*
+ * Learned-initializer features of the resident Hermitian stack: for the 8
+ * scan-frequency grid points (qrow_host[i], qcol_host[i]), the 32 x 32
+ * detector window of G around the rounded BF centre as |G|/max|G| and the
+ * phase of G conj(G_centre) / pi; out is [32][32][16] float32 (device).  No
+ * reference counterpart (the network only seeds the aberration search).
+ * Synchronises.
+ */
+int ssb_learn_features(ssb_plan* plan, const int* qrow_host, const int* qcol_host, float* out, void* stream);
+
+/*
  * Supported square scan size for an H x W scan: 128, 256, 512 or 1024, the
  * next one at or above the longest side (cuda/backend.py:238-246); 0 for an
  * empty scan.

@@ -652,6 +652,26 @@ int ssb_bf_columns_u16(ssb_plan* p, const uint16_t* data, int bf_offset, int cou
   return QSSB_OK;
 }
 
+int ssb_learn_features(ssb_plan* p, const int* qrow_host, const int* qcol_host, float* out, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!qrow_host || !qcol_host || !out) return fail(QSSB_ERR_ARG, "pointers must not be NULL");
+  if (p->gcols != p->n / 2 + 1) return fail(QSSB_ERR_STATE, "features read the resident Hermitian stack");
+  for (int i = 0; i < 8; ++i)
+    if (qrow_host[i] < 0 || qrow_host[i] >= p->n || qcol_host[i] < 0 || qcol_host[i] >= p->n)
+      return fail(QSSB_ERR_ARG, "scan-frequency indices out of range");
+  hipStream_t s = as_stream(stream);
+  int* dq = nullptr;
+  HIP_TRY(dmalloc(&dq, 16), "features allocation");
+  hipError_t e = hipMemcpyAsync(dq, qrow_host, 8 * sizeof(int), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(dq + 8, qcol_host, 8 * sizeof(int), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = launch_features_from_g(p->Gt, p->n, p->bf_rows, p->bf_cols, p->info, dq, dq + 8, out, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  dfree(dq);
+  if (e != hipSuccess) return hip_fail(e, "learn_features");
+  return QSSB_OK;
+}
+
 int ssb_scan_target(int H, int W) {
   if (H < 1 || W < 1) return 0;
   const int longest = H > W ? H : W;

@@ -144,4 +144,10 @@ hipError_t launch_phase_moments(int N, const float2* Gt, int gcols, const BfInfo
                                 const float2* tw, float2* T, float2* M1, float2* M2, float* var_part,
                                 float* phase, float* loss, hipStream_t s);
 
+// Learned-initializer features of the resident Hermitian stack at the 8
+// scan-frequency grid points (qrow[i], qcol[i]) (device arrays): out is
+// [32][32][16] (NHWC).
+hipError_t launch_features_from_g(const float2* Gt, int N, const int* bf_rows, const int* bf_cols,
+                                  const BfInfo* info, const int* qrow, const int* qcol, float* out, hipStream_t s);
+
 }  // namespace qssb

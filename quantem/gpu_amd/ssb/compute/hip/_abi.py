@@ -59,6 +59,7 @@ SIGNATURES: dict[str, list] = {
     "ssb_phase_loss_full": [_vp, _dp, _dp, _vp, _vp, _vp],
     "ssb_batch_phase_loss": [_vp, _c_int, _dp, _dp, _dp, _vp, _vp],
     "ssb_scan_target": [_c_int, _c_int],
+    "ssb_learn_features": [_vp, _ip, _ip, _vp, _vp],
     "ssb_max_u16": [_vp, _i64, ctypes.POINTER(ctypes.c_uint), _vp],
     "ssb_bf_columns_u16": [_vp, _vp, _c_int, _c_int, _c_int, _vp, _vp],
     "ssb_detect_bf_radius": [_vp, _c_double, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int),

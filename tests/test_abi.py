@@ -18,7 +18,7 @@ def declared_symbols() -> list[str]:
     names = []
     for header in sorted((ROOT / "include").glob("*.h")):
         text = re.sub(r"/\*.*?\*/", "", header.read_text(), flags=re.S)
-        names += re.findall(r"^\s*(?:int|const char\s*\*)\s+(ssb_\w+)\s*\(", text, flags=re.M)
+        names += re.findall(r"^\s*(?:int|const char\s*\*)\s+((?:ssb|qnn)_\w+)\s*\(", text, flags=re.M)
     return names
 
 

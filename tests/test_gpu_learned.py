@@ -1,0 +1,104 @@
+"""GPU checks of the learned initializer's kernels against its CPU oracle:
+the bf16 MFMA GEMM, the forward-model features, a full forward/backward
+(gradient check), and that training on the GPU lowers the loss."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import learned_oracle as L
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits(torch, x64):
+    """bf16 bit patterns (int16 tensor on the GPU) of an array already on the bf16 grid."""
+    u = np.asarray(x64, dtype=np.float32).view(np.uint32) >> 16
+    return torch.tensor(u.astype(np.uint16).view(np.int16), device="cuda")
+
+
+@pytest.mark.parametrize("M,N,K,transA,transB", [(70, 45, 37, False, False), (130, 64, 144, True, False),
+                                                 (33, 200, 16, False, True), (256, 3, 256, True, True)])
+def test_gemm_bf16_matches_exact_products(cuda, M, N, K, transA, transB):
+    from quantem.gpu_amd.ssb.learned import LearnedInit
+    net = LearnedInit(seed=0)
+    rng = np.random.default_rng(M + N + K)
+    a = L.bf16(rng.standard_normal((M, K)))
+    b = L.bf16(rng.standard_normal((K, N)))
+    bias = rng.standard_normal(N).astype(np.float32)
+    A = _bits(cuda, a.T.copy() if transA else a)
+    B = _bits(cuda, b.T.copy() if transB else b)
+    sam, sak = (1, M) if transA else (K, 1)
+    sbk, sbn = (1, K) if transB else (N, 1)
+    C = cuda.full((M, N), 7.0, dtype=cuda.float32, device="cuda")
+    net._gemm(A, sam, sak, B, sbk, sbn, C, N, 1, cuda.tensor(bias, device="cuda"), M, N, K, relu=True)
+    ref = np.maximum(a @ b + bias, 0.0)
+    np.testing.assert_allclose(C.cpu().numpy(), ref, rtol=1e-5, atol=1e-4)
+    net._gemm(A, sam, sak, B, sbk, sbn, C, N, 1, None, M, N, K, acc=True)
+    np.testing.assert_allclose(C.cpu().numpy(), ref + a @ b, rtol=1e-5, atol=2e-4)
+
+
+def _cfg():
+    from quantem.gpu_amd.ssb.learned import FeatureConfig
+    lam = 0.019687489006849
+    return FeatureConfig.for_scan(128, 1.0 / (64 * 1.25e-3 / lam), 20e-3, 1.25e-3, lam)
+
+
+def test_synthetic_features_match_oracle(cuda):
+    from quantem.gpu_amd.ssb.learned import LearnedInit
+    net = LearnedInit(seed=0)
+    cfg = _cfg()
+    rng = np.random.default_rng(3)
+    theta = np.array([[200.0, 1.0e5, 0.03], [-350.0, -5.0e4, -0.07], [20.0, 0.0, 0.0]])
+    snr = np.array([0.1, 0.5, 0.02])
+    q = np.ascontiguousarray(cfg.q.astype(np.float32).reshape(-1))
+    import ctypes
+    from quantem.gpu_amd.ssb import learned as Lm
+    X = cuda.empty((3, 32, 32, 16), dtype=cuda.float32, device="cuda")
+    th = cuda.tensor(theta, dtype=cuda.float32, device="cuda")
+    sn = cuda.tensor(snr, dtype=cuda.float32, device="cuda")
+    Lm._call("qnn_features_synth", Lm._p(th), Lm._p(sn), 3, cfg.recip, cfg.semiangle_rad, cfg.ang_rad,
+             cfg.wavelength, q.ctypes.data_as(ctypes.c_void_p), ctypes.c_uint(11), Lm._p(X), net._s())
+    got = X.cpu().numpy().astype(np.float64)
+    ref = L.features_synth(theta.astype(np.float32).astype(np.float64), snr.astype(np.float32), cfg.recip,
+                           cfg.semiangle_rad, cfg.ang_rad, cfg.wavelength, cfg.q.astype(np.float32), 11)
+    mag = np.abs(got[..., 0::2] - ref[..., 0::2])
+    assert np.percentile(mag, 99) < 2e-3
+    dphase = np.abs(np.angle(np.exp(1j * math.pi * (got[..., 1::2] - ref[..., 1::2]))))
+    assert np.percentile(dphase, 99) < 2e-2
+    del rng
+
+
+def test_forward_backward_match_oracle(cuda):
+    """Gradient check: GPU loss and every parameter gradient against the FP64
+    oracle with bf16 rounding at the same points."""
+    from quantem.gpu_amd.ssb.learned import LearnedInit
+    net = LearnedInit(seed=5)
+    rng = np.random.default_rng(9)
+    X = rng.standard_normal((4, 32, 32, 16)).astype(np.float32)
+    t = (rng.standard_normal((4, 3)) * 0.5).astype(np.float32)
+    loss, grads = net.loss_and_grad(cuda.tensor(X, device="cuda"), cuda.tensor(t, device="cuda"))
+    params = [(w.cpu().numpy().astype(np.float64), b.cpu().numpy().astype(np.float64)) for w, b in net.params]
+    pred, cache = L.forward(params, X)
+    ref_loss, dpred = L.mse(pred, t.astype(np.float64))
+    ref = L.backward(params, cache, dpred)
+    assert float(loss.item()) == pytest.approx(ref_loss, rel=1e-4)
+    for (gw, gb), (rw, rb) in zip(grads, ref):
+        gw, gb = gw.cpu().numpy(), gb.cpu().numpy()
+        assert np.linalg.norm(gw - rw) <= 2e-3 * np.linalg.norm(rw) + 1e-9
+        assert np.linalg.norm(gb - rb) <= 2e-3 * np.linalg.norm(rb) + 1e-9
+
+
+def test_training_on_gpu_lowers_loss_and_predicts(cuda):
+    from quantem.gpu_amd.ssb.learned import LearnedInit
+    net = LearnedInit(seed=1)
+    cfg = _cfg()
+    hist = net.train(cfg, steps=400, batch=128, lr=2e-3, seed=2, log_every=50)
+    assert hist[-1][1] < 0.3 * hist[0][1]
+    rng = np.random.default_rng(123)
+    X, t = net.synth_batch(cfg, rng, 256, seed=987654)
+    pred = net.predict(X)
+    truth = t.cpu().numpy() * np.array([500.0, 2.0e5, math.radians(5.0)])
+    assert np.corrcoef(pred[:, 0], truth[:, 0])[0, 1] > 0.8
