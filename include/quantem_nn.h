@@ -24,6 +24,12 @@ int qnn_gemm_bf16(const uint16_t* A, long long sam, long long sak, const uint16_
                   float* C, long long scm, long long scn, const float* bias, int M, int N, int K, int relu,
                   int accumulate, void* stream);
 
+/* The same product, row-major C[M][N] (= only), with K split into up to
+ * `splits` slices of whole 16-steps computed in parallel into workspace
+ * (splits * M * N floats) and summed in slice order. */
+int qnn_gemm_bf16_splitk(const uint16_t* A, long long sam, long long sak, const uint16_t* B, long long sbk,
+                         long long sbn, float* C, int M, int N, int K, int splits, float* workspace, void* stream);
+
 /* NHWC fp32 [B][H][W][Cin] -> bf16 columns [(b,oy,ox)][(ky,kx,c)] of a 3x3,
  * stride 2, pad 1 convolution (H, W even). */
 int qnn_im2col(const float* X, int B, int H, int W, int Cin, uint16_t* cols, void* stream);

@@ -249,11 +249,14 @@ int ssb_recon_u16(ssb_plan* plan, const uint16_t* data, double c10, double c12, 
  * qg/ssb/results.py:58-80).  ssb_timing_begin reserves events for up to
  * max_calls recon calls; ssb_timing_end synchronises, writes the summed
  * milliseconds of each stage (ssb_stage_count() entries) and the number of
- * recorded calls, and stops timing.
+ * recorded calls, and stops timing.  boundary_mask selects the stage
+ * boundaries recorded (bit i = boundary before stage i, bit NSTAGES = end;
+ * 0 = all); a stage is reported only when both of its boundaries are, so
+ * a caller can time one kernel without events between the others.
  */
 int ssb_stage_count(void);
 const char* ssb_stage_name(int stage);
-int ssb_timing_begin(ssb_plan* plan, int max_calls);
+int ssb_timing_begin(ssb_plan* plan, int max_calls, int boundary_mask);
 int ssb_timing_end(ssb_plan* plan, float* stage_ms_host, int n_stages, int* n_calls_host);
 
 #ifdef __cplusplus

@@ -62,6 +62,7 @@ struct ssb_plan {
   std::vector<hipEvent_t> events;   // (NSTAGES + 1) per recorded call
   int timing_calls = 0;
   int timing_max = 0;
+  int timing_mask = 0;        // stage boundaries recorded (bit i = boundary i)
 };
 
 namespace {
@@ -803,7 +804,7 @@ int ssb_recon_u16(ssb_plan* p, const uint16_t* data, double c10, double c12, dou
     p->timing_calls += 1;
   }
   auto mark = [&](int i) {
-    if (ev) (void)hipEventRecord(ev[i], s);
+    if (ev && ((p->timing_mask >> i) & 1)) (void)hipEventRecord(ev[i], s);
   };
   const long long frames = (long long)p->n * p->n;
   mark(0);
@@ -853,10 +854,11 @@ static void release_events(ssb_plan* p) {
   p->timing_calls = 0;
 }
 
-int ssb_timing_begin(ssb_plan* p, int max_calls) {
+int ssb_timing_begin(ssb_plan* p, int max_calls, int boundary_mask) {
   int rc = check_plan(p);
   if (rc) return rc;
   if (max_calls < 1 || max_calls > 100000) return fail(QSSB_ERR_ARG, "max_calls must be in [1, 100000]");
+  p->timing_mask = boundary_mask ? boundary_mask : ((1 << (NSTAGES + 1)) - 1);
   release_events(p);
   p->events.resize((size_t)max_calls * (NSTAGES + 1));
   for (auto& e : p->events) {
@@ -880,8 +882,11 @@ int ssb_timing_end(ssb_plan* p, float* stage_ms_host, int n_stages, int* n_calls
   const int calls = p->timing_calls;
   for (int c = 0; c < calls; ++c) {
     hipEvent_t* ev = &p->events[(size_t)c * (NSTAGES + 1)];
-    HIP_TRY(hipEventSynchronize(ev[NSTAGES]), "timing sync");
+    int last = NSTAGES;
+    while (last > 0 && !((p->timing_mask >> last) & 1)) --last;
+    HIP_TRY(hipEventSynchronize(ev[last]), "timing sync");
     for (int i = 0; i < NSTAGES; ++i) {
+      if (!((p->timing_mask >> i) & 1) || !((p->timing_mask >> (i + 1)) & 1)) continue;
       float ms = 0.0f;
       HIP_TRY(hipEventElapsedTime(&ms, ev[i], ev[i + 1]), "timing elapsed");
       stage_ms_host[i] += ms;

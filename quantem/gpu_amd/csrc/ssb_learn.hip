@@ -42,7 +42,7 @@ template <bool RELU>
 __global__ void __launch_bounds__(256)
 k_gemm_bf16(const uint16_t* __restrict__ A, long long sam, long long sak, const uint16_t* __restrict__ B,
             long long sbk, long long sbn, float* __restrict__ C, long long scm, long long scn,
-            const float* __restrict__ bias, int M, int N, int K, int accumulate) {
+            const float* __restrict__ bias, int M, int N, int K, int accumulate, int kslice, long long pstride) {
   __shared__ uint16_t As[64][20];
   __shared__ uint16_t Bs[16][68];
   const int tid = threadIdx.x;
@@ -54,16 +54,20 @@ k_gemm_bf16(const uint16_t* __restrict__ A, long long sam, long long sak, const 
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = float4v{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < K; k0 += 16) {
+  // split K: block z covers [kb, ke) and writes partial plane z
+  const int kb = blockIdx.z * kslice;
+  const int ke = min(K, kb + kslice);
+  C += (long long)blockIdx.z * pstride;
+  for (int k0 = kb; k0 < ke; k0 += 16) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int idx = tid + r * 256;
       const int mm = idx >> 4, kk = idx & 15;
       const int gm = m0 + mm, gk = k0 + kk;
-      As[mm][kk] = (gm < M && gk < K) ? A[(long long)gm * sam + (long long)gk * sak] : (uint16_t)0;
-      const int kb = idx >> 6, nn = idx & 63;
-      const int gk2 = k0 + kb, gn = n0 + nn;
-      Bs[kb][nn] = (gk2 < K && gn < N) ? B[(long long)gk2 * sbk + (long long)gn * sbn] : (uint16_t)0;
+      As[mm][kk] = (gm < M && gk < ke) ? A[(long long)gm * sam + (long long)gk * sak] : (uint16_t)0;
+      const int kr = idx >> 6, nn = idx & 63;
+      const int gk2 = k0 + kr, gn = n0 + nn;
+      Bs[kr][nn] = (gk2 < ke && gn < N) ? B[(long long)gk2 * sbk + (long long)gn * sbn] : (uint16_t)0;
     }
     __syncthreads();
     const int kq = (lane >> 4) * 4;
@@ -98,6 +102,16 @@ k_gemm_bf16(const uint16_t* __restrict__ A, long long sam, long long sak, const 
           *c = accumulate ? *c + v : v;
         }
       }
+}
+
+// C[m][n] = sum_z P[z][m][n] in order (split-K partial planes, contiguous M x N)
+__global__ void __launch_bounds__(256)
+k_sum_partials(const float* __restrict__ P, int splits, long long n, float* __restrict__ C) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    float s = 0.0f;
+    for (int z = 0; z < splits; ++z) s += P[(long long)z * n + i];
+    C[i] = s;
+  }
 }
 
 // NHWC fp32 -> im2col bf16 for a 3x3, stride 2, pad 1 convolution:
@@ -408,10 +422,24 @@ int qnn_gemm_bf16(const uint16_t* A, long long sam, long long sak, const uint16_
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   if (relu)
     hipLaunchKernelGGL(k_gemm_bf16<true>, grid, dim3(256), 0, s, A, sam, sak, B, sbk, sbn, C, scm, scn, bias, M, N,
-                       K, accumulate);
+                       K, accumulate, K, 0LL);
   else
     hipLaunchKernelGGL(k_gemm_bf16<false>, grid, dim3(256), 0, s, A, sam, sak, B, sbk, sbn, C, scm, scn, bias, M, N,
-                       K, accumulate);
+                       K, accumulate, K, 0LL);
+  return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
+int qnn_gemm_bf16_splitk(const uint16_t* A, long long sam, long long sak, const uint16_t* B, long long sbk,
+                         long long sbn, float* C, int M, int N, int K, int splits, float* workspace, void* stream) {
+  if (!A || !B || !C || !workspace || M < 1 || N < 1 || K < 1 || splits < 1) return -1;
+  const int kslice = ((K + splits - 1) / splits + 15) / 16 * 16;
+  const int z = (K + kslice - 1) / kslice;
+  const dim3 grid((unsigned)((N + 63) / 64), (unsigned)((M + 63) / 64), (unsigned)z);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const long long plane = (long long)M * N;
+  hipLaunchKernelGGL(k_gemm_bf16<false>, grid, dim3(256), 0, s, A, sam, sak, B, sbk, sbn, workspace, (long long)N,
+                     1LL, nullptr, M, N, K, 0, kslice, plane);
+  hipLaunchKernelGGL(k_sum_partials, dim3(grid_of(plane)), dim3(256), 0, s, workspace, z, plane, C);
   return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 

@@ -34,20 +34,20 @@ k_dp_partial(const uint16_t* __restrict__ data, long long frames, int D, int fra
   const uint16_t* p = data + f0 * (long long)D + pix0;
   if constexpr (VEC == 8) {
     long long f = f0;
-    for (; f + 4 <= f1; f += 4) {
-      u32x4 v[4];
+    for (; f + 2 <= f1; f += 2) {
+      u32x4 v[2];
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
+      for (int u = 0; u < 2; ++u)
         v[u] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p + (long long)u * D));
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < 2; ++u) {
 #pragma unroll
         for (int w = 0; w < 4; ++w) {
           acc[2 * w] += v[u][w] & 0xFFFFu;
           acc[2 * w + 1] += v[u][w] >> 16;
         }
       }
-      p += 4LL * D;
+      p += 2LL * D;
     }
     for (; f < f1; ++f) {
       u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
@@ -501,18 +501,25 @@ k_gather_rowfft(const uint16_t* __restrict__ data, int D, int det_cols,
 // ------------------------------------------------------------------------
 // Launch wrappers
 // ------------------------------------------------------------------------
-static void range_split(long long frames, int max_ranges, int& ranges, int& fpr) {
-  ranges = (int)((frames + 255) / 256);
-  if (ranges > max_ranges) ranges = max_ranges;
-  if (ranges < 1) ranges = 1;
-  fpr = (int)((frames + ranges - 1) / ranges);
+// Frame ranges of the stream pass: about 1536 workgroups in all (6 per CU,
+// measured best for a 2 GiB stream on MI355X), at most max_ranges, and at
+// most 65536 frames per range so the uint32 partial sums cannot overflow.
+static void range_split(long long frames, int D, int max_ranges, int& ranges, int& fpr) {
+  const long long tiles = (D % 8 == 0) ? (D / 8 + 255) / 256 : (D + 255) / 256;
+  long long r = 1536 / (tiles > 0 ? tiles : 1);
+  const long long min_r = (frames + 65535) / 65536;
+  if (r < min_r) r = min_r;
+  if (r > max_ranges) r = max_ranges;
+  if (r > frames) r = frames;
+  if (r < 1) r = 1;
+  fpr = (int)((frames + r - 1) / r);
   ranges = (int)((frames + fpr - 1) / fpr);
 }
 
 hipError_t launch_mean_dp_stream(const uint16_t* data, long long frames, int D, uint32_t* partial, int max_ranges,
                                  hipStream_t s) {
   int ranges, fpr;
-  range_split(frames, max_ranges, ranges, fpr);
+  range_split(frames, D, max_ranges, ranges, fpr);
   const bool vec = (D % 8 == 0) && ((reinterpret_cast<uintptr_t>(data) & 15u) == 0);
   if (vec) {
     dim3 grid((D / 8 + 255) / 256, ranges);
@@ -527,7 +534,7 @@ hipError_t launch_mean_dp_stream(const uint16_t* data, long long frames, int D, 
 hipError_t launch_mean_dp_reduce(long long frames, int D, const uint32_t* partial, int max_ranges,
                                  unsigned long long* sums, float* mean, hipStream_t s) {
   int ranges, fpr;
-  range_split(frames, max_ranges, ranges, fpr);
+  range_split(frames, D, max_ranges, ranges, fpr);
   hipLaunchKernelGGL(k_dp_reduce, dim3((D + 63) / 64), dim3(256), 0, s, partial, ranges, D, frames, sums, mean);
   return hipGetLastError();
 }

@@ -497,6 +497,89 @@ class HipSSBBackend:
             self._print_summary(f"Refine (nelder-mead, {n_evals} evals)", self._elapsed_refine)
         return self
 
+    def _search_state(self) -> dict:
+        """Current values in the search's parameter names."""
+        return {"C10_nm": float(self.aberrations.get("C10", 0.0)), "C12_nm": float(self.aberrations.get("C12", 0.0)),
+                "phi12_deg": math.degrees(float(self.aberrations.get("phi12", 0.0))),
+                "C30": float(self.aberrations.get("C30", 0.0)),
+                "rotation_angle_deg": math.degrees(self._rotation_angle_rad)}
+
+    def _set_search_state(self, params: dict) -> None:
+        for key, abr, conv in (("C10_nm", "C10", None), ("C12_nm", "C12", None),
+                               ("phi12_deg", "phi12", math.radians), ("C30", "C30", None)):
+            if key in params:
+                self.aberrations[abr] = conv(params[key]) if conv else float(params[key])
+        if "rotation_angle_deg" in params:
+            self._rotation_angle_rad = math.radians(params["rotation_angle_deg"])
+        self._prepare()
+
+    def refine_keys(self, keys=("C10_nm", "C30", "rotation_angle_deg"), scales=(100.0, 1.0e5, 1.0),
+                    xatol: float = 1e-3, fatol: float = 1e-9, max_iter: int = 200, verbose: bool = False):
+        """Nelder-Mead (reference coefficients and tie rule) on the exact
+        objective over any three search parameters, each divided by its scale;
+        every candidate carries its own rotation and C30."""
+        if len(keys) != 3 or len(scales) != 3:
+            raise ValueError("refine_keys optimises exactly three parameters")
+        t0 = time.perf_counter()
+        base = self._search_state()
+
+        def evaluate(points):
+            params = []
+            for x in np.asarray(points, dtype=np.float64).reshape(-1, 3):
+                p = dict(base)
+                for k, sc, v in zip(keys, scales, x):
+                    p[k] = float(v) * sc
+                params.append(p)
+            return self._evaluate_params(params)
+
+        x0 = np.array([base[k] / sc for k, sc in zip(keys, scales)])
+        best_x, best_loss, n_evals = _opt.nelder_mead(evaluate, x0, xatol, fatol, max_iter)
+        self._set_search_state({k: float(v) * sc for k, sc, v in zip(keys, scales, best_x)})
+        self._best_loss = float(best_loss)
+        self._refine_method = "nelder-mead"
+        self._refine_nfev = int(n_evals)
+        self._elapsed_refine = time.perf_counter() - t0
+        if verbose:
+            self._print_summary(f"Refine (nelder-mead, {n_evals} evals)", self._elapsed_refine)
+        return self
+
+    def learned_start(self, net, feature_config) -> dict:
+        """(C10, C30, rotation) predicted by the learned initializer from the
+        resident Fourier stack, in the search's parameter names."""
+        from ...learned import features_from_engine
+        X = features_from_engine(self._prepare(), feature_config)
+        c10, c30, rot = net.predict(X)[0]
+        return {"C10_nm": float(c10), "C30": float(c30), "rotation_angle_deg": math.degrees(float(rot))}
+
+    def fit_learned(self, net, feature_config, *, trials: int = 64, half_widths=None, bounds=None,
+                    refine: bool = True, seed: int = 42, verbose: bool = False):
+        """Learned start, a local batched search around it, then Nelder-Mead.
+
+        ``bounds`` (the global search box) clips the local box; keys absent
+        from it stay at their current values.  The global batched search
+        (``optimize``) is the fallback when no trained network is available.
+        """
+        t0 = time.perf_counter()
+        start = self.learned_start(net, feature_config)
+        half = {"C10_nm": 75.0, "C30": 7.5e4, "rotation_angle_deg": 2.5}
+        half.update(half_widths or {})
+        bounds = bounds or {"C10_nm": (-500.0, 500.0), "C30": (-2.0e5, 2.0e5), "rotation_angle_deg": (-5.0, 5.0)}
+        spec = {k: v for k, v in self._search_state().items() if k not in start}
+        for k, v in start.items():
+            lo, hi = bounds.get(k, (-math.inf, math.inf))
+            c = min(max(v, lo), hi)
+            spec[k] = (max(lo, c - half[k]), min(hi, c + half[k]))
+        best, best_value, history = _opt.batched_search(self._evaluate_params, spec, self._search_state(), trials,
+                                                        seed=seed, keys=_opt.PARAM_KEYS + ("C30", "rotation_angle_deg"))
+        self._set_search_state(best)
+        self._best_loss = best_value
+        self._optuna_trials = history
+        self._n_trials = trials
+        if refine:
+            self.refine_keys(verbose=verbose)
+        self._elapsed_optimize = time.perf_counter() - t0 - self._elapsed_refine
+        return self
+
     _DEFAULT_GRID_HALF_WIDTHS = {"C10_nm": 50, "C12_nm": 20, "phi12_deg": 30}
     _DEFAULT_GRID_POINTS = {"C10_nm": 21, "C12_nm": 11, "phi12_deg": 13}
 

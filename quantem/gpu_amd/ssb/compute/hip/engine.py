@@ -424,9 +424,14 @@ class HipSSBEngine:
         self._device_bf = True
 
     # ------------------------------------------------------------ timing
-    def timing_begin(self, max_calls: int) -> None:
-        """Record per-stage hipEvents for the next ``max_calls`` recon calls."""
-        _abi.call("ssb_timing_begin", self.plan.handle, int(max_calls))
+    def timing_begin(self, max_calls: int, boundaries=None) -> None:
+        """Record per-stage hipEvents for the next ``max_calls`` recon calls;
+        ``boundaries`` (stage indices, the stage count meaning the end) limits
+        the events to those boundaries (default: all)."""
+        mask = 0
+        for b in boundaries or ():
+            mask |= 1 << int(b)
+        _abi.call("ssb_timing_begin", self.plan.handle, int(max_calls), int(mask))
 
     def timing_end(self) -> dict:
         """Summed per-stage milliseconds of the recorded recon calls (synchronises)."""

@@ -36,6 +36,7 @@ def _lib():
     vp, i64, ci, cf = ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int, ctypes.c_float
     sig = {
         "qnn_gemm_bf16": [vp, i64, i64, vp, i64, i64, vp, i64, i64, vp, ci, ci, ci, ci, ci, vp],
+        "qnn_gemm_bf16_splitk": [vp, i64, i64, vp, i64, i64, vp, ci, ci, ci, ci, vp, vp],
         "qnn_im2col": [vp, ci, ci, ci, ci, vp, vp],
         "qnn_col2im": [vp, ci, ci, ci, ci, vp, vp],
         "qnn_cast_bf16": [vp, i64, vp, vp],
@@ -132,6 +133,15 @@ class LearnedInit:
         _call("qnn_gemm_bf16", _p(A), sam, sak, _p(B), sbk, sbn, _p(C), scm, scn, _p(bias), M, N, K,
               1 if relu else 0, 1 if acc else 0, self._s())
 
+    def _gemm_wgrad(self, A, sam, sak, B, sbk, sbn, C, M, N, K):
+        """Weight gradient C[M][N] with a long reduction K, split over K."""
+        splits = max(1, min(64, K // 1024))
+        if splits == 1:
+            self._gemm(A, sam, sak, B, sbk, sbn, C, N, 1, None, M, N, K)
+            return
+        ws = self.torch.empty(splits * M * N, dtype=self.torch.float32, device=self.dev)
+        _call("qnn_gemm_bf16_splitk", _p(A), sam, sak, _p(B), sbk, sbn, _p(C), M, N, K, splits, _p(ws), self._s())
+
     def state_dict(self) -> dict:
         out = {}
         for i, (w, b) in enumerate(self.params):
@@ -190,7 +200,7 @@ class LearnedInit:
             dzh = torch.empty(dy.shape, dtype=torch.int16, device=self.dev)
             _call("qnn_relu_bwd", _p(y) if fi == 0 else None, _p(dy), int(dy.numel()), _p(dz), _p(dzh), self._s())
             dw = torch.empty((fout, fin), dtype=torch.float32, device=self.dev)
-            self._gemm(dzh, 1, fout, xh, fin, 1, dw, fin, 1, None, fout, fin, Bn)
+            self._gemm_wgrad(dzh, 1, fout, xh, fin, 1, dw, fout, fin, Bn)
             db = torch.empty(fout, dtype=torch.float32, device=self.dev)
             _call("qnn_colsum", _p(dz), Bn, fout, _p(db), self._s())
             grads[li] = (dw, db)
@@ -207,7 +217,7 @@ class LearnedInit:
             dzh = torch.empty((M, cout), dtype=torch.int16, device=self.dev)
             _call("qnn_relu_bwd", _p(y), _p(dyv), M * cout, _p(dz), _p(dzh), self._s())
             dw = torch.empty((cout, K), dtype=torch.float32, device=self.dev)
-            self._gemm(dzh, 1, cout, cols, K, 1, dw, K, 1, None, cout, K, M)
+            self._gemm_wgrad(dzh, 1, cout, cols, K, 1, dw, cout, K, M)
             db = torch.empty(cout, dtype=torch.float32, device=self.dev)
             _call("qnn_colsum", _p(dz), M, cout, _p(db), self._s())
             grads[li] = (dw, db)
