@@ -67,6 +67,102 @@ def cpu_baseline(data_host, cfg) -> dict:
                       f"({dt:.2f} s; FP32 NumPy/SciPy restatement, scipy.fft workers={cores})"}
 
 
+def _phase_of(ssb, state):
+    """Mean per-BF phase (the image the exact objective scores) at a search state."""
+    import numpy as np
+    ssb._set_search_state(state)
+    mags = np.zeros(14)
+    mags[0], mags[4] = state["C10_nm"], state["C30"]
+    return ssb.reconstruct_full(mags, np.zeros(14)).cpu().numpy().astype(np.float64)
+
+
+def _rms_wrapped(a, b) -> float:
+    import numpy as np
+    d = np.angle(np.exp(1j * (a - b)))
+    return float(np.sqrt(np.mean(d * d)))
+
+
+def fit_leg(dev, n_sets: int, train_steps: int, search_trials: int, learned_trials: int, refine: str,
+            rounds: int, seed: int = 2027) -> dict:
+    """Aberration fit at scan 256x256, detector 128x128: learned start + a
+    local batched search + refine, against the plain batched search (global
+    Sobol) + the same refine.
+
+    Per held-out dataset (C10 U(+-500 A), C30 U(+-2e5 A), rotation U(+-5 deg),
+    5e4 e-/probe): both fits run from zero aberrations; the converged optimum
+    of the exact objective (the better fit polished by Nelder-Mead, untimed)
+    defines the reference phase, and each fit's phase error is the RMS of the
+    wrapped phase difference to it.  Training time is reported on its own."""
+    import numpy as np
+    import torch
+    from quantem.gpu_amd.ssb.learned import LearnedInit, FeatureConfig
+    from quantem.gpu_amd.ssb.synthetic import SyntheticConfig, generate
+    from quantem.gpu_amd.ssb.compute.hip.backend import HipSSBBackend
+
+    base = SyntheticConfig(scan=SCAN, det=DET)
+    fcfg = FeatureConfig.for_scan(SCAN, base.step_A, base.semiangle_mrad * 1e-3, base.det_sampling_mrad * 1e-3,
+                                  base.wavelength)
+    net = LearnedInit(device=dev, seed=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = net.train(fcfg, steps=train_steps, batch=256, lr=2e-3, seed=2, log_every=max(1, train_steps // 4))
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    log(f"learned initializer trained: {train_steps} steps in {train_s:.1f} s, loss {hist[0][1]:.4f} -> {hist[-1][1]:.4f}")
+    rng = np.random.default_rng(seed)
+    zero = {"C10_nm": 0.0, "C12_nm": 0.0, "phi12_deg": 0.0, "C30": 0.0, "rotation_angle_deg": 0.0}
+    keys = ("C10_nm", "C30", "rotation_angle_deg")
+    rows = []
+    for i in range(n_sets + 1):                     # set 0 warms both paths up and is not reported
+        truth = {"C10": float(rng.uniform(-500, 500)), "C30": float(rng.uniform(-2e5, 2e5))}
+        rot = float(rng.uniform(-5, 5))
+        sc = SyntheticConfig(scan=SCAN, det=DET, aberrations=truth, rotation_deg=rot, dose=5e4, seed=9000 + i)
+        data = generate(sc, device="cuda")
+        ssb = HipSSBBackend(data, semiangle=sc.semiangle_mrad, scan_sampling=sc.step_A,
+                            det_sampling=sc.det_sampling_mrad, voltage_kV=sc.voltage_kV)
+        out = {}
+        for name in ("search", "learned"):
+            ssb._set_search_state(zero)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            if name == "search":
+                ssb.fit_search(trials=search_trials, refine=refine, rounds=rounds)
+            else:
+                ssb.fit_learned(net, fcfg, trials=learned_trials, refine=refine, rounds=rounds)
+            torch.cuda.synchronize()
+            out[name] = (time.perf_counter() - t0, ssb._search_state(), float(ssb._best_loss), ssb._refine_nfev)
+        best = min(out.values(), key=lambda r: r[2])
+        ssb._set_search_state(best[1])
+        ssb.refine_keys(xatol=1e-5, fatol=1e-12, max_iter=600)
+        opt_state = ssb._search_state()
+        ph_opt = _phase_of(ssb, opt_state)
+        row = {"truth": {"C10": truth["C10"], "C30": truth["C30"], "rotation_deg": rot},
+               "optimum": {k: opt_state[k] for k in keys}, "optimum_loss": float(ssb._best_loss)}
+        for name, (secs, state, loss, nfev) in out.items():
+            row[name] = {"s": secs, "loss": loss, "phase_err_rad": _rms_wrapped(_phase_of(ssb, state), ph_opt),
+                         "params": {k: state[k] for k in keys}, "refine_evals": nfev}
+        if i > 0:
+            rows.append(row)
+            log(f"fit set {i}: search {row['search']['s']:.3f} s err {row['search']['phase_err_rad']:.2e}, "
+                f"learned {row['learned']['s']:.3f} s err {row['learned']['phase_err_rad']:.2e}")
+        ssb.close()
+        del data
+    ts = float(np.median([r["search"]["s"] for r in rows]))
+    tl = float(np.median([r["learned"]["s"] for r in rows]))
+    return {
+        "workload": "aberration fit (C10, C30, rotation), scan 256x256, detector 128x128, u16, exact objective",
+        "search_s": ts, "learned_s": tl, "speedup": ts / tl,
+        "search_trials": search_trials, "learned_trials": learned_trials, "refine": refine,
+        "refine_evals": {"search": float(np.median([r["search"]["refine_evals"] for r in rows])),
+                         "learned": float(np.median([r["learned"]["refine_evals"] for r in rows]))},
+        "max_phase_err_rad": {"search": max(r["search"]["phase_err_rad"] for r in rows),
+                              "learned": max(r["learned"]["phase_err_rad"] for r in rows)},
+        "phase_err_bar_rad": 1e-4,
+        "training_s": train_s, "training_steps": train_steps, "training_loss": [hist[0][1], hist[-1][1]],
+        "sets": rows,
+    }
+
+
 def copy_probe() -> int:
     """Five launches of an elementwise copy reading and writing 1 GiB each: the
     reference kernel of known traffic for the FETCH_SIZE/WRITE_SIZE counters."""
@@ -88,6 +184,13 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true",
                     help="skip the CPU baseline (used for profiler runs)")
+    ap.add_argument("--fit-sets", type=int, default=3,
+                    help="held-out datasets of the aberration-fit comparison (0 skips it)")
+    ap.add_argument("--fit-train-steps", type=int, default=1500)
+    ap.add_argument("--fit-search-trials", type=int, default=512)
+    ap.add_argument("--fit-trials", type=int, default=64, help="trials of the local search around the learned start")
+    ap.add_argument("--fit-refine", default="nelder-mead", choices=["nelder-mead", "quadratic"])
+    ap.add_argument("--fit-rounds", type=int, default=4, help="rounds of the quadratic refine")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB float32 copy kernel (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -197,6 +300,12 @@ def main() -> int:
         result["cpu_baseline"] = cpu_baseline(host, cfg)
     else:
         result["cpu_baseline"] = None
+    eng.plan.close()
+    del data
+    if args.fit_sets > 0:
+        log("aberration fit: learned start + batched refine vs plain batched search")
+        result["fit"] = fit_leg(dev, args.fit_sets, args.fit_train_steps, args.fit_search_trials,
+                                args.fit_trials, args.fit_refine, args.fit_rounds)
     print(json.dumps(result), flush=True)
     return 0
 

@@ -196,7 +196,8 @@ int ssb_object(ssb_plan* plan, double c10, double c12, double phi12, void* obj_o
                void* stream);
 
 /*
- * Exact per-BF phase path from the resident Hermitian stack at the cached
+ * Exact per-BF phase path from the resident Fourier stack (Hermitian half
+ * planes, or full planes after ssb_import_g with gcols = N) at the cached
  * rotation: for every BF pixel O_b = IFFT2(G_b conj(Gamma^_b)) with the DC bin
  * set to dc, phi_b = angle(O_b); phase_out = mean_b phi_b ([N][N] float32)
  * and loss_out = mean_R(mean_b phi_b^2 - phase^2) (one float32, device).
@@ -242,6 +243,22 @@ int ssb_batch_phase_loss(ssb_plan* plan, int n_trials, const double* mags_host, 
 int ssb_recon_u16(ssb_plan* plan, const uint16_t* data, double c10, double c12, double phi12,
                   double rotation_rad, float threshold, float bf_radius, void* obj_out, float* phase_out,
                   void* stream);
+
+/*
+ * ssb_recon_u16 with its raw-data stream pass ordered after the stream pass
+ * of `after`'s most recent ssb_recon_u16_after call (a different plan,
+ * usually on another stream): two plans alternating on two streams keep
+ * two reconstructions in flight, so that one's VALU-bound trotter runs
+ * beside the next one's HBM-bound stream pass while the two stream passes
+ * never overlap.  Each plan records its own event after its stream pass;
+ * the first call of a pair (nothing recorded on `after` yet) waits on
+ * nothing.  Results equal ssb_recon_u16's.  No reference counterpart: the
+ * reference reconstructs one dataset at a time (cuda/backend.py:168-851);
+ * this serves series of scans.
+ */
+int ssb_recon_u16_after(ssb_plan* plan, const ssb_plan* after, const uint16_t* data, double c10, double c12,
+                        double phi12, double rotation_rad, float threshold, float bf_radius, void* obj_out,
+                        float* phase_out, void* stream);
 
 /*
  * Stage timing of ssb_recon_u16 with hipEvents recorded on the call's own

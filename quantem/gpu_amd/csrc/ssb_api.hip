@@ -32,6 +32,10 @@ struct ssb_plan {
   float* dp_mean = nullptr;
   int* bf_rows = nullptr;
   int* bf_cols = nullptr;
+  int* gtab = nullptr;   // gather chunk table, int[max_bf + 2]
+  // recorded after the raw-data stream pass of each recon (ssb_recon_u16_after)
+  hipEvent_t ev_stream = nullptr;
+  bool ev_stream_recorded = false;
   BfInfo* info = nullptr;
   float* kx = nullptr;
   float* kxl = nullptr;
@@ -123,7 +127,7 @@ QuadChi quad_chi(const ssb_plan* p, double c10, double c12, double phi12) {
   return m;
 }
 
-int check_plan(ssb_plan* p) {
+int check_plan(const ssb_plan* p) {
   if (!p) return fail(QSSB_ERR_ARG, "plan is NULL");
   g_last_error.clear();
   return QSSB_OK;
@@ -231,6 +235,7 @@ int ssb_plan_create(int scan_n, int det_rows, int det_cols, int max_bf, double w
   if (e == hipSuccess) e = dmalloc(&p->dp_mean, (size_t)p->D);
   if (e == hipSuccess) e = dmalloc(&p->bf_rows, (size_t)max_bf);
   if (e == hipSuccess) e = dmalloc(&p->bf_cols, (size_t)max_bf);
+  if (e == hipSuccess) e = dmalloc(&p->gtab, (size_t)max_bf + 2);
   if (e == hipSuccess) e = dmalloc(&p->info, 1);
   if (e == hipSuccess) e = dmalloc(&p->kx, (size_t)max_bf);
   if (e == hipSuccess) e = dmalloc(&p->kxl, (size_t)max_bf);
@@ -307,6 +312,8 @@ int ssb_plan_destroy(ssb_plan* p) {
   dfree(p->dp_mean);
   dfree(p->bf_rows);
   dfree(p->bf_cols);
+  dfree(p->gtab);
+  if (p->ev_stream) (void)hipEventDestroy(p->ev_stream);
   dfree(p->info);
   dfree(p->kx);
   dfree(p->kxl);
@@ -484,7 +491,7 @@ int ssb_prepare_g_u16(ssb_plan* p, const uint16_t* data, void* stream) {
   rc = alloc_h(p, p->nbf_hint);
   if (rc) return rc;
   HIP_TRY(launch_gather_rowfft(p->n, data, p->D, p->det_cols, p->bf_rows, p->bf_cols, p->info, p->nbf_hint, p->tw,
-                               p->H, s),
+                               p->gtab, p->H, s),
           "gather_rowfft");
   HIP_TRY(launch_trotter_slots(p->n, true, p->H, p->info, geom_ptrs(p), p->rowph, p->colph, p->echi, p->optics, p->tw,
                                p->S_part, p->Gt, false, s),
@@ -792,12 +799,19 @@ int ssb_batch_phase_loss(ssb_plan* p, int n_trials, const double* mags_host, con
   return QSSB_OK;
 }
 
-int ssb_recon_u16(ssb_plan* p, const uint16_t* data, double c10, double c12, double phi12, double rotation_rad,
-                  float threshold, float bf_radius, void* obj_out, float* phase_out, void* stream) {
+static int recon_u16(ssb_plan* p, const ssb_plan* after, const uint16_t* data, double c10, double c12,
+                     double phi12, double rotation_rad, float threshold, float bf_radius, void* obj_out,
+                     float* phase_out, hipStream_t s) {
   int rc = check_plan(p);
   if (rc) return rc;
   if (!data) return fail(QSSB_ERR_ARG, "data is NULL");
-  hipStream_t s = as_stream(stream);
+  if (after == p) return fail(QSSB_ERR_ARG, "a plan cannot be ordered after itself");
+  if (after) {
+    rc = check_plan(after);
+    if (rc) return rc;
+    if (!p->ev_stream) HIP_TRY(hipEventCreateWithFlags(&p->ev_stream, hipEventDisableTiming), "stream event");
+    if (after->ev_stream_recorded) HIP_TRY(hipStreamWaitEvent(s, after->ev_stream, 0), "wait stream pass");
+  }
   hipEvent_t* ev = nullptr;
   if (p->timing_max > 0 && p->timing_calls < p->timing_max) {
     ev = &p->events[(size_t)p->timing_calls * (NSTAGES + 1)];
@@ -809,6 +823,10 @@ int ssb_recon_u16(ssb_plan* p, const uint16_t* data, double c10, double c12, dou
   const long long frames = (long long)p->n * p->n;
   mark(0);
   HIP_TRY(launch_mean_dp_stream(data, frames, p->D, p->dp_partial, p->max_ranges, s), "mean_dp");
+  if (p->ev_stream) {
+    HIP_TRY(hipEventRecord(p->ev_stream, s), "stream pass event");
+    p->ev_stream_recorded = true;
+  }
   mark(1);
   HIP_TRY(launch_mean_dp_reduce(frames, p->D, p->dp_partial, p->max_ranges, p->dp_sums, p->dp_mean, s), "mean_dp");
   mark(2);
@@ -824,7 +842,7 @@ int ssb_recon_u16(ssb_plan* p, const uint16_t* data, double c10, double c12, dou
   if (rc) return rc;
   mark(4);
   HIP_TRY(launch_gather_rowfft(p->n, data, p->D, p->det_cols, p->bf_rows, p->bf_cols, p->info, p->nbf_hint, p->tw,
-                               p->H, s),
+                               p->gtab, p->H, s),
           "gather_rowfft");
   mark(5);
   const QuadChi m = quad_chi(p, c10, c12, phi12);
@@ -838,6 +856,20 @@ int ssb_recon_u16(ssb_plan* p, const uint16_t* data, double c10, double c12, dou
           "finish_object");
   mark(7);
   return QSSB_OK;
+}
+
+int ssb_recon_u16(ssb_plan* p, const uint16_t* data, double c10, double c12, double phi12, double rotation_rad,
+                  float threshold, float bf_radius, void* obj_out, float* phase_out, void* stream) {
+  return recon_u16(p, nullptr, data, c10, c12, phi12, rotation_rad, threshold, bf_radius, obj_out, phase_out,
+                   as_stream(stream));
+}
+
+int ssb_recon_u16_after(ssb_plan* p, const ssb_plan* after, const uint16_t* data, double c10, double c12,
+                        double phi12, double rotation_rad, float threshold, float bf_radius, void* obj_out,
+                        float* phase_out, void* stream) {
+  if (!after) return fail(QSSB_ERR_ARG, "after is NULL");
+  return recon_u16(p, after, data, c10, c12, phi12, rotation_rad, threshold, bf_radius, obj_out, phase_out,
+                   as_stream(stream));
 }
 
 int ssb_stage_count(void) { return NSTAGES; }

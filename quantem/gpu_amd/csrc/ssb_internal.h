@@ -77,9 +77,10 @@ hipError_t launch_detect_bf_radius(const float* mean, int R, int C, df ratio, in
 hipError_t launch_bf_geometry(const int* rows, const int* cols, const BfInfo* info, int max_bf, df recip_y,
                               df recip_x, df cos_a, df sin_a, const Optics& o, float* kx, float* kxl, float* ky,
                               float* kyl, float* ak, hipStream_t s);
+// chunk_tab: device int[max_bf + 2] workspace for the gather chunk table.
 hipError_t launch_gather_rowfft(int N, const uint16_t* data, int D, int det_cols, const int* rows,
-                                const int* cols, const BfInfo* info, int nbf_hint, const float2* tw, float2* H,
-                                hipStream_t s);
+                                const int* cols, const BfInfo* info, int nbf_hint, const float2* tw, int* chunk_tab,
+                                float2* H, hipStream_t s);
 // Exact BF columns out[b][s] (uint16, or uint8 when narrow) for BF pixels
 // [bf_offset, bf_offset + count) of the list rows/cols.
 hipError_t launch_bf_columns(const uint16_t* data, long long frames, int D, int det_cols, const int* rows,

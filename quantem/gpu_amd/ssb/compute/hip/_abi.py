@@ -52,6 +52,8 @@ SIGNATURES: dict[str, list] = {
     "ssb_object": [_vp, _c_double, _c_double, _c_double, _vp, _vp, _vp],
     "ssb_recon_u16": [_vp, _vp, _c_double, _c_double, _c_double, _c_double, _c_float, _c_float, _vp, _vp,
                       _vp],
+    "ssb_recon_u16_after": [_vp, _vp, _vp, _c_double, _c_double, _c_double, _c_double, _c_float, _c_float, _vp,
+                            _vp, _vp],
     "ssb_stage_count": [],
     "ssb_timing_begin": [_vp, _c_int, _c_int],
     "ssb_timing_end": [_vp, _fp, _c_int, ctypes.POINTER(_c_int)],

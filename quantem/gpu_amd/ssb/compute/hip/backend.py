@@ -23,6 +23,17 @@ from . import optimizer as _opt
 from .engine import HipSSBEngine, PlanSpec, SSBPlan, normalize_scan, scan_target
 
 
+def _check_objective_mode(mode) -> None:
+    """Objective modes of ``cuda/backend.py:1021-1026``.  The reference's
+    "native" mode is a sparse-row objective with its own landscape
+    (``cuda/kernels/fft256.py:343-350``); only the exact objective is built."""
+    if mode not in {"native", "exact"}:
+        raise ValueError(f"objective_mode must be 'native' or 'exact'; got {mode!r}.")
+    if mode == "native":
+        raise NotImplementedError("objective_mode='native' (the sparse-row objective) is not supported by the "
+                                  "hip SSB backend; use objective_mode='exact'.")
+
+
 def _wavelength_angstrom(energy_eV: float) -> float:
     from ....optics import electron_wavelength_angstrom
     return electron_wavelength_angstrom(energy_eV)
@@ -422,8 +433,7 @@ class HipSSBBackend:
         sequence and one synchronisation.
         """
         t0 = time.perf_counter()
-        if objective_mode not in {"native", "exact"}:
-            raise ValueError(f"objective_mode must be 'native' or 'exact'; got {objective_mode!r}.")
+        _check_objective_mode(objective_mode)
         if bf_subsample is not None and bf_subsample != 1.0:
             raise NotImplementedError("bf_subsample is not supported by the hip SSB backend yet.")
         if aberrations is None:
@@ -468,8 +478,7 @@ class HipSSBBackend:
         t0 = time.perf_counter()
         if objective_mode is None:
             objective_mode = self._optimizer_objective_mode
-        if objective_mode not in {"native", "exact"}:
-            raise ValueError(f"objective_mode must be 'native' or 'exact'; got {objective_mode!r}.")
+        _check_objective_mode(objective_mode)
         if bf_subsample is not None and bf_subsample != 1.0:
             raise NotImplementedError("bf_subsample is not supported by the hip SSB backend yet.")
         self._optimizer_objective_mode = objective_mode
@@ -543,6 +552,41 @@ class HipSSBBackend:
             self._print_summary(f"Refine (nelder-mead, {n_evals} evals)", self._elapsed_refine)
         return self
 
+    def refine_quadratic(self, keys=("C10_nm", "C30", "rotation_angle_deg"), radius=(75.0, 7.5e4, 2.5),
+                         rounds: int = 4, batch: int = 16, shrink: float = 0.3, bounds=None, seed: int = 0,
+                         verbose: bool = False):
+        """Batched trust-region refinement on the exact objective: ``rounds``
+        batches of ``batch`` candidates (one launch sequence each) plus one
+        model-minimiser evaluation per round.  ``radius`` is the first box's
+        half-width per key; ``bounds`` clips candidates."""
+        t0 = time.perf_counter()
+        base = self._search_state()
+        keys = tuple(keys)
+        bounds = bounds or {}
+        lo = np.array([bounds.get(k, (-math.inf, math.inf))[0] for k in keys])
+        hi = np.array([bounds.get(k, (-math.inf, math.inf))[1] for k in keys])
+
+        def evaluate(points):
+            params = []
+            for x in np.asarray(points, dtype=np.float64).reshape(-1, len(keys)):
+                p = dict(base)
+                p.update({k: float(v) for k, v in zip(keys, x)})
+                params.append(p)
+            return self._evaluate_params(params)
+
+        x0 = np.array([base[k] for k in keys])
+        best_x, best_loss, n_evals = _opt.quadratic_refine(evaluate, x0, np.asarray(radius, dtype=np.float64),
+                                                           rounds=rounds, batch=batch, shrink=shrink, seed=seed,
+                                                           lo=lo, hi=hi)
+        self._set_search_state({k: float(v) for k, v in zip(keys, best_x)})
+        self._best_loss = float(best_loss)
+        self._refine_method = "quadratic"
+        self._refine_nfev = int(n_evals)
+        self._elapsed_refine = time.perf_counter() - t0
+        if verbose:
+            self._print_summary(f"Refine (quadratic, {n_evals} evals)", self._elapsed_refine)
+        return self
+
     def learned_start(self, net, feature_config) -> dict:
         """(C10, C30, rotation) predicted by the learned initializer from the
         resident Fourier stack, in the search's parameter names."""
@@ -551,33 +595,68 @@ class HipSSBBackend:
         c10, c30, rot = net.predict(X)[0]
         return {"C10_nm": float(c10), "C30": float(c30), "rotation_angle_deg": math.degrees(float(rot))}
 
-    def fit_learned(self, net, feature_config, *, trials: int = 64, half_widths=None, bounds=None,
-                    refine: bool = True, seed: int = 42, verbose: bool = False):
-        """Learned start, a local batched search around it, then Nelder-Mead.
+    _LEARNED_HALF_WIDTHS = {"C10_nm": 75.0, "C30": 7.5e4, "rotation_angle_deg": 2.5}
+    _FIT_BOUNDS = {"C10_nm": (-500.0, 500.0), "C30": (-2.0e5, 2.0e5), "rotation_angle_deg": (-5.0, 5.0)}
 
-        ``bounds`` (the global search box) clips the local box; keys absent
-        from it stay at their current values.  The global batched search
-        (``optimize``) is the fallback when no trained network is available.
-        """
+    def _local_refine(self, refine: str, keys, radius, rounds: int, batch: int, bounds, seed: int,
+                      xatol: float, verbose: bool) -> None:
+        if refine == "nelder-mead":
+            self.refine_keys(keys, xatol=xatol, fatol=1e-10, max_iter=400, verbose=verbose)
+        elif refine == "quadratic":
+            self.refine_quadratic(keys, radius=radius, rounds=rounds, batch=batch, bounds=bounds, seed=seed,
+                                  verbose=verbose)
+        elif refine is not None:
+            raise ValueError(f"refine must be 'nelder-mead', 'quadratic' or None; got {refine!r}")
+
+    def fit_learned(self, net, feature_config, *, trials: int = 64, refine: str | None = "nelder-mead",
+                    xatol: float = 1e-4, rounds: int = 4, half_widths=None, bounds=None, seed: int = 42,
+                    verbose: bool = False):
+        """Learned start, a ``trials``-candidate batched Sobol search in a box
+        around it (clipped to ``bounds``, the global search box), then a local
+        refinement on the exact objective: Nelder-Mead over (C10, C30,
+        rotation) with the reference's coefficients (``refine_keys``), or the
+        batched trust-region refine.  The global batched search
+        (``fit_search``) is the fallback when no trained network is available."""
         t0 = time.perf_counter()
         start = self.learned_start(net, feature_config)
-        half = {"C10_nm": 75.0, "C30": 7.5e4, "rotation_angle_deg": 2.5}
+        half = dict(self._LEARNED_HALF_WIDTHS)
         half.update(half_widths or {})
-        bounds = bounds or {"C10_nm": (-500.0, 500.0), "C30": (-2.0e5, 2.0e5), "rotation_angle_deg": (-5.0, 5.0)}
+        bounds = bounds or dict(self._FIT_BOUNDS)
         spec = {k: v for k, v in self._search_state().items() if k not in start}
         for k, v in start.items():
             lo, hi = bounds.get(k, (-math.inf, math.inf))
             c = min(max(v, lo), hi)
             spec[k] = (max(lo, c - half[k]), min(hi, c + half[k]))
-        best, best_value, history = _opt.batched_search(self._evaluate_params, spec, self._search_state(), trials,
-                                                        seed=seed, keys=_opt.PARAM_KEYS + ("C30", "rotation_angle_deg"))
-        self._set_search_state(best)
-        self._best_loss = best_value
-        self._optuna_trials = history
+        if trials > 0:
+            best, best_value, history = _opt.batched_search(self._evaluate_params, spec, self._search_state(),
+                                                            trials, seed=seed, refine_fraction=0.0,
+                                                            keys=_opt.PARAM_KEYS + ("C30", "rotation_angle_deg"))
+            self._set_search_state(best)
+            self._best_loss = best_value
+            self._optuna_trials = history
+        else:
+            self._set_search_state({k: min(max(v, bounds.get(k, (-math.inf, math.inf))[0]),
+                                           bounds.get(k, (-math.inf, math.inf))[1]) for k, v in start.items()})
+        keys = tuple(start)
+        self._local_refine(refine, keys, [half[k] / 2 for k in keys], rounds, 16, bounds, seed, xatol, verbose)
+        self._learned_start = start
         self._n_trials = trials
-        if refine:
-            self.refine_keys(verbose=verbose)
-        self._elapsed_optimize = time.perf_counter() - t0 - self._elapsed_refine
+        self._elapsed_optimize = time.perf_counter() - t0
+        return self
+
+    def fit_search(self, *, trials: int = 512, refine: str | None = "nelder-mead", xatol: float = 1e-4,
+                   rounds: int = 4, bounds=None, seed: int = 42, verbose: bool = False):
+        """Plain batched fit without the network: a ``trials``-candidate global
+        Sobol search over ``bounds`` (C10, C30, rotation), then the same local
+        refinement as ``fit_learned`` from its best point."""
+        t0 = time.perf_counter()
+        bounds = bounds or dict(self._FIT_BOUNDS)
+        self.optimize(aberrations=dict(bounds, C12_nm=0.0, phi12_deg=0.0), n_trials=trials, seed=seed,
+                      verbose=False)
+        keys = tuple(bounds)
+        self._local_refine(refine, keys, [0.05 * (hi - lo) for lo, hi in bounds.values()], rounds, 16, bounds,
+                           seed, xatol, verbose)
+        self._elapsed_optimize = time.perf_counter() - t0
         return self
 
     _DEFAULT_GRID_HALF_WIDTHS = {"C10_nm": 50, "C12_nm": 20, "phi12_deg": 30}

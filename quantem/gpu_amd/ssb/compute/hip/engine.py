@@ -411,13 +411,23 @@ class HipSSBEngine:
         return phase, float(loss.item())
 
     def recon_u16(self, data, c10: float, c12: float, phi12: float, rotation_rad: float = 0.0,
-                  threshold: float = 0.0, bf_radius: float | None = None, obj_out=None, phase_out=None):
-        """End-to-end plain SSB from resident raw data (no host round trip)."""
+                  threshold: float = 0.0, bf_radius: float | None = None, obj_out=None, phase_out=None,
+                  after: "HipSSBEngine | None" = None):
+        """End-to-end plain SSB from resident raw data (no host round trip).
+
+        ``after``: another engine whose most recent ``after``-ordered recon's
+        raw-data stream pass this one's stream pass must follow (two engines
+        on two streams keep two reconstructions in flight)."""
         self._check_u16(data)
-        _abi.call("ssb_recon_u16", self.plan.handle, _abi.ptr(data), float(c10), float(c12),
-                  float(phi12), float(rotation_rad), float(threshold),
-                  float(bf_radius) if bf_radius is not None else 0.0,
-                  _abi.ptr(obj_out), _abi.ptr(phase_out), self._stream())
+        radius = float(bf_radius) if bf_radius is not None else 0.0
+        if after is None:
+            _abi.call("ssb_recon_u16", self.plan.handle, _abi.ptr(data), float(c10), float(c12),
+                      float(phi12), float(rotation_rad), float(threshold), radius,
+                      _abi.ptr(obj_out), _abi.ptr(phase_out), self._stream())
+        else:
+            _abi.call("ssb_recon_u16_after", self.plan.handle, after.plan.handle, _abi.ptr(data), float(c10),
+                      float(c12), float(phi12), float(rotation_rad), float(threshold), radius,
+                      _abi.ptr(obj_out), _abi.ptr(phase_out), self._stream())
         self._rotation = float(rotation_rad)
         self._has_g = False
         self._bf = None            # selected on the device; read back on demand

@@ -12,6 +12,11 @@ done).  Nothing here computes a loss.
   earlier vertex, convergence when the simplex spread is below ``xatol`` and
   the loss spread below ``fatol``, the 4 initial vertices and the 3 shrunk
   vertices evaluated as batches.
+* ``quadratic_refine`` is the batched local refinement: each round
+  evaluates one batch (the centre, the axis points and a Sobol fill of a
+  box), fits a full quadratic model by least squares and moves to its
+  minimiser inside the box; it needs a few batches where Nelder-Mead needs
+  100+ sequential evaluations.  It has no reference counterpart.
 * ``batched_search`` replaces the reference's Optuna TPE loop
   (``cuda/optimizer.py:59-230``), whose trial sequence cannot be reproduced
   offline: a scrambled Sobol design over the search box (seeded), evaluated
@@ -166,3 +171,96 @@ def batched_search(evaluate_params: Callable[[list], np.ndarray], aberrations: d
         best = min(range(len(history)), key=lambda i: (history[i]["loss"], i))
     best_params = {k: float(full_params[best][k]) for k, _, _ in free}
     return best_params, float(history[best]["loss"]), history
+
+
+def _quadratic_terms(u: np.ndarray) -> np.ndarray:
+    """Design matrix [1, u_i, u_i u_j (i <= j)] of a full quadratic in d variables."""
+    k, d = u.shape
+    cols = [np.ones(k)] + [u[:, i] for i in range(d)]
+    cols += [u[:, i] * u[:, j] for i in range(d) for j in range(i, d)]
+    return np.stack(cols, axis=1)
+
+
+def _quadratic_minimiser(u: np.ndarray, f: np.ndarray):
+    """Least-squares quadratic through (u, f); returns the Newton step of the
+    model with its Hessian's eigenvalues replaced by their magnitudes (floored
+    at 1e-3 of the largest), so a saddle or a ridge still gives a descent
+    step, or None when the fit is degenerate."""
+    d = u.shape[1]
+    A = _quadratic_terms(u)
+    if np.linalg.matrix_rank(A) < A.shape[1]:
+        return None
+    coef, *_ = np.linalg.lstsq(A, f, rcond=None)
+    g = coef[1:1 + d]
+    H = np.empty((d, d))
+    idx = 1 + d
+    for i in range(d):
+        for j in range(i, d):
+            H[i, j] = H[j, i] = coef[idx] * (2.0 if i == j else 1.0)
+            idx += 1
+    w, V = np.linalg.eigh(H)
+    top = float(np.max(np.abs(w)))
+    if not np.isfinite(top) or top == 0.0:
+        return None
+    w = np.maximum(np.abs(w), 1e-3 * top)
+    return -(V @ ((V.T @ g) / w))
+
+
+def quadratic_refine(evaluate: Evaluate, x0, radius, rounds: int = 3, batch: int = 16, shrink: float = 0.3,
+                     seed: int = 0, lo=None, hi=None) -> tuple[np.ndarray, float, int]:
+    """Batched trust-region refinement from ``x0``; returns (best_x, best_loss, n_evals).
+
+    Round r evaluates ``batch`` points of the box centre +/- radius_r in one
+    ``evaluate`` call: the centre, the 2d axis points and a scrambled Sobol
+    fill.  A full quadratic (1 + d + d(d+1)/2 coefficients) is fitted to
+    every point seen so far inside the box, in the box's unit coordinates;
+    its Newton step (Hessian eigenvalues taken by magnitude, so an indefinite
+    model still yields a descent step), pulled back into the box, is
+    evaluated on its own.  The next centre is the best point seen so far.
+    When the model's point is that best point (the model is trusted) the box
+    scales by max(shrink, 2 x the step's reach in box units), at most 1;
+    otherwise it halves.  ``lo``/``hi`` clip every candidate.  Ties keep the
+    earliest point.
+    """
+    from scipy.stats import qmc
+
+    x0 = np.asarray(x0, dtype=np.float64).reshape(-1)
+    radius = np.broadcast_to(np.asarray(radius, dtype=np.float64), x0.shape).copy()
+    d = x0.size
+    need = 1 + d + d * (d + 1) // 2
+    if batch < max(need, 1 + 2 * d):
+        raise ValueError(f"batch must hold at least {max(need, 1 + 2 * d)} points for {d} parameters")
+    if np.any(radius <= 0):
+        raise ValueError("radius must be positive")
+    lo = np.full(d, -np.inf) if lo is None else np.asarray(lo, dtype=np.float64)
+    hi = np.full(d, np.inf) if hi is None else np.asarray(hi, dtype=np.float64)
+    sobol = qmc.Sobol(d=d, scramble=True, seed=seed)
+    fill = 2.0 * sobol.random_base2(int(math.ceil(math.log2(max(batch, 2))))) - 1.0
+    axes = np.concatenate([np.eye(d), -np.eye(d)])
+    unit = np.concatenate([np.zeros((1, d)), axes, fill[:batch - 1 - 2 * d]])
+    xs: list[np.ndarray] = []
+    fs: list[float] = []
+    centre = np.clip(x0, lo, hi)
+    for _ in range(rounds):
+        pts = np.clip(centre + unit * radius, lo, hi)
+        f = np.asarray(evaluate(pts), dtype=np.float64).reshape(-1)
+        xs.extend(pts)
+        fs.extend(float(v) for v in f)
+        # every point seen so far inside the current box enters the fit
+        u_all = (np.asarray(xs) - centre) / radius
+        inside = np.max(np.abs(u_all), axis=1) <= 1.0 + 1e-12
+        step = _quadratic_minimiser(u_all[inside], np.asarray(fs)[inside])
+        scale = 0.5
+        if step is not None and np.all(np.isfinite(step)):
+            reach = float(np.max(np.abs(step)))
+            step /= max(1.0, reach)
+            xm = np.clip(centre + step * radius, lo, hi)
+            fm = float(np.asarray(evaluate(xm[None, :]), dtype=np.float64).reshape(-1)[0])
+            xs.append(xm)
+            fs.append(fm)
+            if fm <= min(fs):
+                scale = min(1.0, max(shrink, 2.0 * min(reach, 1.0)))
+        centre = xs[int(np.argmin(fs))].copy()
+        radius = radius * scale
+    best = int(np.argmin(fs))
+    return np.asarray(xs[best]), float(fs[best]), len(fs)

@@ -241,6 +241,24 @@ def test_synthetic_recon_matches_fp64_oracle(cuda, scan, det, ab, rot):
     assert rms2 < 1e-6
 
 
+@pytest.mark.parametrize("scan,radius", [(128, 24.0), (512, 10.0)])
+def test_gather_chunks_split_and_pack_rows(cuda, scan, radius):
+    """An explicit BF radius wider than the disk gives detector rows of up to
+    49 pixels (split across gather chunks of 32) and short rows that pack
+    together; scan 512 gathers 16 pixels per chunk.  Phase vs FP64."""
+    cfg, data = synthetic(scan, 32 if scan == 512 else 128, {"C10": 150.0})
+    p = O.prepare(data, voltage_kV=300, semiangle_mrad=20.0, scan_sampling=cfg.step_A, det_sampling=1.25,
+                  bf_radius=radius)
+    ref = np.angle(O.object_wave(p.G, p.dc, p.geom, 150.0, 0.0, 0.0))
+    eng = engine_for_data(cuda, data, semiangle_mrad=20.0, det_sampling=1.25, step=cfg.step_A)
+    d = cuda.from_numpy(data).cuda()
+    phase = cuda.empty((scan, scan), dtype=cuda.float32, device="cuda")
+    eng.recon_u16(d, 150.0, 0.0, 0.0, 0.0, 0.0, radius, phase_out=phase)
+    assert eng.num_bf == p.bf.rows.size
+    rms, p999, _ = C.phase_errors(phase.cpu().numpy(), ref)
+    assert rms < PHASE_RMS and p999 < PHASE_P999
+
+
 def test_full_size_benchmark_config(cuda):
     """Scan 256x256, detector 128x128, u16, C10 = 200: bit-exact mean DP, phase vs FP64,
     and a bit-identical repeat (size-independent determinism)."""

@@ -53,3 +53,14 @@ def test_synthetic_dataset_contract():
     ph = object_phase(cfg)
     assert abs(np.sqrt(np.mean(ph ** 2)) - 0.1) < 1e-12       # 0.1 rad RMS
     assert cfg.wavelength == pytest.approx(0.019687489006849, rel=1e-12)
+
+
+def test_objective_modes_follow_the_reference_contract():
+    """``cuda/backend.py:1021-1026``: unknown modes are a ValueError; the
+    sparse-row "native" objective is not built and says so."""
+    from quantem.gpu_amd.ssb.compute.hip.backend import _check_objective_mode
+    _check_objective_mode("exact")
+    with pytest.raises(NotImplementedError, match="native"):
+        _check_objective_mode("native")
+    with pytest.raises(ValueError, match="objective_mode"):
+        _check_objective_mode("fast")

@@ -54,3 +54,48 @@ def test_batched_search_is_seeded_and_finds_the_basin():
     assert all(set(h["params"]) == {"C10_nm"} for h in fixed[2])
     with pytest.raises(ValueError):
         opt.batched_search(f, {"C10_nm": (5.0, -5.0)}, {}, 8)
+
+
+def _counting(fn):
+    calls = []
+
+    def evaluate(pts):
+        pts = np.asarray(pts, dtype=np.float64)
+        calls.append(len(pts))
+        return np.array([fn(p) for p in pts])
+    return evaluate, calls
+
+
+def test_quadratic_refine_solves_a_correlated_quadratic_in_one_round():
+    """A full quadratic with a strongly correlated valley (the C10/C30
+    trade-off shape) is fitted exactly, so one round lands on its minimiser."""
+    H = np.array([[2.0, 1.9, 0.1], [1.9, 2.0, 0.0], [0.1, 0.0, 0.5]])
+    xs = np.array([0.3, -0.2, 0.1])
+    f = lambda x: 0.01 + 0.5 * (x - xs) @ H @ (x - xs)
+    evaluate, calls = _counting(f)
+    x, fx, n = opt.quadratic_refine(evaluate, np.zeros(3), radius=1.0, rounds=1, batch=16)
+    assert np.allclose(x, xs, atol=1e-9) and fx == pytest.approx(0.01, abs=1e-12)
+    assert n == 17 and calls == [16, 1]      # the batch, then the model's minimiser
+
+
+def test_quadratic_refine_converges_on_a_non_quadratic_bowl():
+    v = np.array([1.0, -1.0, 0.5]) / 1.5
+    xs = np.array([0.7, -0.4, 0.25])
+    f = lambda x: float(np.sum((x - xs) ** 2 * [1.0, 3.0, 0.3]) + 2.0 * ((x - xs) @ v) ** 4
+                        + 0.3 * ((x - xs) @ v) ** 3)
+    evaluate, calls = _counting(f)
+    x, fx, n = opt.quadratic_refine(evaluate, np.zeros(3), radius=np.array([1.0, 1.0, 1.0]), rounds=7, batch=16)
+    assert np.max(np.abs(x - xs)) < 1e-4
+    assert n == sum(calls) and 7 * 16 < n <= 7 * 17 and calls[0] == 16
+    x2, fx2, _ = opt.quadratic_refine(evaluate, np.zeros(3), radius=1.0, rounds=7, batch=16)
+    assert np.array_equal(x, x2) and fx == fx2   # seeded: deterministic
+
+
+def test_quadratic_refine_falls_back_to_the_best_point_and_respects_bounds():
+    f = lambda x: -float(x @ x)      # concave: no positive-definite model
+    evaluate, _ = _counting(f)
+    x, fx, _ = opt.quadratic_refine(evaluate, np.zeros(2), radius=0.5, rounds=2, batch=8,
+                                    lo=[-0.6, -0.6], hi=[0.6, 0.6])
+    assert fx <= -0.25 and np.all(np.abs(x) <= 0.6)
+    with pytest.raises(ValueError, match="batch"):
+        opt.quadratic_refine(evaluate, np.zeros(3), radius=1.0, batch=8)
