@@ -188,9 +188,11 @@ def main() -> int:
                     help="held-out datasets of the aberration-fit comparison (0 skips it)")
     ap.add_argument("--fit-train-steps", type=int, default=1500)
     ap.add_argument("--fit-search-trials", type=int, default=512)
-    ap.add_argument("--fit-trials", type=int, default=64, help="trials of the local search around the learned start")
+    ap.add_argument("--fit-trials", type=int, default=16, help="trials of the local search around the learned start")
     ap.add_argument("--fit-refine", default="nelder-mead", choices=["nelder-mead", "quadratic"])
     ap.add_argument("--fit-rounds", type=int, default=4, help="rounds of the quadratic refine")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch shapes instead of autotuning them before timing")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB float32 copy kernel (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -230,6 +232,14 @@ def main() -> int:
     for _ in range(max(1, args.warmup)):
         step()
     torch.cuda.synchronize()
+    tuning = None
+    if not args.no_tune:
+        from quantem.gpu_amd.ssb.compute.hip import tuner
+        tuning = tuner.autotune(eng, data, c10=C10, repeats=5)
+        log(f"autotuned launch shapes in {tuning['seconds']:.1f} s: {tuning['config']}")
+        for _ in range(max(1, args.warmup)):
+            step()
+        torch.cuda.synchronize()
     eng.mean_dp(data)
     nbf = eng.select_bf(0.0, None).rows.size
     log(f"warmup done; N_bf = {nbf}")
@@ -293,6 +303,7 @@ def main() -> int:
                            "unit": "GB/s", "frac": recon_achieved / HBM_PEAK_GBS,
                            "algorithmic_bytes_per_recon": recon_bytes},
         "stages_ms_per_recon": {k: v / stages["calls"] for k, v in stages.items() if k != "calls"},
+        "tuning": tuning,
     }
     if not args.no_cpu_baseline:
         log("timing the CPU baseline (FP32 NumPy/SciPy restatement, one reconstruction)")

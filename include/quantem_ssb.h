@@ -261,6 +261,26 @@ int ssb_recon_u16_after(ssb_plan* plan, const ssb_plan* after, const uint16_t* d
                         float* phase_out, void* stream);
 
 /*
+ * Tuning knobs of the reconstruction path (no reference counterpart: the
+ * reference fixes its launch shapes per scan size, cuda/kernels/fft256.py:1051-1058).
+ * QSSB_TUNE_STREAM_WG: workgroups of the raw-data stream pass;
+ * QSSB_TUNE_GATHER_PAIRS: complex FFTs (two BF pixels each) per gather
+ * workgroup; QSSB_TUNE_TROTTER: tile-shape variant of the fused column FFT +
+ * trotter.  ssb_tune_candidates writes the allowed values for a scan size
+ * (the first is the default) and returns their count; ssb_tune_set accepts
+ * one of them or 0 (default); ssb_tune_get reports the value in use.  The
+ * knobs change launch shapes only: the mean DP and BF selection are
+ * bit-identical for every value; gather and trotter shapes are separate
+ * instantiations of the same arithmetic and agree to float rounding.
+ */
+#define QSSB_TUNE_STREAM_WG 0
+#define QSSB_TUNE_GATHER_PAIRS 1
+#define QSSB_TUNE_TROTTER 2
+int ssb_tune_candidates(int scan_n, int knob, int* values_host, int capacity);
+int ssb_tune_set(ssb_plan* plan, int knob, int value);
+int ssb_tune_get(ssb_plan* plan, int knob, int* value_host);
+
+/*
  * Stage timing of ssb_recon_u16 with hipEvents recorded on the call's own
  * stream (the reference keeps wall-clock stage timings in SSBResult.timings,
  * qg/ssb/results.py:58-80).  ssb_timing_begin reserves events for up to

@@ -32,7 +32,11 @@ struct ssb_plan {
   float* dp_mean = nullptr;
   int* bf_rows = nullptr;
   int* bf_cols = nullptr;
-  int* gtab = nullptr;   // gather chunk table, int[max_bf + 2]
+  // tuning knobs (ssb_tune_set): stream-pass workgroups, gather FFT pairs,
+  // trotter tile shape; 0 = built-in default
+  int tune_stream_wg = 0;
+  int tune_gather_pairs = 0;
+  int tune_trotter = 0;
   // recorded after the raw-data stream pass of each recon (ssb_recon_u16_after)
   hipEvent_t ev_stream = nullptr;
   bool ev_stream_recorded = false;
@@ -235,7 +239,6 @@ int ssb_plan_create(int scan_n, int det_rows, int det_cols, int max_bf, double w
   if (e == hipSuccess) e = dmalloc(&p->dp_mean, (size_t)p->D);
   if (e == hipSuccess) e = dmalloc(&p->bf_rows, (size_t)max_bf);
   if (e == hipSuccess) e = dmalloc(&p->bf_cols, (size_t)max_bf);
-  if (e == hipSuccess) e = dmalloc(&p->gtab, (size_t)max_bf + 2);
   if (e == hipSuccess) e = dmalloc(&p->info, 1);
   if (e == hipSuccess) e = dmalloc(&p->kx, (size_t)max_bf);
   if (e == hipSuccess) e = dmalloc(&p->kxl, (size_t)max_bf);
@@ -312,7 +315,6 @@ int ssb_plan_destroy(ssb_plan* p) {
   dfree(p->dp_mean);
   dfree(p->bf_rows);
   dfree(p->bf_cols);
-  dfree(p->gtab);
   if (p->ev_stream) (void)hipEventDestroy(p->ev_stream);
   dfree(p->info);
   dfree(p->kx);
@@ -348,7 +350,7 @@ int ssb_mean_dp_u16(ssb_plan* p, const uint16_t* data, int64_t frames, uint64_t*
   if (rc) return rc;
   if (!data || frames < 1) return fail(QSSB_ERR_ARG, "data is NULL or has no frames");
   hipStream_t s = as_stream(stream);
-  HIP_TRY(launch_mean_dp(data, frames, p->D, p->dp_partial, p->max_ranges, p->dp_sums, p->dp_mean, s),
+  HIP_TRY(launch_mean_dp(data, frames, p->D, p->dp_partial, p->max_ranges, p->tune_stream_wg, p->dp_sums, p->dp_mean, s),
           "mean_dp");
   if (sums_out)
     HIP_TRY(hipMemcpyAsync(sums_out, p->dp_sums, (size_t)p->D * 8, hipMemcpyDeviceToDevice, s), "mean_dp copy");
@@ -491,10 +493,10 @@ int ssb_prepare_g_u16(ssb_plan* p, const uint16_t* data, void* stream) {
   rc = alloc_h(p, p->nbf_hint);
   if (rc) return rc;
   HIP_TRY(launch_gather_rowfft(p->n, data, p->D, p->det_cols, p->bf_rows, p->bf_cols, p->info, p->nbf_hint, p->tw,
-                               p->gtab, p->H, s),
+                               p->tune_gather_pairs, p->H, s),
           "gather_rowfft");
   HIP_TRY(launch_trotter_slots(p->n, true, p->H, p->info, geom_ptrs(p), p->rowph, p->colph, p->echi, p->optics, p->tw,
-                               p->S_part, p->Gt, false, s),
+                               p->S_part, p->Gt, false, p->tune_trotter, s),
           "colfft");
   p->gcols = p->n / 2 + 1;
   p->g_nbf = p->nbf_hint;
@@ -548,7 +550,7 @@ int ssb_object(ssb_plan* p, double c10, double c12, double phi12, void* obj_out,
   HIP_TRY(launch_phasors(p->n, p->info, p->nbf_hint, geom_ptrs(p), m, p->rowph, p->colph, p->echi, s), "phasors");
   if (p->gcols == p->n / 2 + 1) {
     HIP_TRY(launch_trotter_slots(p->n, false, p->Gt, p->info, geom_ptrs(p), p->rowph, p->colph, p->echi, p->optics, p->tw,
-                                 p->S_part, nullptr, true, s),
+                                 p->S_part, nullptr, true, 0, s),
             "trotter");
   } else {
     HIP_TRY(launch_trotter_full(p->n, p->Gt, p->info, geom_ptrs(p), p->rowph, m, p->optics, p->S_part, s),
@@ -721,6 +723,7 @@ static ChiPoly chi_poly(const ssb_plan* p, const double* mags, const double* ang
     cp.K[i] = (float)(2.0 * M_PI / lam * std::pow(lam, n + 1) / (n + 1) * mags[i]);
     cp.ca[i] = (float)std::cos(m * angles[i]);
     cp.sa[i] = (float)std::sin(m * angles[i]);
+    if ((m & 1) && cp.K[i] != 0.0f) cp.odd = 1;
   }
   return cp;
 }
@@ -822,13 +825,13 @@ static int recon_u16(ssb_plan* p, const ssb_plan* after, const uint16_t* data, d
   };
   const long long frames = (long long)p->n * p->n;
   mark(0);
-  HIP_TRY(launch_mean_dp_stream(data, frames, p->D, p->dp_partial, p->max_ranges, s), "mean_dp");
+  HIP_TRY(launch_mean_dp_stream(data, frames, p->D, p->dp_partial, p->max_ranges, p->tune_stream_wg, s), "mean_dp");
   if (p->ev_stream) {
     HIP_TRY(hipEventRecord(p->ev_stream, s), "stream pass event");
     p->ev_stream_recorded = true;
   }
   mark(1);
-  HIP_TRY(launch_mean_dp_reduce(frames, p->D, p->dp_partial, p->max_ranges, p->dp_sums, p->dp_mean, s), "mean_dp");
+  HIP_TRY(launch_mean_dp_reduce(frames, p->D, p->dp_partial, p->max_ranges, p->tune_stream_wg, p->dp_sums, p->dp_mean, s), "mean_dp");
   mark(2);
   HIP_TRY(launch_select_bf(p->dp_mean, p->dp_sums, p->det_rows, p->det_cols, threshold, bf_radius, p->max_bf,
                            p->bf_rows, p->bf_cols, p->info, s),
@@ -842,13 +845,13 @@ static int recon_u16(ssb_plan* p, const ssb_plan* after, const uint16_t* data, d
   if (rc) return rc;
   mark(4);
   HIP_TRY(launch_gather_rowfft(p->n, data, p->D, p->det_cols, p->bf_rows, p->bf_cols, p->info, p->nbf_hint, p->tw,
-                               p->gtab, p->H, s),
+                               p->tune_gather_pairs, p->H, s),
           "gather_rowfft");
   mark(5);
   const QuadChi m = quad_chi(p, c10, c12, phi12);
   HIP_TRY(launch_phasors(p->n, p->info, p->nbf_hint, geom_ptrs(p), m, p->rowph, p->colph, p->echi, s), "phasors");
   HIP_TRY(launch_trotter_slots(p->n, true, p->H, p->info, geom_ptrs(p), p->rowph, p->colph, p->echi, p->optics, p->tw, p->S_part,
-                               nullptr, true, s),
+                               nullptr, true, p->tune_trotter, s),
           "colfft_trotter");
   mark(6);
   HIP_TRY(launch_finish_object(p->n, p->S_part, p->info, p->tw, p->S,
@@ -930,3 +933,65 @@ int ssb_timing_end(ssb_plan* p, float* stage_ms_host, int n_stages, int* n_calls
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------------
+// Tuning knobs
+// ------------------------------------------------------------------------
+static const int kStreamWg[] = {1536, 1024, 2048, 768};
+
+int ssb_tune_candidates(int scan_n, int knob, int* values_host, int capacity) {
+  if (scan_n != 128 && scan_n != 256 && scan_n != 512 && scan_n != 1024)
+    return fail(QSSB_ERR_UNSUPPORTED, "scan size " + std::to_string(scan_n) + " is not supported");
+  int vals[8];
+  int n = 0;
+  switch (knob) {
+    case QSSB_TUNE_STREAM_WG:
+      for (int v : kStreamWg) vals[n++] = v;
+      break;
+    case QSSB_TUNE_GATHER_PAIRS:
+      n = gather_pairs_candidates(scan_n, vals);
+      break;
+    case QSSB_TUNE_TROTTER:
+      n = trotter_variants(scan_n);
+      for (int i = 0; i < n; ++i) vals[i] = i;
+      break;
+    default:
+      return fail(QSSB_ERR_ARG, "unknown tuning knob " + std::to_string(knob));
+  }
+  if (values_host)
+    for (int i = 0; i < n && i < capacity; ++i) values_host[i] = vals[i];
+  return n;
+}
+
+int ssb_tune_set(ssb_plan* p, int knob, int value) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  int vals[8];
+  const int n = ssb_tune_candidates(p->n, knob, vals, 8);
+  if (n < 0) return n;
+  bool ok = (value == 0);
+  for (int i = 0; i < n; ++i) ok = ok || (vals[i] == value);
+  if (!ok) return fail(QSSB_ERR_ARG, "value " + std::to_string(value) + " is not a candidate of knob " +
+                                         std::to_string(knob));
+  switch (knob) {
+    case QSSB_TUNE_STREAM_WG: p->tune_stream_wg = value; break;
+    case QSSB_TUNE_GATHER_PAIRS: p->tune_gather_pairs = value; break;
+    default: p->tune_trotter = value; break;
+  }
+  return QSSB_OK;
+}
+
+int ssb_tune_get(ssb_plan* p, int knob, int* value_host) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!value_host) return fail(QSSB_ERR_ARG, "value_host is NULL");
+  switch (knob) {
+    case QSSB_TUNE_STREAM_WG: *value_host = p->tune_stream_wg ? p->tune_stream_wg : kStreamWg[0]; break;
+    case QSSB_TUNE_GATHER_PAIRS:
+      *value_host = p->tune_gather_pairs ? p->tune_gather_pairs : gather_pairs_default(p->n);
+      break;
+    case QSSB_TUNE_TROTTER: *value_host = p->tune_trotter; break;
+    default: return fail(QSSB_ERR_ARG, "unknown tuning knob " + std::to_string(knob));
+  }
+  return QSSB_OK;
+}

@@ -113,14 +113,17 @@ __device__ __forceinline__ void fft_lds(float2* __restrict__ buf, const float2* 
   fft_stages<N, 1, COUNT, BLOCK, STRIDE, INV>(buf, tw);
 }
 
-// Same Stockham FFT for COUNT sequences stored interleaved, element (s, i)
-// at buf[i * COUNT + s] -- the layout of COUNT adjacent columns of a
-// row-major array copied row by row.  Work items run s fastest, so a wave
-// touches contiguous LDS.  The last stage writes to `out` in the padded
-// per-sequence layout out[s * OSTRIDE + i] (a separate buffer).
-template <int N, int R, int NS, int COUNT, int BLOCK, bool INV, int OSTRIDE>
+// Same Stockham FFT for COUNT sequences stored interleaved in blocks of SB:
+// element (s, i) at buf[(s / SB) * BSTR + i * SB + s % SB] -- SB adjacent
+// columns of a row-major array copied row by row, COUNT / SB such tiles
+// BSTR apart.  Work items run s fastest, so a wave touches contiguous LDS.
+// The last stage writes to `out` in the padded per-sequence layout
+// out[s * OSTRIDE + i] (a separate buffer).
+template <int N, int R, int NS, int COUNT, int BLOCK, bool INV, int OSTRIDE, int SB, int BSTR>
 __device__ __forceinline__ void fft_stage_il(float2* __restrict__ buf, float2* __restrict__ out,
                                              const float2* __restrict__ tw) {
+  static_assert(COUNT % SB == 0, "sequences come in whole tiles");
+  auto at = [](int i, int s) { return (s / SB) * BSTR + i * SB + (s % SB); };
   constexpr int ITEMS = COUNT * (N / R);
   constexpr int PER = (ITEMS + BLOCK - 1) / BLOCK;
   constexpr bool LAST = (NS * R == N);
@@ -135,7 +138,7 @@ __device__ __forceinline__ void fft_stage_il(float2* __restrict__ buf, float2* _
       const int k = j % NS;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        float2 x = buf[(j + r * (N / R)) * COUNT + s];
+        float2 x = buf[at(j + r * (N / R), s)];
         if (NS > 1 && r > 0) {
           float2 w = tw[(r * k * (N / (NS * R))) & (N - 1)];
           if (INV) w.y = -w.y;
@@ -170,29 +173,29 @@ __device__ __forceinline__ void fft_stage_il(float2* __restrict__ buf, float2* _
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         if (LAST) out[s * OSTRIDE + base + r * NS] = y[r];
-        else buf[(base + r * NS) * COUNT + s] = y[r];
+        else buf[at(base + r * NS, s)] = y[r];
       }
     }
   }
   __syncthreads();
 }
 
-template <int N, int NS, int COUNT, int BLOCK, bool INV, int OSTRIDE>
+template <int N, int NS, int COUNT, int BLOCK, bool INV, int OSTRIDE, int SB, int BSTR>
 __device__ __forceinline__ void fft_stages_il(float2* __restrict__ buf, float2* __restrict__ out,
                                               const float2* __restrict__ tw) {
   if constexpr (NS < N) {
     constexpr int R = (N / NS >= 4) ? 4 : 2;
-    fft_stage_il<N, R, NS, COUNT, BLOCK, INV, OSTRIDE>(buf, out, tw);
-    fft_stages_il<N, NS * R, COUNT, BLOCK, INV, OSTRIDE>(buf, out, tw);
+    fft_stage_il<N, R, NS, COUNT, BLOCK, INV, OSTRIDE, SB, BSTR>(buf, out, tw);
+    fft_stages_il<N, NS * R, COUNT, BLOCK, INV, OSTRIDE, SB, BSTR>(buf, out, tw);
   }
 }
 
 // FFT of interleaved sequences in `buf` (destroyed) with the result in `out`
 // (padded per-sequence layout).  Caller synchronises before; ends with a barrier.
-template <int N, int COUNT, int BLOCK, int OSTRIDE, bool INV>
+template <int N, int COUNT, int BLOCK, int OSTRIDE, bool INV, int SB = COUNT, int BSTR = 0>
 __device__ __forceinline__ void fft_lds_il(float2* __restrict__ buf, float2* __restrict__ out,
                                            const float2* __restrict__ tw) {
-  fft_stages_il<N, 1, COUNT, BLOCK, INV, OSTRIDE>(buf, out, tw);
+  fft_stages_il<N, 1, COUNT, BLOCK, INV, OSTRIDE, SB, BSTR>(buf, out, tw);
 }
 
 // Asynchronous global -> LDS copy of 16 bytes per lane (gfx950

@@ -54,6 +54,7 @@ struct ChiPoly {
   float K[14];
   float ca[14];
   float sa[14];
+  int odd;   // any odd-m term (C21, C23, C41, C43, C45) non-zero: chi(-u) != chi(u)
 };
 
 // Matrix of the quadratic aberration phase chi(u) = u^T M u,
@@ -62,12 +63,13 @@ struct QuadChi {
   float mxx, mxy, myy;
 };
 
+// target_wg: workgroups of the stream pass (0 = default 1536).
 hipError_t launch_mean_dp(const uint16_t* data, long long frames, int D, uint32_t* partial, int max_ranges,
-                          unsigned long long* sums, float* mean, hipStream_t s);
+                          int target_wg, unsigned long long* sums, float* mean, hipStream_t s);
 // the two passes of launch_mean_dp, separately (stage timing)
 hipError_t launch_mean_dp_stream(const uint16_t* data, long long frames, int D, uint32_t* partial, int max_ranges,
-                                 hipStream_t s);
-hipError_t launch_mean_dp_reduce(long long frames, int D, const uint32_t* partial, int max_ranges,
+                                 int target_wg, hipStream_t s);
+hipError_t launch_mean_dp_reduce(long long frames, int D, const uint32_t* partial, int max_ranges, int target_wg,
                                  unsigned long long* sums, float* mean, hipStream_t s);
 hipError_t launch_select_bf(const float* mean, const unsigned long long* sums, int det_rows, int det_cols,
                             float threshold, float bf_radius, int max_bf, int* rows, int* cols, BfInfo* info,
@@ -77,9 +79,12 @@ hipError_t launch_detect_bf_radius(const float* mean, int R, int C, df ratio, in
 hipError_t launch_bf_geometry(const int* rows, const int* cols, const BfInfo* info, int max_bf, df recip_y,
                               df recip_x, df cos_a, df sin_a, const Optics& o, float* kx, float* kxl, float* ky,
                               float* kyl, float* ak, hipStream_t s);
-// chunk_tab: device int[max_bf + 2] workspace for the gather chunk table.
+// pairs: complex FFTs (2 BF pixels each) per workgroup, one of
+// gather_pairs_candidates(N) (0 = gather_pairs_default(N)).
+int gather_pairs_default(int N);
+int gather_pairs_candidates(int N, int* out);   // out[2]; returns the count
 hipError_t launch_gather_rowfft(int N, const uint16_t* data, int D, int det_cols, const int* rows,
-                                const int* cols, const BfInfo* info, int nbf_hint, const float2* tw, int* chunk_tab,
+                                const int* cols, const BfInfo* info, int nbf_hint, const float2* tw, int pairs,
                                 float2* H, hipStream_t s);
 // Exact BF columns out[b][s] (uint16, or uint8 when narrow) for BF pixels
 // [bf_offset, bf_offset + count) of the list rows/cols.
@@ -119,9 +124,13 @@ hipError_t launch_phasors(int N, const BfInfo* info, int nbf_hint, const GeomPtr
 // (fused scan-column FFT; from_h) or from the resident stack Gt[b][c][q],
 // c = 0..N/2.  Writes partial planes S_part[g][c][q] (do_trotter) and, if Gt
 // is given with from_h, the resident stack itself.
+// variant: tile shape of the reconstruction path's trotter, 0 (default) to
+// trotter_variants(N) - 1; other paths always use variant 0.
+int trotter_variants(int N);
 hipError_t launch_trotter_slots(int N, bool from_h, const float2* src, const BfInfo* info, const GeomPtrs& gp,
                                 const float2* rowph, const float2* colph, const float2* echi, const Optics& o,
-                                const float2* tw, float2* S_part, float2* Gt, bool do_trotter, hipStream_t s);
+                                const float2* tw, float2* S_part, float2* Gt, bool do_trotter, int variant,
+                                hipStream_t s);
 // Trotter from an imported full-plane (non-Hermitian) stack Gt[b][c][q], c < N.
 hipError_t launch_trotter_full(int N, const float2* Gt, const BfInfo* info, const GeomPtrs& gp, const float2* rowph,
                                QuadChi m, const Optics& o, float2* S_part, hipStream_t s);

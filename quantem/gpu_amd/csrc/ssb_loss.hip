@@ -165,7 +165,13 @@ k_corr_colifft(const float2* __restrict__ Gt, const BfInfo* __restrict__ info, G
         }
         eB = cmul(rowA, (j == 0) ? colHalf : cconj(colp));
       }
-      float2 gA, gB;
+      // Mirror points with an even chi (every quadratic chi; a full chi
+      // without odd-m terms): B sees the apertures swapped, the phasors
+      // conjugated and conj(G), so Gamma_B = -conj(Gamma_A) with the same
+      // norm and C_B = -conj(C_A) exactly (every operation is
+      // sign-symmetric); it is not recomputed.
+      const bool same = mirror && !FULL && (!POLY || !cp.odd);
+      float2 gA, gB = make_float2(0.f, 0.f);
       if constexpr (POLY) {
         // chi at Q -/+ k split into even and odd parts: chi(-u) = E(u) - O(u)
         const float qxf = qxA.hi, qyf = qy.hi;
@@ -173,7 +179,8 @@ k_corr_colifft(const float2* __restrict__ Gt, const BfInfo* __restrict__ info, G
         chi_parts(cp, qxf - kx.hi, qyf - ky.hi, em, om);
         chi_parts(cp, qxf + kx.hi, qyf + ky.hi, ep, op);
         gA = gamma_hat_chi(ak, am, app, em + om - chik, ep + op - chik);
-        if (mirror) {
+        if (same) {
+        } else if (mirror) {
           gB = gamma_hat_chi(ak, bm, bp, ep - op - chik, em - om - chik);
         } else {
           const float qxb = (j == 0) ? qxf : qxB0.hi;
@@ -185,10 +192,10 @@ k_corr_colifft(const float2* __restrict__ Gt, const BfInfo* __restrict__ info, G
         }
       } else {
         gA = gamma_hat(ak, eA, am, app, echiA);
-        gB = gamma_hat(ak, eB, bm, bp, echiB);
+        if (!same) gB = gamma_hat(ak, eB, bm, bp, echiB);
       }
       float2 CA = mul_conj(GA, gA);
-      const float2 CB = mul_conj(GB, gB);
+      const float2 CB = same ? make_float2(-CA.x, CA.y) : mul_conj(GB, gB);
       if (j == 0 && q == 0) CA = dcv;
       const int rowB = (j == 0) ? q : qm;
       buf[s * STRIDE + q] = CA;

@@ -84,6 +84,8 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
   // slot groups sharing one 128-byte line of H
   constexpr int LG = (FROM_H && SG * 8 < 128) ? 128 / (SG * 8) : 1;
   static_assert(TILE % 128 == 0, "tile must split into whole waves of 16-byte copies");
+  // a lane's 16-byte copy takes two adjacent slots of one row of H
+  static_assert(!FROM_H || SG % 2 == 0, "row-transformed stacks need an even slot count");
   // Async landing buffers (double-buffered).  FROM_H: interleaved rows of H,
   // element (s, r) at [r * SG + s]; otherwise Gt columns, (s, q) at [s * N + q].
   __shared__ __attribute__((aligned(16))) float2 abuf[2][TILE];
@@ -274,17 +276,31 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
           aperture_pair<ISO>(qxA, qy, kx, ky, cb, q2A[i], ap, o, am, app);
         }
         const float2 eA = cmul(rowA, colp[i]);
+        // Mirror points (j >= 1, q != N/2): output B sees the apertures
+        // swapped, conj(e^{i theta}), the same e^{i chi} and conj(G), so its
+        // Gamma is -conj(Gamma_A) with the same norm and its term is exactly
+        // -conj(term_A) (every operation is sign-symmetric, Kahan included):
+        // B is not accumulated and is written as -conj(A) at the end.
         const bool mirror = (j != 0) && (q != HALF);
-        float bm, bp;
-        float2 eB, echiB;
-        if (mirror) {
-          bm = app;
-          bp = am;
-          eB = cconj(eA);
-          echiB = echiA[i];
-        } else {
+        if (am != 0.0f || app != 0.0f) {
+          const float wr = (am - app) * echiA[i].x;
+          const float wi = -(am + app) * echiA[i].y;
+          float gr = ak * (eA.x * wr - eA.y * wi);
+          float gi = ak * (eA.x * wi + eA.y * wr);
+          const float mag2 = gr * gr + gi * gi;
+          const float sc = (mag2 > 1e-16f) ? rsqrtf(mag2) : 1e8f;
+          gr *= sc;
+          gi *= sc;
+          arA[i].add(GA.x * gr + GA.y * gi);
+          aiA[i].add(GA.y * gr - GA.x * gi);
+        }
+        // wave-uniform test first: most waves hold only mirror points and
+        // branch over the B path
+        const bool wave_b = __ballot(!mirror) != 0ull;
+        if (wave_b) if (!mirror) {
           // j == 0: row q, column N/2; q == N/2: row N/2, column N-j
-          echiB = (j == 0) ? ehalfc[q] : enyqr[s];
+          const float2 echiB = (j == 0) ? ehalfc[q] : enyqr[s];
+          float bm, bp;
           if constexpr (ISO) {
             // j == 0: row q, column N/2; q == N/2: row N/2 (this row), column -qy
             const df dA = (j == 0) ? df_add(rowTA, colA_s[cur][SG]) : df_add(rowTA, colB_s[cur][s]);
@@ -298,31 +314,19 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
             aperture_pair<ISO>(qxB, qyB, kx, ky, cb, q2B, ap, o, bm, bp);
           }
           // rows q and -q coincide at q == N/2, so the row phasor is rowA
-          eB = cmul(rowA, (j == 0) ? colHalf : cconj(colp[i]));
-        }
-        if (am != 0.0f || app != 0.0f) {
-          const float wr = (am - app) * echiA[i].x;
-          const float wi = -(am + app) * echiA[i].y;
-          float gr = ak * (eA.x * wr - eA.y * wi);
-          float gi = ak * (eA.x * wi + eA.y * wr);
-          const float mag2 = gr * gr + gi * gi;
-          const float sc = (mag2 > 1e-16f) ? rsqrtf(mag2) : 1e8f;
-          gr *= sc;
-          gi *= sc;
-          arA[i].add(GA.x * gr + GA.y * gi);
-          aiA[i].add(GA.y * gr - GA.x * gi);
-        }
-        if (bm != 0.0f || bp != 0.0f) {
-          const float wr = (bm - bp) * echiB.x;
-          const float wi = -(bm + bp) * echiB.y;
-          float gr = ak * (eB.x * wr - eB.y * wi);
-          float gi = ak * (eB.x * wi + eB.y * wr);
-          const float mag2 = gr * gr + gi * gi;
-          const float sc = (mag2 > 1e-16f) ? rsqrtf(mag2) : 1e8f;
-          gr *= sc;
-          gi *= sc;
-          arB[i].add(GB.x * gr + GB.y * gi);
-          aiB[i].add(GB.y * gr - GB.x * gi);
+          const float2 eB = cmul(rowA, (j == 0) ? colHalf : cconj(colp[i]));
+          if (bm != 0.0f || bp != 0.0f) {
+            const float wr = (bm - bp) * echiB.x;
+            const float wi = -(bm + bp) * echiB.y;
+            float gr = ak * (eB.x * wr - eB.y * wi);
+            float gi = ak * (eB.x * wi + eB.y * wr);
+            const float mag2 = gr * gr + gi * gi;
+            const float sc = (mag2 > 1e-16f) ? rsqrtf(mag2) : 1e8f;
+            gr *= sc;
+            gi *= sc;
+            arB[i].add(GB.x * gr + GB.y * gi);
+            aiB[i].add(GB.y * gr - GB.x * gi);
+          }
         }
         if (Gt) {
           float2* gb = Gt + (long long)b * GCOLS * N;
@@ -342,8 +346,11 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
         const int j = sg * SG + s;
         const int colB = (j == 0) ? HALF : N - j;
         const int rowB = (j == 0) ? q : qm;
-        sp[(long long)j * N + q] = make_float2(arA[i].s - arA[i].c, aiA[i].s - aiA[i].c);
-        sp[(long long)colB * N + rowB] = make_float2(arB[i].s - arB[i].c, aiB[i].s - aiB[i].c);
+        const float2 outA = make_float2(arA[i].s - arA[i].c, aiA[i].s - aiA[i].c);
+        const bool mirror = (j != 0) && (q != HALF);
+        sp[(long long)j * N + q] = outA;
+        sp[(long long)colB * N + rowB] =
+            mirror ? make_float2(-outA.x, outA.y) : make_float2(arB[i].s - arB[i].c, aiB[i].s - aiB[i].c);
       }
     }
   }
@@ -537,9 +544,16 @@ hipError_t launch_phasors(int N, const BfInfo* info, int nbf_hint, const GeomPtr
   return hipGetLastError();
 }
 
+// Tile shapes (SG slot columns, BLOCK threads) of the fused column FFT +
+// trotter on the row-transformed stack (the reconstruction path); variant 0
+// is the default, the others are the tuner's candidates.  SG is even there:
+// each lane's 16-byte staging copy moves two adjacent slots of a row of H.
+int trotter_variants(int N) { return N == 1024 ? 1 : 3; }
+
 hipError_t launch_trotter_slots(int N, bool from_h, const float2* src, const BfInfo* info, const GeomPtrs& gp,
                                 const float2* rowph, const float2* colph, const float2* echi, const Optics& o,
-                                const float2* tw, float2* S_part, float2* Gt, bool do_trotter, hipStream_t s) {
+                                const float2* tw, float2* S_part, float2* Gt, bool do_trotter, int variant,
+                                hipStream_t s) {
   df rout2;
   const ApIso ap = iso_aperture(o, rout2);
   const int groups = object_groups(N);
@@ -556,22 +570,44 @@ hipError_t launch_trotter_slots(int N, bool from_h, const float2* src, const BfI
     else if (iso) QSSB_SLOTS_ONE(NN, SGV, BL, false, true);                                   \
     else QSSB_SLOTS_ONE(NN, SGV, BL, false, false);                                           \
   } while (0)
+#define QSSB_SLOTS_TUNED(NN, SGV, BL)                                                          \
+  do {                                                                                        \
+    const unsigned grid = grid_items((long long)groups * ((NN / 2) / SGV), 65536);            \
+    QSSB_SLOTS_ONE(NN, SGV, BL, true, true);                                                  \
+  } while (0)
+  const bool tuned = from_h && iso && variant > 0;
   switch (N) {
     case 128:
-      QSSB_SLOTS(128, 4, 256);
+      if (tuned && variant == 1) QSSB_SLOTS_TUNED(128, 2, 256);
+      else if (tuned && variant == 2) QSSB_SLOTS_TUNED(128, 8, 256);
+      else QSSB_SLOTS(128, 4, 256);
       break;
     case 256:
-      QSSB_SLOTS(256, 2, 256);
+      if (tuned && variant == 1) QSSB_SLOTS_TUNED(256, 4, 256);
+      else if (tuned && variant == 2) QSSB_SLOTS_TUNED(256, 2, 512);
+      else QSSB_SLOTS(256, 2, 256);
       break;
     case 512:
-      QSSB_SLOTS(512, 2, 512);
+      if (tuned && variant == 1) QSSB_SLOTS_TUNED(512, 2, 1024);
+      else if (tuned && variant == 2) QSSB_SLOTS_TUNED(512, 4, 512);
+      else QSSB_SLOTS(512, 2, 512);
       break;
     case 1024:
-      QSSB_SLOTS(1024, 1, 1024);
+      if (from_h) {
+        // one slot per tile would split the 16-byte staging copies across rows
+        const unsigned grid = grid_items((long long)groups * ((1024 / 2) / 2), 65536);
+        if (iso) QSSB_SLOTS_ONE(1024, 2, 1024, true, true);
+        else QSSB_SLOTS_ONE(1024, 2, 1024, true, false);
+      } else {
+        const unsigned grid = grid_items((long long)groups * ((1024 / 2) / 1), 65536);
+        if (iso) QSSB_SLOTS_ONE(1024, 1, 1024, false, true);
+        else QSSB_SLOTS_ONE(1024, 1, 1024, false, false);
+      }
       break;
     default:
       return hipErrorInvalidValue;
   }
+#undef QSSB_SLOTS_TUNED
 #undef QSSB_SLOTS
 #undef QSSB_SLOTS_ONE
   return hipGetLastError();

@@ -414,83 +414,8 @@ k_bf_geometry(const int* __restrict__ bf_rows, const int* __restrict__ bf_cols,
 }
 
 // ------------------------------------------------------------------------
-// Gather chunk table: BF pixels (raster order) split into chunks of at most
-// CH pixels that never split a detector row unless the row alone holds
-// more than CH pixels (such a row tops up the open chunk, then fills
-// whole chunks).  A frame's 128-byte lines under a detector row are
-// then read by one chunk only.  Row runs are packed greedily in order.
-// tab[0] = chunk count, tab[1 + k] = first BF index of chunk k,
-// tab[1 + count] = nbf.  One workgroup of MASK_THREADS threads.
-// ------------------------------------------------------------------------
-#define CHUNK_RUNS 4096
-__global__ void __launch_bounds__(MASK_THREADS)
-k_gather_chunks(const int* __restrict__ bf_rows, const BfInfo* __restrict__ info, int ch, int* __restrict__ tab) {
-  __shared__ int runs[CHUNK_RUNS];
-  __shared__ int sh_scan[MASK_WAVES];
-  __shared__ int nruns_s;
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, w = tid >> 6;
-  const int nbf = info->nbf;
-  if (tid == 0) nruns_s = 0;
-  __syncthreads();
-  // run starts (first pixel of each detector row), compacted in order
-  for (int base = 0; base < nbf; base += MASK_THREADS) {
-    const int b = base + tid;
-    const bool start = (b < nbf) && (b == 0 || bf_rows[b] != bf_rows[b - 1]);
-    const unsigned long long bal = __ballot(start);
-    const int below = __popcll(bal & ((1ull << lane) - 1ull));
-    if (lane == 0) sh_scan[w] = __popcll(bal);
-    __syncthreads();
-    int off = nruns_s;
-    for (int k = 0; k < w; ++k) off += sh_scan[k];
-    if (start && off + below < CHUNK_RUNS) runs[off + below] = b;
-    __syncthreads();
-    if (tid == 0) {
-      int tot = 0;
-      for (int k = 0; k < MASK_WAVES; ++k) tot += sh_scan[k];
-      nruns_s += tot;
-    }
-    __syncthreads();
-  }
-  if (tid != 0) return;
-  const int nr = nruns_s;
-  int k = 0;
-  if (nr > CHUNK_RUNS) {
-    // too many rows to pack: fixed chunks of ch pixels
-    for (int b = 0; b < nbf; b += ch) tab[1 + k++] = b;
-  } else {
-    int start = 0, len = 0;
-    for (int r = 0; r < nr; ++r) {
-      const int s0 = runs[r];
-      const int e0 = (r + 1 < nr) ? runs[r + 1] : nbf;
-      int L = e0 - s0;
-      // a row that fits in a chunk is not split: close the current chunk
-      // first; a longer row is split anyway, so it tops up the current one
-      if (len > 0 && len + L > ch && L <= ch) {
-        tab[1 + k++] = start;
-        len = 0;
-      }
-      if (len == 0) start = s0;
-      int p0 = s0;
-      while (len + L > ch) {          // a row longer than one chunk: full pieces
-        tab[1 + k++] = start;
-        p0 += ch - len;
-        L -= ch - len;
-        start = p0;
-        len = 0;
-      }
-      len += L;
-    }
-    if (len > 0) tab[1 + k++] = start;
-  }
-  tab[0] = k;
-  tab[1 + k] = nbf;
-}
-
-// ------------------------------------------------------------------------
 // BF gather + scan-row real FFT.
-// Work item = (scan row R_r, chunk of at most 2*PAIRS BF pixels from the
-// chunk table).  The chunk's
+// Work item = (scan row R_r, chunk of 2*PAIRS BF pixels).  The chunk's
 // pixels from the N frames of the scan row are read (u16 -> f32), paired as
 // re/im of PAIRS complex sequences along R_c, transformed with one complex
 // FFT each, and split into the two real-input half spectra.  Output H is
@@ -501,7 +426,7 @@ template <int N, int PAIRS>
 __global__ void __launch_bounds__(256)
 k_gather_rowfft(const uint16_t* __restrict__ data, int D, int det_cols,
                 const int* __restrict__ bf_rows, const int* __restrict__ bf_cols,
-                const int* __restrict__ tab, const float2* __restrict__ tw_g,
+                const BfInfo* __restrict__ info, const float2* __restrict__ tw_g,
                 float2* __restrict__ H) {
   constexpr int STRIDE = N + 1;
   constexpr int CH = 2 * PAIRS;
@@ -510,22 +435,22 @@ k_gather_rowfft(const uint16_t* __restrict__ data, int D, int det_cols,
   __shared__ int offs[CH];
   const int tid = threadIdx.x;
   for (int i = tid; i < N; i += 256) tw[i] = tw_g[i];
-  const int nchunks = tab[0];
+  const int nbf = info->nbf;
+  const int nchunks = (nbf + CH - 1) / CH;
   const long long items = (long long)nchunks * N;
   for (long long item = blockIdx.x; item < items; item += gridDim.x) {
     // XCD-aware order: workgroups go round-robin over the 8 XCDs, so give
     // XCD x the items (row, chunk) of rows x, x+8, ... with the chunks of one
-    // scan row consecutive -- they read the same frames, so a line shared by
-    // two chunks (a detector row split across chunks) hits in that XCD's L2.
+    // scan row consecutive -- they read the same frames, and the 128-byte
+    // lines that straddle chunk boundaries then hit in that XCD's L2.
     const long long xcd = item & 7, t = item >> 3;
     const int rr = (int)((t / nchunks) * 8 + xcd);
     const int chunk = (int)(t % nchunks);
-    const int b0 = tab[1 + chunk];
-    const int cnt = tab[2 + chunk] - b0;   // <= CH
+    const int b0 = chunk * CH;
     __syncthreads();
     if (tid < CH) {
       const int b = b0 + tid;
-      offs[tid] = (tid < cnt) ? bf_rows[b] * det_cols + bf_cols[b] : -1;
+      offs[tid] = (b < nbf) ? bf_rows[b] * det_cols + bf_cols[b] : -1;
     }
     __syncthreads();
     // gather: lane -> BF pixel within the chunk, lane group -> frame; all
@@ -554,8 +479,8 @@ k_gather_rowfft(const uint16_t* __restrict__ data, int D, int det_cols,
     for (int it = tid; it < CH * HALF; it += 256) {
       const int jb = it / HALF;           // BF within chunk
       const int k = it - jb * HALF;       // slot
-      if (jb >= cnt) continue;
       const int b = b0 + jb;
+      if (b >= nbf) continue;
       const int p = jb >> 1;
       const float2 zk = z[p * STRIDE + k];
       float2 out;
@@ -580,12 +505,13 @@ k_gather_rowfft(const uint16_t* __restrict__ data, int D, int det_cols,
 // ------------------------------------------------------------------------
 // Launch wrappers
 // ------------------------------------------------------------------------
-// Frame ranges of the stream pass: about 1536 workgroups in all (6 per CU,
-// measured best for a 2 GiB stream on MI355X), at most max_ranges, and at
-// most 65536 frames per range so the uint32 partial sums cannot overflow.
-static void range_split(long long frames, int D, int max_ranges, int& ranges, int& fpr) {
+// Frame ranges of the stream pass: about target_wg workgroups in all (the
+// default 1536 = 6 per CU was measured best for a 2 GiB stream on MI355X;
+// the tuner may choose another), at most max_ranges, and at most 65536
+// frames per range so the uint32 partial sums cannot overflow.
+static void range_split(long long frames, int D, int max_ranges, int target_wg, int& ranges, int& fpr) {
   const long long tiles = (D % 8 == 0) ? (D / 8 + 255) / 256 : (D + 255) / 256;
-  long long r = 1536 / (tiles > 0 ? tiles : 1);
+  long long r = (target_wg > 0 ? target_wg : 1536) / (tiles > 0 ? tiles : 1);
   const long long min_r = (frames + 65535) / 65536;
   if (r < min_r) r = min_r;
   if (r > max_ranges) r = max_ranges;
@@ -596,9 +522,9 @@ static void range_split(long long frames, int D, int max_ranges, int& ranges, in
 }
 
 hipError_t launch_mean_dp_stream(const uint16_t* data, long long frames, int D, uint32_t* partial, int max_ranges,
-                                 hipStream_t s) {
+                                 int target_wg, hipStream_t s) {
   int ranges, fpr;
-  range_split(frames, D, max_ranges, ranges, fpr);
+  range_split(frames, D, max_ranges, target_wg, ranges, fpr);
   const bool vec = (D % 8 == 0) && ((reinterpret_cast<uintptr_t>(data) & 15u) == 0);
   if (vec) {
     dim3 grid((D / 8 + 255) / 256, ranges);
@@ -610,19 +536,19 @@ hipError_t launch_mean_dp_stream(const uint16_t* data, long long frames, int D, 
   return hipGetLastError();
 }
 
-hipError_t launch_mean_dp_reduce(long long frames, int D, const uint32_t* partial, int max_ranges,
+hipError_t launch_mean_dp_reduce(long long frames, int D, const uint32_t* partial, int max_ranges, int target_wg,
                                  unsigned long long* sums, float* mean, hipStream_t s) {
   int ranges, fpr;
-  range_split(frames, D, max_ranges, ranges, fpr);
+  range_split(frames, D, max_ranges, target_wg, ranges, fpr);
   hipLaunchKernelGGL(k_dp_reduce, dim3((D + 63) / 64), dim3(256), 0, s, partial, ranges, D, frames, sums, mean);
   return hipGetLastError();
 }
 
 hipError_t launch_mean_dp(const uint16_t* data, long long frames, int D, uint32_t* partial, int max_ranges,
-                          unsigned long long* sums, float* mean, hipStream_t s) {
-  hipError_t e = launch_mean_dp_stream(data, frames, D, partial, max_ranges, s);
+                          int target_wg, unsigned long long* sums, float* mean, hipStream_t s) {
+  hipError_t e = launch_mean_dp_stream(data, frames, D, partial, max_ranges, target_wg, s);
   if (e != hipSuccess) return e;
-  return launch_mean_dp_reduce(frames, D, partial, max_ranges, sums, mean, s);
+  return launch_mean_dp_reduce(frames, D, partial, max_ranges, target_wg, sums, mean, s);
 }
 
 // ------------------------------------------------------------------------
@@ -784,37 +710,44 @@ hipError_t launch_bf_geometry(const int* rows, const int* cols, const BfInfo* in
   return hipGetLastError();
 }
 
-hipError_t launch_gather_rowfft(int N, const uint16_t* data, int D, int det_cols, const int* rows,
-                                const int* cols, const BfInfo* info, int nbf_hint, const float2* tw,
-                                int* chunk_tab, float2* H, hipStream_t s) {
-  const int ch = (N <= 256) ? 32 : 8192 / N;   // 2 * PAIRS of the instantiation below
-  hipLaunchKernelGGL(k_gather_chunks, dim3(1), dim3(MASK_THREADS), 0, s, rows, info, ch, chunk_tab);
-  // grid: an estimate of the chunk count (row packing adds about half a
-  // chunk per chunk); the kernel strides over the true count
-  long long items = (long long)(((nbf_hint + ch - 1) / ch) * 3 / 2 + 1) * N;
-  if (items > 16384) items = 16384;
-  const dim3 grid((unsigned)items);
+int gather_pairs_default(int N) { return N == 128 ? 16 : (N == 1024 ? 4 : 8); }
+
+int gather_pairs_candidates(int N, int* out) {
   switch (N) {
-    case 128:
-      hipLaunchKernelGGL((k_gather_rowfft<128, 16>), grid, dim3(256), 0, s, data, D, det_cols, rows, cols,
-                         chunk_tab, tw, H);
-      break;
-    case 256:
-      hipLaunchKernelGGL((k_gather_rowfft<256, 16>), grid, dim3(256), 0, s, data, D, det_cols, rows, cols,
-                         chunk_tab, tw, H);
-      break;
-    case 512:
-      hipLaunchKernelGGL((k_gather_rowfft<512, 8>), grid, dim3(256), 0, s, data, D, det_cols, rows, cols,
-                         chunk_tab, tw, H);
-      break;
-    case 1024:
-      hipLaunchKernelGGL((k_gather_rowfft<1024, 4>), grid, dim3(256), 0, s, data, D, det_cols, rows, cols,
-                         chunk_tab, tw, H);
-      break;
-    default:
-      return hipErrorInvalidValue;
+    case 128: out[0] = 16; out[1] = 8; return 2;
+    case 256: out[0] = 8; out[1] = 16; return 2;
+    case 512: out[0] = 8; out[1] = 4; return 2;
+    case 1024: out[0] = 4; out[1] = 2; return 2;
+    default: return 0;
   }
-  return hipGetLastError();
+}
+
+hipError_t launch_gather_rowfft(int N, const uint16_t* data, int D, int det_cols, const int* rows,
+                                const int* cols, const BfInfo* info, int nbf_hint, const float2* tw, int pairs,
+                                float2* H, hipStream_t s) {
+  if (pairs <= 0) pairs = gather_pairs_default(N);
+  auto grid_for = [&](int ch) {
+    long long items = (long long)((nbf_hint + ch - 1) / ch) * N;
+    if (items < 1) items = 1;
+    if (items > 16384) items = 16384;
+    return dim3((unsigned)items);
+  };
+#define QSSB_GATHER(NN, PP)                                                                             \
+  if (N == NN && pairs == PP) {                                                                         \
+    hipLaunchKernelGGL((k_gather_rowfft<NN, PP>), grid_for(2 * PP), dim3(256), 0, s, data, D, det_cols, \
+                       rows, cols, info, tw, H);                                                        \
+    return hipGetLastError();                                                                           \
+  }
+  QSSB_GATHER(128, 16)
+  QSSB_GATHER(128, 8)
+  QSSB_GATHER(256, 8)
+  QSSB_GATHER(256, 16)
+  QSSB_GATHER(512, 8)
+  QSSB_GATHER(512, 4)
+  QSSB_GATHER(1024, 4)
+  QSSB_GATHER(1024, 2)
+#undef QSSB_GATHER
+  return hipErrorInvalidValue;
 }
 
 // ------------------------------------------------------------------------

@@ -55,6 +55,9 @@ SIGNATURES: dict[str, list] = {
     "ssb_recon_u16_after": [_vp, _vp, _vp, _c_double, _c_double, _c_double, _c_double, _c_float, _c_float, _vp,
                             _vp, _vp],
     "ssb_stage_count": [],
+    "ssb_tune_candidates": [_c_int, _c_int, ctypes.POINTER(_c_int), _c_int],
+    "ssb_tune_set": [_vp, _c_int, _c_int],
+    "ssb_tune_get": [_vp, _c_int, ctypes.POINTER(_c_int)],
     "ssb_timing_begin": [_vp, _c_int, _c_int],
     "ssb_timing_end": [_vp, _fp, _c_int, ctypes.POINTER(_c_int)],
     "ssb_phase_loss": [_vp, _c_double, _c_double, _c_double, _vp, _vp, _vp],

@@ -608,7 +608,7 @@ class HipSSBBackend:
         elif refine is not None:
             raise ValueError(f"refine must be 'nelder-mead', 'quadratic' or None; got {refine!r}")
 
-    def fit_learned(self, net, feature_config, *, trials: int = 64, refine: str | None = "nelder-mead",
+    def fit_learned(self, net, feature_config, *, trials: int = 16, refine: str | None = "nelder-mead",
                     xatol: float = 1e-4, rounds: int = 4, half_widths=None, bounds=None, seed: int = 42,
                     verbose: bool = False):
         """Learned start, a ``trials``-candidate batched Sobol search in a box

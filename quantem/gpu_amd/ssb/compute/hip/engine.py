@@ -434,6 +434,15 @@ class HipSSBEngine:
         self._device_bf = True
 
     # ------------------------------------------------------------ timing
+    def tune_set(self, knob: int, value: int) -> None:
+        """Set a launch-shape knob (``tuner.KNOBS``); 0 restores the default."""
+        _abi.call("ssb_tune_set", self.plan.handle, int(knob), int(value))
+
+    def tune_get(self, knob: int) -> int:
+        out = ctypes.c_int(0)
+        _abi.call("ssb_tune_get", self.plan.handle, int(knob), ctypes.byref(out))
+        return int(out.value)
+
     def timing_begin(self, max_calls: int, boundaries=None) -> None:
         """Record per-stage hipEvents for the next ``max_calls`` recon calls;
         ``boundaries`` (stage indices, the stage count meaning the end) limits

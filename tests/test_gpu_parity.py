@@ -242,10 +242,10 @@ def test_synthetic_recon_matches_fp64_oracle(cuda, scan, det, ab, rot):
 
 
 @pytest.mark.parametrize("scan,radius", [(128, 24.0), (512, 10.0)])
-def test_gather_chunks_split_and_pack_rows(cuda, scan, radius):
-    """An explicit BF radius wider than the disk gives detector rows of up to
-    49 pixels (split across gather chunks of 32) and short rows that pack
-    together; scan 512 gathers 16 pixels per chunk.  Phase vs FP64."""
+def test_wide_bf_rows_and_small_gather_chunks(cuda, scan, radius):
+    """An explicit BF radius wider than the disk: detector rows of up to 49
+    BF pixels, split across gather chunks of 32 (scan 128) or 16 (scan
+    512).  Phase vs FP64."""
     cfg, data = synthetic(scan, 32 if scan == 512 else 128, {"C10": 150.0})
     p = O.prepare(data, voltage_kV=300, semiangle_mrad=20.0, scan_sampling=cfg.step_A, det_sampling=1.25,
                   bf_radius=radius)
@@ -257,6 +257,28 @@ def test_gather_chunks_split_and_pack_rows(cuda, scan, radius):
     assert eng.num_bf == p.bf.rows.size
     rms, p999, _ = C.phase_errors(phase.cpu().numpy(), ref)
     assert rms < PHASE_RMS and p999 < PHASE_P999
+
+
+@pytest.mark.parametrize("scan", [512, 1024])
+def test_large_scan_end_to_end(cuda, scan, golden):
+    """Scans 512 and 1024 through the fused column FFT + trotter (and the
+    staged path) on a Poisson fixture of the reference's shape: phase vs FP64."""
+    fx = golden["fixtures"]["poisson"]
+    data = C.poisson_data(321, scan=(scan, scan))
+    c10, c12, phi12 = C.MAKE_ENGINE_ABERRATIONS
+    p = O.prepare(data, bf_radius=fx["bf_radius"], **C.POISSON_KW)
+    ref = O.object_wave(p.G, p.dc, p.geom, c10, c12, phi12)
+    eng = engine_for_data(cuda, data, semiangle_mrad=21.4, det_sampling=1.0, step=0.5)
+    d = cuda.from_numpy(data).cuda()
+    phase = cuda.empty((scan, scan), dtype=cuda.float32, device="cuda")
+    obj = cuda.empty((scan, scan), dtype=cuda.complex64, device="cuda")
+    eng.recon_u16(d, c10, c12, phi12, 0.0, 0.0, fx["bf_radius"], obj_out=obj, phase_out=phase)
+    rms, p999, _ = C.phase_errors(phase.cpu().numpy(), np.angle(ref))
+    assert rms < PHASE_RMS and p999 < PHASE_P999
+    assert scaled_error(obj.cpu().numpy(), ref) < OBJ_SCALED
+    eng.prepare_g(d)
+    staged = eng.reconstruct_object(c10, c12, phi12).cpu().numpy()
+    assert scaled_error(staged, ref) < OBJ_SCALED
 
 
 def test_full_size_benchmark_config(cuda):
