@@ -163,6 +163,48 @@ def fit_leg(dev, n_sets: int, train_steps: int, search_trials: int, learned_tria
     }
 
 
+def decode_leg(data, reps: int = 5, distinct: int = 64) -> dict:
+    """GPU decode of bitshuffle-LZ4 chunks (one frame each) into the resident
+    uint16 array: ``distinct`` frames of the benchmark input compressed on
+    the host by the library's encoder, tiled to every frame of the scan
+    (65,536 chunks, 2 GiB decoded), decoded ``reps`` times; checks the
+    decoded frames bit for bit."""
+    import numpy as np
+    import torch
+    from quantem.gpu_amd.ssb.compute.hip.bslz4 import Bslz4Chunks, encode_u16
+
+    frames = data.reshape(-1, data.shape[-2], data.shape[-1])
+    n = frames.shape[0]
+    uniq = frames[:distinct].cpu().numpy()
+    chunks = [np.frombuffer(encode_u16(f), dtype=np.uint8) for f in uniq]
+    period = np.concatenate(chunks)
+    sizes = np.tile(np.array([c.size for c in chunks], dtype=np.int64), n // distinct)
+    buf = np.tile(period, n // distinct)
+    dec = Bslz4Chunks((buf, sizes), (1,) + tuple(frames.shape[1:]))
+    out = torch.empty_like(frames)
+    dec.decode(out)                                  # warm, and validates every chunk
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        dec.decode(out, sync=False)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    dec.check()
+    ms = e0.elapsed_time(e1) / reps
+    ok = bool(torch.equal(out.view(n // distinct, distinct, *frames.shape[1:]),
+                          torch.from_numpy(uniq).to(out.device).unsqueeze(0).expand(n // distinct, -1, -1, -1)))
+    out_bytes = 2.0 * out.numel()
+    alg = out_bytes + dec.compressed_bytes
+    return {"workload": f"bitshuffle-LZ4 decode, {n} chunks of one {frames.shape[1]}x{frames.shape[2]} u16 frame",
+            "ms": ms, "decoded_GBps": out_bytes / (ms * 1e-3) / 1e9,
+            "compression_ratio": out_bytes / dec.compressed_bytes,
+            "roofline": {"bound": "hbm", "achieved": alg / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "algorithmic_bytes": alg},
+            "bit_exact": ok}
+
+
 def copy_probe() -> int:
     """Five launches of an elementwise copy reading and writing 1 GiB each: the
     reference kernel of known traffic for the FETCH_SIZE/WRITE_SIZE counters."""
@@ -191,6 +233,8 @@ def main() -> int:
     ap.add_argument("--fit-trials", type=int, default=16, help="trials of the local search around the learned start")
     ap.add_argument("--fit-refine", default="nelder-mead", choices=["nelder-mead", "quadratic"])
     ap.add_argument("--fit-rounds", type=int, default=4, help="rounds of the quadratic refine")
+    ap.add_argument("--no-decode", action="store_true",
+                    help="skip the bitshuffle-LZ4 decode measurement")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch shapes instead of autotuning them before timing")
     ap.add_argument("--copy-probe", action="store_true",
@@ -312,6 +356,9 @@ def main() -> int:
     else:
         result["cpu_baseline"] = None
     eng.plan.close()
+    if not args.no_decode:
+        log("bitshuffle-LZ4 decode of the same frames")
+        result["decode"] = decode_leg(data)
     del data
     if args.fit_sets > 0:
         log("aberration fit: learned start + batched refine vs plain batched search")

@@ -261,6 +261,37 @@ int ssb_recon_u16_after(ssb_plan* plan, const ssb_plan* after, const uint16_t* d
                         float* phase_out, void* stream);
 
 /*
+ * Decode HDF5 bitshuffle-LZ4 chunks (one frame or several per chunk) into a
+ * uint16 array on the device: chunk c is comp[chunk_offsets[c] ..
+ * + chunk_sizes[c]) (device arrays of int64; comp_bytes bytes in all) and
+ * decodes to chunk_elems elements at out + c * chunk_elems.  block_bytes is
+ * the filter's block size (8192 for the default uint16 blocks; a multiple of
+ * 16 up to 8192); every chunk header must agree with it and with
+ * chunk_elems.  workspace (device, 16-byte aligned, at least
+ * ssb_bslz4_workspace_bytes(...) bytes) holds the status word and the
+ * block table.  With status_host non-NULL the call waits and reports a
+ * malformed chunk as QSSB_ERR_ARG; with NULL it stays asynchronous and
+ * the status word (int at workspace[0], 0 = ok) can be read later.
+ * Replaces the CUDA bslz4 decoder (qg/io/backends/cuda/decoder.py:258-1146;
+ * format as qg/io/backends/webgpu/bslz4.ts:1-30).
+ */
+long long ssb_bslz4_workspace_bytes(int n_chunks, long long chunk_elems, int block_bytes);
+
+/*
+ * Host-side writer of one bitshuffle-LZ4 chunk of n_elems uint16 values
+ * (host memory) in the layout above; returns the chunk's byte count, or a
+ * negative status.  out_capacity must be at least
+ * ssb_bslz4_encode_bound(n_elems, block_bytes).  The reference compresses
+ * its BF companion the same way on the GPU (qg/io/backends/cuda/decoder.py:466-1000).
+ */
+long long ssb_bslz4_encode_bound(long long n_elems, int block_bytes);
+long long ssb_bslz4_encode_u16(const uint16_t* values_host, long long n_elems, int block_bytes, uint8_t* out_host,
+                               long long out_capacity);
+int ssb_bslz4_decode_u16(const uint8_t* comp, long long comp_bytes, const long long* chunk_offsets,
+                         const long long* chunk_sizes, int n_chunks, long long chunk_elems, int block_bytes,
+                         uint16_t* out, void* workspace, long long workspace_bytes, int* status_host, void* stream);
+
+/*
  * Tuning knobs of the reconstruction path (no reference counterpart: the
  * reference fixes its launch shapes per scan size, cuda/kernels/fft256.py:1051-1058).
  * QSSB_TUNE_STREAM_WG: workgroups of the raw-data stream pass;

@@ -995,3 +995,51 @@ int ssb_tune_get(ssb_plan* p, int knob, int* value_host) {
   }
   return QSSB_OK;
 }
+
+// ------------------------------------------------------------------------
+// bitshuffle-LZ4 chunk decode
+// ------------------------------------------------------------------------
+long long ssb_bslz4_workspace_bytes(int n_chunks, long long chunk_elems, int block_bytes) {
+  if (n_chunks < 0 || chunk_elems < 1 || block_bytes < 16 || block_bytes > 8192 || block_bytes % 16)
+    return fail(QSSB_ERR_ARG, "block_bytes must be a multiple of 16 in [16, 8192] and chunks non-empty");
+  return bslz4_workspace_bytes(n_chunks, chunk_elems, block_bytes / 2);
+}
+
+int ssb_bslz4_decode_u16(const uint8_t* comp, long long comp_bytes, const long long* chunk_offsets,
+                         const long long* chunk_sizes, int n_chunks, long long chunk_elems, int block_bytes,
+                         uint16_t* out, void* workspace, long long workspace_bytes, int* status_host, void* stream) {
+  const long long need = ssb_bslz4_workspace_bytes(n_chunks, chunk_elems, block_bytes);
+  if (need < 0) return (int)need;
+  if (n_chunks == 0) return QSSB_OK;
+  if (!comp || !chunk_offsets || !chunk_sizes || !out || !workspace)
+    return fail(QSSB_ERR_ARG, "bslz4 buffers must not be NULL");
+  if (workspace_bytes < need) return fail(QSSB_ERR_ARG, "bslz4 workspace too small");
+  if ((reinterpret_cast<uintptr_t>(out) & 15u) || (reinterpret_cast<uintptr_t>(workspace) & 15u))
+    return fail(QSSB_ERR_ARG, "bslz4 output and workspace must be 16-byte aligned");
+  hipStream_t s = as_stream(stream);
+  HIP_TRY(launch_bslz4_decode_u16(comp, comp_bytes, chunk_offsets, chunk_sizes, n_chunks, chunk_elems,
+                                  block_bytes / 2, out, workspace, s),
+          "bslz4_decode");
+  if (status_host) {
+    HIP_TRY(hipMemcpyAsync(status_host, workspace, sizeof(int), hipMemcpyDeviceToHost, s), "bslz4 status");
+    HIP_TRY(hipStreamSynchronize(s), "bslz4 status");
+    if (*status_host)
+      return fail(QSSB_ERR_ARG, "malformed bitshuffle-LZ4 chunk (status " + std::to_string(*status_host) + ")");
+  }
+  return QSSB_OK;
+}
+
+long long ssb_bslz4_encode_bound(long long n_elems, int block_bytes) {
+  if (n_elems < 1 || block_bytes < 16 || block_bytes > 8192 || block_bytes % 16)
+    return fail(QSSB_ERR_ARG, "block_bytes must be a multiple of 16 in [16, 8192] and the chunk non-empty");
+  return bslz4_encode_bound(n_elems, block_bytes);
+}
+
+long long ssb_bslz4_encode_u16(const uint16_t* values_host, long long n_elems, int block_bytes, uint8_t* out_host,
+                               long long out_capacity) {
+  const long long bound = ssb_bslz4_encode_bound(n_elems, block_bytes);
+  if (bound < 0) return bound;
+  if (!values_host || !out_host) return fail(QSSB_ERR_ARG, "bslz4 encode buffers must not be NULL");
+  if (out_capacity < bound) return fail(QSSB_ERR_ARG, "bslz4 encode output too small (see ssb_bslz4_encode_bound)");
+  return bslz4_encode_u16(values_host, n_elems, block_bytes, out_host, out_capacity);
+}

@@ -1,0 +1,363 @@
+// GPU decode of HDF5 bitshuffle-LZ4 ("bslz4") detector chunks straight into
+// the resident uint16 frame array the SSB path reads.
+//
+// Reference behaviour (paths relative to bobleesj/quantem.gpu):
+//   CUDA decoder              src/quantem/gpu/io/backends/cuda/decoder.py:258-1146
+//   WebGPU decoder (format)   src/quantem/gpu/io/backends/webgpu/bslz4.ts:1-30, 114-167
+//   CPU reader (hdf5plugin)   src/quantem/gpu/io/backends/cpu/reference.py:45-113
+//
+// Chunk layout: big-endian uint64 uncompressed bytes, big-endian uint32
+// block bytes, then per block a big-endian uint32 compressed size and an
+// LZ4 block; full blocks of block_bytes/2 elements, one last block of the
+// remaining elements rounded down to a multiple of 8, and the final n mod 8
+// elements raw.  A block decodes to 16 bit-planes of m/8 bytes: bit b of
+// element p is bit (p mod 8) of byte b*m/8 + p/8.
+//
+// Two kernels:
+//   k_bslz4_index   one thread per chunk: validates the header and walks the
+//                   block sizes into a descriptor table (bounds-checked);
+//   k_bslz4_decode  one 64-lane workgroup per block: stages the compressed
+//                   block in LDS, decodes it in LDS (every lane parses the
+//                   same sequence headers; literal and match bytes are
+//                   copied by all lanes, a match of offset d < length
+//                   copied in one pass as out[o + t] = out[o - d + t mod d]),
+//                   then inverts the bit-shuffle, 8 elements per lane step,
+//                   with 16-byte stores.
+// Malformed input never reads or writes out of bounds: it sets a bit in the
+// status word and the block is skipped.
+#include <cstring>
+#include <vector>
+
+#include "ssb_internal.h"
+
+namespace qssb {
+
+namespace {
+
+constexpr int kMaxBlockBytes = 8192;                                  // 4096 uint16 elements
+constexpr int kMaxCompBytes = kMaxBlockBytes + kMaxBlockBytes / 255 + 16;  // LZ4 worst case
+
+struct BlockDesc {
+  long long src;   // byte offset of the block's payload in the compressed buffer
+  long long dst;   // first output element
+  int csize;       // compressed bytes; -1 = raw little-endian tail
+  int elems;       // elements (0 = unused slot)
+};
+
+enum : int {
+  kErrHeader = 1,      // size or block size differs from what the caller declared
+  kErrBounds = 2,      // a block or the tail runs past the chunk
+  kErrStream = 4,      // a malformed LZ4 sequence
+  kErrBlockSize = 8,   // a compressed block larger than the LZ4 bound
+};
+
+__device__ __forceinline__ unsigned be32(const uint8_t* p) {
+  return ((unsigned)p[0] << 24) | ((unsigned)p[1] << 16) | ((unsigned)p[2] << 8) | (unsigned)p[3];
+}
+
+__global__ void __launch_bounds__(256)
+k_bslz4_index(const uint8_t* __restrict__ comp, long long comp_bytes, const long long* __restrict__ chunk_off,
+              const long long* __restrict__ chunk_len, int n_chunks, long long chunk_elems, int block_elems,
+              int slots, BlockDesc* __restrict__ table, int* __restrict__ status) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= n_chunks) return;
+  BlockDesc* t = table + (long long)c * slots;
+  for (int k = 0; k < slots; ++k) t[k].elems = 0;
+  const long long base = chunk_off[c];
+  const long long len = chunk_len[c];
+  if (base < 0 || len < 12 || base > comp_bytes - len) {
+    atomicOr(status, kErrBounds);
+    return;
+  }
+  const uint8_t* p = comp + base;
+  const unsigned long long total = ((unsigned long long)be32(p) << 32) | be32(p + 4);
+  const unsigned bbytes = be32(p + 8);
+  if (total != (unsigned long long)chunk_elems * 2ull || bbytes != (unsigned)block_elems * 2u) {
+    atomicOr(status, kErrHeader);
+    return;
+  }
+  const long long full = chunk_elems / block_elems;
+  const long long rest = chunk_elems - full * block_elems;
+  const int last = (int)(rest - rest % 8);
+  const int tail = (int)(rest % 8);
+  long long pos = 12;
+  int k = 0;
+  const long long nblk = full + (last ? 1 : 0);
+  for (long long i = 0; i < nblk; ++i) {
+    if (pos + 4 > len) {
+      atomicOr(status, kErrBounds);
+      return;
+    }
+    const unsigned cs = be32(p + pos);
+    pos += 4;
+    if (cs > (unsigned)kMaxCompBytes) {
+      atomicOr(status, kErrBlockSize);
+      return;
+    }
+    if (pos + (long long)cs > len) {
+      atomicOr(status, kErrBounds);
+      return;
+    }
+    t[k].src = base + pos;
+    t[k].dst = (long long)c * chunk_elems + i * block_elems;
+    t[k].csize = (int)cs;
+    t[k].elems = (i < full) ? block_elems : last;
+    ++k;
+    pos += cs;
+  }
+  if (pos + 2LL * tail != len) {
+    atomicOr(status, kErrBounds);
+    return;
+  }
+  if (tail) {
+    t[k].src = base + pos;
+    t[k].dst = (long long)c * chunk_elems + full * block_elems + last;
+    t[k].csize = -1;
+    t[k].elems = tail;
+  }
+}
+
+__global__ void __launch_bounds__(64)
+k_bslz4_decode(const uint8_t* __restrict__ comp, const BlockDesc* __restrict__ table, long long n_desc,
+               uint16_t* __restrict__ out, int* __restrict__ status) {
+  __shared__ uint8_t cin[kMaxCompBytes];
+  __shared__ __attribute__((aligned(16))) uint8_t cout[kMaxBlockBytes];
+  const int lane = threadIdx.x;
+  for (long long d = blockIdx.x; d < n_desc; d += gridDim.x) {
+    const BlockDesc bd = table[d];
+    if (bd.elems == 0) continue;                       // uniform
+    if (bd.csize < 0) {                                // raw tail, little-endian
+      if (lane < bd.elems) {
+        const uint8_t* s = comp + bd.src + 2 * lane;
+        out[bd.dst + lane] = (uint16_t)(s[0] | (s[1] << 8));
+      }
+      continue;
+    }
+    const int cs = bd.csize;
+    const int osz = bd.elems * 2;
+    __syncthreads();                                   // previous block's readers are done
+    for (int t = lane; t < cs; t += 64) cin[t] = comp[bd.src + t];
+    __syncthreads();
+    // LZ4 block: every lane walks the same sequence headers
+    int ip = 0, op = 0, err = 0;
+    while (true) {
+      if (ip >= cs) { err = kErrStream; break; }
+      const int token = cin[ip++];
+      int lit = token >> 4;
+      if (lit == 15) {
+        int b;
+        do {
+          if (ip >= cs) { err = kErrStream; break; }
+          b = cin[ip++];
+          lit += b;
+        } while (b == 255);
+        if (err) break;
+      }
+      if (ip + lit > cs || op + lit > osz) { err = kErrStream; break; }
+      for (int t = lane; t < lit; t += 64) cout[op + t] = cin[ip + t];
+      ip += lit;
+      op += lit;
+      if (ip == cs) break;                             // the last sequence has no match
+      if (ip + 2 > cs) { err = kErrStream; break; }
+      const int off = cin[ip] | (cin[ip + 1] << 8);
+      ip += 2;
+      int ml = token & 15;
+      if (ml == 15) {
+        int b;
+        do {
+          if (ip >= cs) { err = kErrStream; break; }
+          b = cin[ip++];
+          ml += b;
+        } while (b == 255);
+        if (err) break;
+      }
+      ml += 4;
+      if (off == 0 || off > op || op + ml > osz) { err = kErrStream; break; }
+      __syncthreads();                                 // literals written before the match reads them
+      for (int t = lane; t < ml; t += 64) cout[op + t] = cout[op - off + (t % off)];
+      op += ml;
+      __syncthreads();
+    }
+    if (!err && op != osz) err = kErrStream;
+    if (err) {
+      if (lane == 0) atomicOr(status, err);
+      continue;                                        // uniform: every lane saw the same stream
+    }
+    __syncthreads();
+    // inverse bit-shuffle: 8 elements per step from 16 plane bytes
+    const int planeb = bd.elems / 8;
+    for (int g = lane; g < planeb; g += 64) {
+      unsigned v[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+#pragma unroll
+      for (int b = 0; b < 16; ++b) {
+        const unsigned byte = cout[b * planeb + g];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] |= ((byte >> i) & 1u) << b;
+      }
+      if ((bd.dst & 7) == 0) {                       // uniform: 16-byte aligned groups
+        uint4 w;
+        w.x = v[0] | (v[1] << 16);
+        w.y = v[2] | (v[3] << 16);
+        w.z = v[4] | (v[5] << 16);
+        w.w = v[6] | (v[7] << 16);
+        *reinterpret_cast<uint4*>(out + bd.dst + 8LL * g) = w;
+      } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) out[bd.dst + 8LL * g + i] = (uint16_t)v[i];
+      }
+    }
+  }
+}
+
+__global__ void k_bslz4_clear(int* __restrict__ status) {
+  if (threadIdx.x == 0) *status = 0;
+}
+
+}  // namespace
+
+long long bslz4_slots(long long chunk_elems, int block_elems) {
+  if (block_elems <= 0) return 0;
+  const long long full = chunk_elems / block_elems;
+  const long long rest = chunk_elems - full * block_elems;
+  return full + ((rest - rest % 8) ? 1 : 0) + ((rest % 8) ? 1 : 0);
+}
+
+long long bslz4_workspace_bytes(int n_chunks, long long chunk_elems, int block_elems) {
+  return 256 + (long long)n_chunks * bslz4_slots(chunk_elems, block_elems) * (long long)sizeof(BlockDesc);
+}
+
+hipError_t launch_bslz4_decode_u16(const uint8_t* comp, long long comp_bytes, const long long* chunk_off,
+                                   const long long* chunk_len, int n_chunks, long long chunk_elems, int block_elems,
+                                   uint16_t* out, void* workspace, hipStream_t s) {
+  int* status = reinterpret_cast<int*>(workspace);
+  BlockDesc* table = reinterpret_cast<BlockDesc*>(reinterpret_cast<char*>(workspace) + 256);
+  const long long slots = bslz4_slots(chunk_elems, block_elems);
+  hipLaunchKernelGGL(k_bslz4_clear, dim3(1), dim3(64), 0, s, status);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_bslz4_index, dim3((unsigned)((n_chunks + 255) / 256)), dim3(256), 0, s, comp, comp_bytes,
+                     chunk_off, chunk_len, n_chunks, chunk_elems, block_elems, (int)slots, table, status);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const long long n_desc = (long long)n_chunks * slots;
+  const unsigned grid = (unsigned)(n_desc < 65536 ? n_desc : 65536);
+  hipLaunchKernelGGL(k_bslz4_decode, dim3(grid > 0 ? grid : 1), dim3(64), 0, s, comp, table, n_desc, out, status);
+  return hipGetLastError();
+}
+
+}  // namespace qssb
+
+// ------------------------------------------------------------------------
+// Host-side encoder (writers of bslz4 companions and test inputs): the
+// chunk layout above, bit-shuffled blocks, greedy LZ4 block compression
+// (4-byte hash, one candidate per slot; the last 5 bytes are literals and
+// no match starts in the last 12, as the block format requires).
+// ------------------------------------------------------------------------
+namespace qssb {
+
+namespace {
+
+void put_be32(uint8_t* p, unsigned v) {
+  p[0] = (uint8_t)(v >> 24);
+  p[1] = (uint8_t)(v >> 16);
+  p[2] = (uint8_t)(v >> 8);
+  p[3] = (uint8_t)v;
+}
+
+void bitshuffle_u16(const uint16_t* v, int m, uint8_t* out) {
+  const int pb = m / 8;
+  for (int i = 0; i < 16 * pb; ++i) out[i] = 0;
+  for (int p = 0; p < m; ++p)
+    for (int b = 0; b < 16; ++b)
+      if ((v[p] >> b) & 1u) out[b * pb + p / 8] |= (uint8_t)(1u << (p % 8));
+}
+
+// LZ4 block of src[0, n) into dst; returns its size.
+long long lz4_compress_block(const uint8_t* src, int n, uint8_t* dst) {
+  constexpr int kHashBits = 12;
+  int table[1 << kHashBits];
+  for (int& t : table) t = -1;
+  auto hash = [&](int i) {
+    unsigned x;
+    std::memcpy(&x, src + i, 4);
+    return (x * 2654435761u) >> (32 - kHashBits);
+  };
+  long long o = 0;
+  auto put_len = [&](int x) {
+    while (x >= 255) {
+      dst[o++] = 255;
+      x -= 255;
+    }
+    dst[o++] = (uint8_t)x;
+  };
+  int anchor = 0, i = 0;
+  const int limit = n - 12;
+  while (i <= limit) {
+    const unsigned h = hash(i);
+    const int cand = table[h];
+    table[h] = i;
+    if (cand < 0 || i - cand > 65535 || std::memcmp(src + cand, src + i, 4) != 0) {
+      ++i;
+      continue;
+    }
+    int len = 4;
+    while (i + len < n - 5 && src[cand + len] == src[i + len]) ++len;
+    const int lit = i - anchor;
+    const int ml = len - 4;
+    dst[o++] = (uint8_t)(((lit < 15 ? lit : 15) << 4) | (ml < 15 ? ml : 15));
+    if (lit >= 15) put_len(lit - 15);
+    std::memcpy(dst + o, src + anchor, (size_t)lit);
+    o += lit;
+    const int off = i - cand;
+    dst[o++] = (uint8_t)off;
+    dst[o++] = (uint8_t)(off >> 8);
+    if (ml >= 15) put_len(ml - 15);
+    i += len;
+    anchor = i;
+  }
+  const int lit = n - anchor;
+  dst[o++] = (uint8_t)((lit < 15 ? lit : 15) << 4);
+  if (lit >= 15) put_len(lit - 15);
+  std::memcpy(dst + o, src + anchor, (size_t)lit);
+  o += lit;
+  return o;
+}
+
+}  // namespace
+
+long long bslz4_encode_bound(long long n_elems, int block_bytes) {
+  const int be = block_bytes / 2;
+  const long long blocks = n_elems / be + 1;
+  return 12 + blocks * (4 + (long long)block_bytes + block_bytes / 255 + 16) + 16;
+}
+
+long long bslz4_encode_u16(const uint16_t* v, long long n, int block_bytes, uint8_t* dst, long long cap) {
+  if (cap < bslz4_encode_bound(n, block_bytes)) return -1;
+  const int be = block_bytes / 2;
+  const unsigned long long total = (unsigned long long)n * 2ull;
+  put_be32(dst, (unsigned)(total >> 32));
+  put_be32(dst + 4, (unsigned)total);
+  put_be32(dst + 8, (unsigned)block_bytes);
+  long long o = 12;
+  const long long full = n / be;
+  const long long rest = n - full * be;
+  const int last = (int)(rest - rest % 8);
+  const int tail = (int)(rest % 8);
+  std::vector<uint8_t> shuf((size_t)block_bytes);
+  long long pos = 0;
+  for (long long k = 0; k < full + (last ? 1 : 0); ++k) {
+    const int m = (k < full) ? be : last;
+    bitshuffle_u16(v + pos, m, shuf.data());
+    const long long cs = lz4_compress_block(shuf.data(), 2 * m, dst + o + 4);
+    put_be32(dst + o, (unsigned)cs);
+    o += 4 + cs;
+    pos += m;
+  }
+  for (int t = 0; t < tail; ++t) {
+    dst[o++] = (uint8_t)v[pos + t];
+    dst[o++] = (uint8_t)(v[pos + t] >> 8);
+  }
+  return o;
+}
+
+}  // namespace qssb

@@ -160,4 +160,13 @@ hipError_t launch_phase_moments(int N, const float2* Gt, int gcols, const BfInfo
 hipError_t launch_features_from_g(const float2* Gt, int N, const int* bf_rows, const int* bf_cols,
                                   const BfInfo* info, const int* qrow, const int* qcol, float* out, hipStream_t s);
 
+// bitshuffle-LZ4 chunk decode (ssb_bslz4.hip)
+long long bslz4_slots(long long chunk_elems, int block_elems);
+long long bslz4_workspace_bytes(int n_chunks, long long chunk_elems, int block_elems);
+long long bslz4_encode_bound(long long n_elems, int block_bytes);
+long long bslz4_encode_u16(const uint16_t* v, long long n, int block_bytes, uint8_t* dst, long long cap);
+hipError_t launch_bslz4_decode_u16(const uint8_t* comp, long long comp_bytes, const long long* chunk_off,
+                                   const long long* chunk_len, int n_chunks, long long chunk_elems, int block_elems,
+                                   uint16_t* out, void* workspace, hipStream_t s);
+
 }  // namespace qssb

@@ -55,6 +55,11 @@ SIGNATURES: dict[str, list] = {
     "ssb_recon_u16_after": [_vp, _vp, _vp, _c_double, _c_double, _c_double, _c_double, _c_float, _c_float, _vp,
                             _vp, _vp],
     "ssb_stage_count": [],
+    "ssb_bslz4_workspace_bytes": [_c_int, ctypes.c_longlong, _c_int],
+    "ssb_bslz4_encode_bound": [ctypes.c_longlong, _c_int],
+    "ssb_bslz4_encode_u16": [_vp, ctypes.c_longlong, _c_int, _vp, ctypes.c_longlong],
+    "ssb_bslz4_decode_u16": [_vp, ctypes.c_longlong, _vp, _vp, _c_int, ctypes.c_longlong, _c_int, _vp, _vp,
+                             ctypes.c_longlong, ctypes.POINTER(_c_int), _vp],
     "ssb_tune_candidates": [_c_int, _c_int, ctypes.POINTER(_c_int), _c_int],
     "ssb_tune_set": [_vp, _c_int, _c_int],
     "ssb_tune_get": [_vp, _c_int, ctypes.POINTER(_c_int)],
@@ -101,6 +106,9 @@ def lib() -> ctypes.CDLL:
     handle.ssb_last_error.restype = ctypes.c_char_p
     handle.ssb_stage_name.argtypes = [_c_int]
     handle.ssb_stage_name.restype = ctypes.c_char_p
+    for name in ("ssb_bslz4_workspace_bytes", "ssb_bslz4_encode_bound", "ssb_bslz4_encode_u16"):
+        if hasattr(handle, name):
+            getattr(handle, name).restype = ctypes.c_longlong
     _lib = handle
     return handle
 

@@ -23,6 +23,42 @@ from . import optimizer as _opt
 from .engine import HipSSBEngine, PlanSpec, SSBPlan, normalize_scan, scan_target
 
 
+def _bf_subset_indices(full_num_bf: int, ratio: float) -> np.ndarray:
+    """Uniform-stride BF subset for ``bf_subsample`` (``cuda/backend.py:21-43``):
+    every round(1/ratio)-th pixel."""
+    if ratio <= 0.0:
+        raise ValueError(f"bf_subsample must be in (0, 1], got {ratio}")
+    stride = max(1, int(round(1.0 / float(ratio))))
+    return np.arange(0, full_num_bf, stride, dtype=np.int64)
+
+
+class _PreviewSubset:
+    """Prepared reduced-BF preview state; ``with`` routes previews to it."""
+
+    def __init__(self, ssb: "HipSSBBackend", num_bf: int) -> None:
+        full = ssb.num_bf
+        count = max(1, min(int(num_bf), full))
+        step = max(1, full // count)
+        self._ssb = ssb
+        self._engine = ssb._engine.subset(np.arange(0, full, step, dtype=np.int64)[:count])
+        self._active = False
+
+    @property
+    def num_bf(self) -> int:
+        return self._engine.num_bf
+
+    def __enter__(self) -> "_PreviewSubset":
+        if self._active or self._ssb._subset_engine is not None:
+            raise RuntimeError("The prepared SSB BF subset is already active.")
+        self._ssb._subset_engine = self._engine
+        self._active = True
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self._ssb._subset_engine = None
+        self._active = False
+
+
 def _check_objective_mode(mode) -> None:
     """Objective modes of ``cuda/backend.py:1021-1026``.  The reference's
     "native" mode is a sparse-row objective with its own landscape
@@ -85,6 +121,7 @@ class HipSSBBackend:
             data = data.reshape(scan_shape[0], scan_shape[1], data.shape[1], data.shape[2])
         elif data.ndim != 4:
             raise ValueError("data must be 3D or 4D.")
+        self._source_shape = tuple(int(v) for v in data.shape)
         data = normalize_scan(data.contiguous())
 
         if isinstance(scan_sampling, (int, float)):
@@ -147,6 +184,7 @@ class HipSSBBackend:
         self._n_trials: int | None = None
         self._optuna_trials: list[dict] = []
         self._optimizer_objective_mode = "exact"
+        self._subset_engine: HipSSBEngine | None = None
         self._bf_source_path: pathlib.Path | None = None
         self._bf_source_dtype: np.dtype | None = None
         self._bf_source_max_value: int | None = None
@@ -188,8 +226,21 @@ class HipSSBBackend:
             raise ValueError("this SSB session is closed")
         if rotation_angle_rad is None:
             rotation_angle_rad = self._rotation_angle_rad
-        self._engine.cache_rotation(float(rotation_angle_rad))
-        return self._engine
+        eng = self._subset_engine if self._subset_engine is not None else self._engine
+        eng.cache_rotation(float(rotation_angle_rad))
+        return eng
+
+    @contextlib.contextmanager
+    def _bf_subset(self, indices):
+        """Route every evaluation inside the block to a BF subset."""
+        if self._subset_engine is not None:
+            raise RuntimeError("A BF subset is already active.")
+        self._subset_engine = self._engine.subset(indices)
+        try:
+            yield self._subset_engine
+        finally:
+            sub, self._subset_engine = self._subset_engine, None
+            sub.plan.close()
 
     # ---------------------------------------------------------- properties
     @property
@@ -324,9 +375,12 @@ class HipSSBBackend:
             phase, loss = self.reconstruct(aberrations["C10"], aberrations["C12"], aberrations["phi12"]), None
         return self.phase_to_numpy(phase), None if loss is None else float(loss)
 
-    def preview_context(self, num_bf: int):
-        """Reduced-BF previews are not built yet: previews run on the full disk."""
-        return contextlib.nullcontext()
+    def preview_context(self, num_bf: int) -> "_PreviewSubset":
+        """A reusable reduced-BF preview context (``cuda/backend.py:683-686``;
+        ``_PreparedCudaBfSubset``, ``cuda/engine.py:306-380``): inside
+        ``with``, previews use ``num_bf`` BF pixels taken at a uniform stride
+        (every ``full // num_bf``-th, the first ``num_bf`` of them)."""
+        return _PreviewSubset(self, num_bf)
 
     def browser_state(self) -> SSBExportState:
         """Compact host state (``cuda/engine.py:529-575``)."""
@@ -434,8 +488,10 @@ class HipSSBBackend:
         """
         t0 = time.perf_counter()
         _check_objective_mode(objective_mode)
-        if bf_subsample is not None and bf_subsample != 1.0:
-            raise NotImplementedError("bf_subsample is not supported by the hip SSB backend yet.")
+        if bf_subsample is not None and bf_subsample < 1.0:
+            with self._bf_subset(_bf_subset_indices(self.num_bf, bf_subsample)):
+                return self.optimize(aberrations, rotation_angle_deg, n_trials, seed, verbose, None,
+                                     objective_mode)
         if aberrations is None:
             aberrations = dict(self._DEFAULT_OPTIMIZE_RANGES)
         spec = dict(aberrations)
@@ -479,8 +535,9 @@ class HipSSBBackend:
         if objective_mode is None:
             objective_mode = self._optimizer_objective_mode
         _check_objective_mode(objective_mode)
-        if bf_subsample is not None and bf_subsample != 1.0:
-            raise NotImplementedError("bf_subsample is not supported by the hip SSB backend yet.")
+        if bf_subsample is not None and bf_subsample < 1.0:
+            with self._bf_subset(_bf_subset_indices(self.num_bf, bf_subsample)):
+                return self.refine(verbose, xatol, fatol, lock, None, objective_mode)
         self._optimizer_objective_mode = objective_mode
         lock = set(lock or [])
         keys = ["C10", "C12", "phi12"]
@@ -692,14 +749,64 @@ class HipSSBBackend:
         return self
 
     def fit(self, *, trials: int, refinement: str | None, search_ranges, refine_lock, seed: int,
-            verbose: bool) -> SSBResult:
-        """The shared fit contract (``cuda/backend.py:853-879``)."""
-        if trials:
-            self.optimize(aberrations=search_ranges, n_trials=int(trials), seed=int(seed), verbose=verbose,
-                          objective_mode="exact")
-        if refinement == "nelder-mead":
-            self.refine(verbose=verbose, lock=refine_lock, objective_mode="exact")
-        return self.result()
+            verbose: bool, save_to=None, force: bool = False, source_path=None) -> SSBResult:
+        """The shared fit contract (``cuda/backend.py:853-879``).
+
+        ``save_to`` (with ``source_path``, the detector file the data came
+        from) reuses an exact prior result when nothing that determines it
+        changed, as ``SSB.fit`` does (``qg/ssb/workflow.py:1043-1103``);
+        ``force`` recomputes."""
+        def run():
+            if trials:
+                self.optimize(aberrations=search_ranges, n_trials=int(trials), seed=int(seed), verbose=verbose,
+                              objective_mode="exact")
+            if refinement == "nelder-mead":
+                self.refine(verbose=verbose, lock=refine_lock, objective_mode="exact")
+            return self.result()
+
+        if save_to is None:
+            return run()
+        settings = {"trials": int(trials), "refinement": refinement, "search_ranges": search_ranges,
+                    "refine_lock": refine_lock, "seed": int(seed),
+                    "starting_aberrations": dict(self.aberrations),
+                    "starting_rotation_deg": math.degrees(self._rotation_angle_rad)}
+        return self._saved("fit", settings, run, save_to, force, source_path, verbose)
+
+    def reconstruct_saved(self, aberrations: dict[str, float], *, save_to, source_path, compute_loss: bool = True,
+                          force: bool = False, verbose: bool = False) -> SSBResult:
+        """``reconstruct_result`` with the saved-result reuse of ``SSB.reconstruct``
+        (``qg/ssb/workflow.py:1113-1170``)."""
+        coefs = {**self.aberrations, **aberrations}
+        settings = {"aberrations": coefs, "compute_loss": bool(compute_loss)}
+        return self._saved("reconstruct", settings,
+                           lambda: self.reconstruct_result(aberrations, compute_loss=compute_loss),
+                           save_to, force, source_path, verbose)
+
+    def _saved(self, operation, settings, run, save_to, force, source_path, verbose) -> SSBResult:
+        from ...persistence import SCHEMA, json_value, path_signature, reuse_or_run, software_signature
+        if source_path is None:
+            raise ValueError("save_to requires source_path when SSB.from_array() is used. Pass the detector "
+                             "source path so saved results cannot be reused for a different array accidentally.")
+        if not pathlib.Path(source_path).expanduser().exists():
+            raise FileNotFoundError(f"Cannot save an SSB result because source_path does not exist: "
+                                    f"{pathlib.Path(source_path).expanduser()}")
+        signature = json_value({
+            "schema": SCHEMA, "software": software_signature(), "operation": operation,
+            "source": {"detector": path_signature(source_path)},
+            "data": {"dtype": "uint16", "shape": self._source_shape, "scan_shape": self.scan_shape},
+            "instrument": {"energy_eV": self.energy, "semiangle_mrad": self.semiangle_mrad,
+                           "scan_sampling_A": self.scan_sampling, "det_sampling": self.angular_sampling},
+            "ssb": {"backend": self.backend, "rotation_angle_deg": math.degrees(self._rotation_angle_rad),
+                    "bf_intensity_threshold": self.bf_intensity_threshold, "bf_radius": self._bf_radius_arg},
+            "settings": settings,
+        })
+        result = reuse_or_run(operation, save_to, signature, run, force=force, device=self._plan.device,
+                              verbose=verbose, input_metadata={"source_path": str(source_path)})
+        if result.reused:
+            self.aberrations.update({k: float(v) for k, v in result.aberrations.items()})
+            self._rotation_angle_rad = math.radians(result.rotation_angle_deg)
+            self._prepare()
+        return result
 
     def _print_summary(self, stage: str, elapsed: float) -> None:
         a = self.aberrations

@@ -248,6 +248,24 @@ class HipSSBEngine:
             self.bf_state()
         return int(self._bf.rows.size)
 
+    def subset(self, indices) -> "HipSSBEngine":
+        """A new engine on BF pixels ``indices`` of this one's resident stack
+        (same geometry and dc; its own plan), for reduced-BF previews and
+        searches (``_PreparedCudaBfSubset``, ``use_bf_subset``:
+        ``cuda/engine.py:306-380``, ``720-800``)."""
+        from dataclasses import replace
+        idx = np.asarray(indices, dtype=np.int64).reshape(-1)
+        bf = self.bf_state() if self._bf is None else self._bf
+        if idx.size == 0 or idx.min() < 0 or idx.max() >= bf.rows.size:
+            raise ValueError("BF subset indices must be non-empty and inside the BF list")
+        if not self._has_g:
+            raise ValueError("no resident Fourier stack")
+        g = self.G_qk[self.torch.as_tensor(idx, device=self.plan.device)]
+        sub = HipSSBEngine(SSBPlan(replace(self.plan.spec, max_bf=int(idx.size)), device=self.plan.device))
+        sub.set_bf(bf.rows[idx], bf.cols[idx], bf.center, bf.dc)
+        sub.import_g(g)
+        return sub
+
     def set_detector_sampling(self, recip_row: float, recip_col: float) -> None:
         _abi.call("ssb_set_detector_sampling", self.plan.handle, float(recip_row), float(recip_col))
         self._rotation = None

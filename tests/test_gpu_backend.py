@@ -138,3 +138,67 @@ def test_grid_search_picks_the_grid_minimum(cuda):
     losses = [ssb.reconstruct_with_loss(c, 0.0, 0.0)[1] for c in grid]
     assert ssb.aberrations["C10"] == pytest.approx(grid[int(np.argmin(losses))])
     assert abs(ssb.aberrations["C10"] - 200.0) <= 10.0
+
+
+def test_fit_save_to_reuses_an_exact_prior_result(cuda, tmp_path):
+    """``fit(save_to=...)``: the second identical call loads the saved result
+    (same object wave, marked reused); changed settings or ``force`` rerun."""
+    data = C.poisson_data(124)
+    src = tmp_path / "scan.raw"
+    src.write_bytes(data.tobytes())
+    ssb = backend(cuda.from_numpy(data).cuda(), aberrations=dict(ABER))
+    kw = dict(trials=8, refinement=None, search_ranges={"C10_nm": (-200.0, 200.0), "C12_nm": 0.0,
+                                                         "phi12_deg": 0.0},
+              refine_lock=None, seed=42, verbose=False, save_to=tmp_path / "out", source_path=src)
+    with pytest.raises(ValueError, match="source_path"):
+        ssb.fit(**dict(kw, source_path=None))
+    first = ssb.fit(**kw)
+    assert not first.reused and (tmp_path / "out" / "ssb-fit.npz").is_file()
+    ssb.aberrations.update(ABER)
+    again = ssb.fit(**kw)
+    assert again.reused
+    np.testing.assert_array_equal(again.object_wave.cpu().numpy(), first.object_wave.cpu().numpy())
+    assert again.aberrations == first.aberrations and ssb.aberrations["C10"] == first.aberrations["C10"]
+    assert not ssb.fit(**dict(kw, seed=7)).reused
+    assert not ssb.fit(**dict(kw, seed=7, force=True)).reused
+
+
+def test_preview_context_uses_a_strided_bf_subset(cuda):
+    """``preview_context(k)``: previews inside the block equal the FP64 oracle
+    on every (full // k)-th BF pixel (the first k), with the full-disk dc;
+    outside it they use the full disk again."""
+    data = C.poisson_data(123)
+    ssb = backend(cuda.from_numpy(data).cuda(), aberrations=dict(ABER))
+    p = O.prepare(data, bf_radius=3, **C.POISSON_KW)
+    full = p.bf.rows.size
+    k = full // 3
+    idx = np.arange(0, full, max(1, full // k))[:k]
+    ctx = ssb.preview_context(k)
+    assert ctx.num_bf == k
+    with ctx:
+        phase, loss = ssb.preview(ABER, compute_loss=True, higher_order_magnitudes=None,
+                                  higher_order_angles=None)
+    geom = O.detector_geometry(p.bf.rows[idx], p.bf.cols[idx], p.bf.center, (16, 16), (1.0, 1.0),
+                               p.geom.wavelength, 21.4, 0.0, 128, 0.5)
+    ref_phase, ref_loss = O.phase_and_loss(p.G[idx], p.dc, geom, *C.MAKE_ENGINE_ABERRATIONS)
+    rms, p999, _ = C.phase_errors(phase, ref_phase)
+    assert rms < 1e-4 and p999 < 3e-4
+    assert loss == pytest.approx(ref_loss, rel=1e-4, abs=1e-6)
+    full_phase, _ = ssb.preview(ABER, compute_loss=False, higher_order_magnitudes=None, higher_order_angles=None)
+    ref_full, _ = O.phase_and_loss(p.G, p.dc, p.geom, *C.MAKE_ENGINE_ABERRATIONS)
+    assert C.phase_errors(full_phase, ref_full)[0] < 1e-4
+
+
+def test_bf_subsample_searches_on_the_subset(cuda):
+    data = C.poisson_data(125)
+    ssb = backend(cuda.from_numpy(data).cuda(), aberrations=dict(ABER))
+    ranges = {"C10_nm": (-150.0, 150.0), "C12_nm": 0.0, "phi12_deg": 0.0}
+    ssb.optimize(aberrations=ranges, n_trials=8, verbose=False, bf_subsample=0.5)
+    hist = ssb._optuna_trials
+    assert len(hist) == 8 and ssb._subset_engine is None
+    with ssb._bf_subset(np.arange(0, ssb.num_bf, 2)):
+        again = ssb._evaluate_params([dict(h["params"], C12_nm=0.0, phi12_deg=0.0, C30=0.0,
+                                           rotation_angle_deg=0.0) for h in hist[:3]])
+    np.testing.assert_allclose(again, [h["loss"] for h in hist[:3]], rtol=1e-6)
+    with pytest.raises(ValueError, match="bf_subsample"):
+        ssb.optimize(aberrations=ranges, n_trials=4, verbose=False, bf_subsample=-1.0)

@@ -1,0 +1,75 @@
+"""GPU bitshuffle-LZ4 decode against the CPU oracle's encoder: bit-exact
+frames over the chunk-layout edge cases, malformed chunks rejected, and a
+decoded dataset reconstructing exactly like the raw one."""
+from __future__ import annotations
+
+import struct
+
+import numpy as np
+import pytest
+
+from oracle import bslz4_oracle as B
+from tests import _cases as C
+from tests._gpu import engine_for_data
+
+pytestmark = pytest.mark.gpu
+
+
+def _frames(kind, n, shape, seed):
+    rng = np.random.default_rng(seed)
+    size = (n,) + shape
+    return {"poisson": lambda: rng.poisson(0.8, size), "random": lambda: rng.integers(0, 65536, size),
+            "zeros": lambda: np.zeros(size), "max": lambda: np.full(size, 65535)}[kind]().astype(np.uint16)
+
+
+@pytest.mark.parametrize("kind,n,shape,per_chunk", [
+    ("poisson", 6, (64, 64), 1),        # one full block per frame
+    ("poisson", 3, (96, 96), 1),        # full blocks + a last block
+    ("random", 5, (7, 7), 1),           # 48-element last block + 1 raw element; unaligned chunks
+    ("poisson", 4, (130, 40), 1),       # 4096 + 1104
+    ("zeros", 2, (128, 128), 1),        # long overlapping matches
+    ("max", 2, (33, 33), 1),            # 1089 = 1088 + 1 raw
+    ("poisson", 8, (32, 32), 4),        # chunks of 4 frames
+])
+def test_decode_is_bit_exact(cuda, kind, n, shape, per_chunk):
+    from quantem.gpu_amd.ssb.compute.hip.bslz4 import decode_u16
+    v = _frames(kind, n, shape, n + shape[0])
+    chunks = [B.encode_chunk(v[i:i + per_chunk]) for i in range(0, n, per_chunk)]
+    out = decode_u16(chunks, (per_chunk,) + shape)
+    np.testing.assert_array_equal(out.cpu().numpy(), v)
+
+
+def test_malformed_chunks_are_rejected(cuda):
+    from quantem.gpu_amd.ssb.compute.hip.bslz4 import decode_u16
+    v = _frames("poisson", 1, (64, 64), 0)
+    good = B.encode_chunk(v)
+    with pytest.raises(ValueError, match="malformed"):
+        decode_u16([good[:-3]], (1, 64, 64))                          # truncated block
+    with pytest.raises(ValueError, match="malformed"):
+        decode_u16([good], (1, 64, 32))                               # header disagrees
+    bad = bytearray(good)
+    cs = struct.unpack(">I", bad[12:16])[0]
+    bad[16 + cs - 7] ^= 0xFF                                         # corrupt the stream's tail
+    bad[16 + 1:16 + 3] = b"\xff\xff"                                 # an offset past the output
+    with pytest.raises(ValueError, match="malformed"):
+        decode_u16([bytes(bad)], (1, 64, 64))
+    out = decode_u16([good], (1, 64, 64))                             # the decoder recovers
+    np.testing.assert_array_equal(out.cpu().numpy(), v)
+
+
+def test_decoded_dataset_reconstructs_like_the_raw_one(cuda):
+    """The reference's Poisson fixture (128x128 scan, 16x16 detector), one
+    frame per chunk: decode on the GPU, then the R1 reconstruction."""
+    from quantem.gpu_amd.ssb.compute.hip.bslz4 import decode_u16
+    data = C.poisson_data(123)
+    frames = data.reshape(-1, 16, 16)
+    chunks = [B.encode_chunk(f) for f in frames]
+    dec = decode_u16(chunks, (1, 16, 16)).reshape(data.shape)
+    raw = cuda.from_numpy(data).cuda()
+    assert cuda.equal(dec, raw)
+    eng = engine_for_data(cuda, data, semiangle_mrad=21.4, det_sampling=1.0, step=0.5)
+    a = cuda.empty((128, 128), dtype=cuda.float32, device="cuda")
+    b = cuda.empty_like(a)
+    eng.recon_u16(raw, *C.MAKE_ENGINE_ABERRATIONS, 0.0, 0.0, 3.0, phase_out=a)
+    eng.recon_u16(dec, *C.MAKE_ENGINE_ABERRATIONS, 0.0, 0.0, 3.0, phase_out=b)
+    assert cuda.equal(a, b)
