@@ -245,8 +245,9 @@ def main() -> int:
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one MI355X: the SSB path runs on one GPU and this build has no "
-              "multi-GPU mode (see DESIGN.md for how the path would shard).", file=sys.stderr)
+        print("bench.py measures one MI355X: the SSB path runs on one GPU and this library has no "
+              "multi-GPU mode (the path would shard by BF pixels, one all-reduce of the Fourier-sum plane, or by "
+              "trials).", file=sys.stderr)
         return 2
 
     import torch

@@ -3,7 +3,7 @@
 TEST INFRASTRUCTURE ONLY: ``tests/`` use it to check the GPU kernels of
 ``quantem/gpu_amd/csrc/ssb_learn.hip``; nothing in the package imports it.
 The network has no counterpart in the reference (it only seeds the batched
-search), so this file restates the build's own design:
+search), so this file restates the library's own design:
 
 * ``forward``/``backward`` in FP64, with bf16 rounding (round to nearest
   even) applied at exactly the points where the GPU rounds: GEMM operands

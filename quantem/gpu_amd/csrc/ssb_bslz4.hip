@@ -174,7 +174,14 @@ k_bslz4_decode(const uint8_t* __restrict__ comp, const BlockDesc* __restrict__ t
       ml += 4;
       if (off == 0 || off > op || op + ml > osz) { err = kErrStream; break; }
       __syncthreads();                                 // literals written before the match reads them
-      for (int t = lane; t < ml; t += 64) cout[op + t] = cout[op - off + (t % off)];
+      if (off >= ml) {                                 // uniform: no overlap, a plain copy
+        for (int t = lane; t < ml; t += 64) cout[op + t] = cout[op - off + t];
+      } else if (off == 1) {                           // a run of one byte
+        const uint8_t r = cout[op - 1];
+        for (int t = lane; t < ml; t += 64) cout[op + t] = r;
+      } else {
+        for (int t = lane; t < ml; t += 64) cout[op + t] = cout[op - off + (t % off)];
+      }
       op += ml;
       __syncthreads();
     }

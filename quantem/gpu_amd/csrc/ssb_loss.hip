@@ -258,13 +258,23 @@ k_rowifft_moments(const float2* __restrict__ T, const BfInfo* __restrict__ info,
       k2[e].c = c.y;
     }
   }
+  // the next BF pixel's rows are loaded into registers while this one's
+  // are transformed, so the loads' latency overlaps the FFT
+  float2 nxt[E];
+  auto load = [&](int b) {
+    const float2* tb = T + (long long)b * NN + (long long)x0 * N;
+#pragma unroll
+    for (int e = 0; e < E; ++e) nxt[e] = tb[tid + e * BLOCK];
+  };
+  if (b0 < b1) load(b0);
   for (int b = b0; b < b1; ++b) {
     __syncthreads();
-    const float2* tb = T + (long long)b * NN + (long long)x0 * N;
-    for (int idx = tid; idx < ROWS * N; idx += BLOCK) {
-      const int r = idx / N, c = idx - (idx / N) * N;
-      buf[r * STRIDE + c] = tb[idx];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int idx = tid + e * BLOCK;
+      buf[(idx / N) * STRIDE + (idx % N)] = nxt[e];
     }
+    if (b + 1 < b1) load(b + 1);
     __syncthreads();
     fft_lds<N, ROWS, BLOCK, STRIDE, true>(buf, tw);
 #pragma unroll

@@ -13,7 +13,7 @@
 
 namespace qssb {
 
-int object_groups(int N) { return N == 128 ? 64 : (N == 256 ? 32 : (N == 512 ? 8 : 4)); }
+int object_groups(int N) { return N == 128 ? 64 : (N == 256 ? 16 : (N == 512 ? 8 : 4)); }
 
 // Phasor tables of the factorised overlap e^{i theta} = e^{i qx a_b} e^{i qy b_b},
 // (a_b, b_b) = 2 M k_b: rowph[b][q] for all rows, colph[b][c] for columns

@@ -462,6 +462,10 @@ class HipSSBBackend:
     def _evaluate_params(self, params: list[dict]) -> np.ndarray:
         """Exact loss of search candidates (``C10_nm``, ``C12_nm``, ``phi12_deg``,
         ``rotation_angle_deg``, ``C30``), one synchronisation."""
+        return self._enqueue_params(params).cpu().numpy().astype(np.float64)
+
+    def _enqueue_params(self, params: list[dict]):
+        """The same losses as a device tensor, enqueued without synchronising."""
         mags, angs, _ = self._ho_arrays_from_aberrations()
         m = np.repeat(mags[None, :], len(params), axis=0)
         a = np.repeat(angs[None, :], len(params), axis=0)
@@ -472,8 +476,7 @@ class HipSSBBackend:
             a[i, 1] = np.float32(math.radians(p["phi12_deg"]))
             m[i, 4] = np.float32(p["C30"])
             rot[i] = math.radians(p["rotation_angle_deg"])
-        losses = self._prepare().batch_phase_loss(m, a, rot)
-        return losses.cpu().numpy().astype(np.float64)
+        return self._prepare().batch_phase_loss(m, a, rot)
 
     def optimize(self, aberrations=None, rotation_angle_deg=None, n_trials: int = 200, seed: int = 42,
                  verbose: bool = True, bf_subsample: float | None = None,

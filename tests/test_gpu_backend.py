@@ -202,3 +202,20 @@ def test_bf_subsample_searches_on_the_subset(cuda):
     np.testing.assert_allclose(again, [h["loss"] for h in hist[:3]], rtol=1e-6)
     with pytest.raises(ValueError, match="bf_subsample"):
         ssb.optimize(aberrations=ranges, n_trials=4, verbose=False, bf_subsample=-1.0)
+
+
+def test_search_over_uneven_datasets_matches_single_searches(cuda):
+    """Scans of 128 (native), 96 (padded to 128) and 192 (padded to 256)
+    searched together give each dataset the losses of its own search."""
+    from quantem.gpu_amd.ssb.compute.hip import multi
+    datasets = [C.poisson_data(123), C.poisson_data(124, scan=(96, 96)), C.poisson_data(125, scan=(192, 192))]
+    bks = [backend(cuda.from_numpy(d).cuda(), aberrations=dict(ABER)) for d in datasets]
+    assert [b.scan_shape[0] for b in bks] == [128, 128, 256]
+    ranges = {"C10_nm": (-150.0, 150.0), "C12_nm": 0.0, "phi12_deg": 0.0, "C30": 0.0,
+              "rotation_angle_deg": (-2.0, 2.0)}
+    res = multi.search_datasets(bks, ranges, n_trials=12, batch=5)
+    for b, r in zip(bks, res):
+        single = b._evaluate_params([dict(h["params"], C12_nm=0.0, phi12_deg=0.0, C30=0.0)
+                                     for h in r["history"]])
+        np.testing.assert_array_equal(single, [h["loss"] for h in r["history"]])
+        assert r["loss"] == min(h["loss"] for h in r["history"])
