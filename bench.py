@@ -83,7 +83,7 @@ def _rms_wrapped(a, b) -> float:
 
 
 def fit_leg(dev, n_sets: int, train_steps: int, search_trials: int, learned_trials: int, refine: str,
-            rounds: int, seed: int = 2027) -> dict:
+            rounds: int | None, seed: int = 2027) -> dict:
     """Aberration fit at scan 256x256, detector 128x128: learned start + a
     local batched search + refine, against the plain batched search (global
     Sobol) + the same refine.
@@ -158,6 +158,8 @@ def fit_leg(dev, n_sets: int, train_steps: int, search_trials: int, learned_tria
         "max_phase_err_rad": {"search": max(r["search"]["phase_err_rad"] for r in rows),
                               "learned": max(r["learned"]["phase_err_rad"] for r in rows)},
         "phase_err_bar_rad": 1e-4,
+        "sets_within_bar": {name: sum(r[name]["phase_err_rad"] <= 1e-4 for r in rows) for name in ("search", "learned")},
+        "n_sets": len(rows),
         "training_s": train_s, "training_steps": train_steps, "training_loss": [hist[0][1], hist[-1][1]],
         "sets": rows,
     }
@@ -226,13 +228,14 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true",
                     help="skip the CPU baseline (used for profiler runs)")
-    ap.add_argument("--fit-sets", type=int, default=3,
+    ap.add_argument("--fit-sets", type=int, default=5,
                     help="held-out datasets of the aberration-fit comparison (0 skips it)")
     ap.add_argument("--fit-train-steps", type=int, default=1500)
     ap.add_argument("--fit-search-trials", type=int, default=512)
     ap.add_argument("--fit-trials", type=int, default=16, help="trials of the local search around the learned start")
-    ap.add_argument("--fit-refine", default="nelder-mead", choices=["nelder-mead", "quadratic"])
-    ap.add_argument("--fit-rounds", type=int, default=4, help="rounds of the quadratic refine")
+    ap.add_argument("--fit-refine", default="newton", choices=["newton", "nelder-mead", "quadratic"])
+    ap.add_argument("--fit-rounds", type=int, default=None,
+                    help="batches of the newton refine (default 8) or rounds of the quadratic refine (default 4)")
     ap.add_argument("--no-decode", action="store_true",
                     help="skip the bitshuffle-LZ4 decode measurement")
     ap.add_argument("--no-tune", action="store_true",

@@ -647,6 +647,48 @@ class HipSSBBackend:
             self._print_summary(f"Refine (quadratic, {n_evals} evals)", self._elapsed_refine)
         return self
 
+    def refine_newton(self, keys=("C10_nm", "C30", "rotation_angle_deg"), step=(10.0, 1.0e4, 0.25),
+                      xtol=(0.05, 50.0, 2.0e-3), hmin=(1.0, 2500.0, 0.1), ftol: float = 1e-8, max_iter: int = 8,
+                      polish: bool = True, bounds=None, verbose: bool = False):
+        """Batched finite-difference Newton refinement on the exact objective
+        (``optimizer.newton_refine``): one batch of 1 + 2d + d(d-1)/2
+        candidates per iteration (10 for three keys), one synchronisation
+        each.  ``step`` is the first stencil's half-width per key, ``hmin``
+        its smallest (the exact loss moves in steps of ~3e-8 as BF pixels
+        cross the aperture edge under rotation, so rotation differences
+        below ~0.1 degree are noise), ``xtol`` the step size per key and
+        ``ftol`` the predicted decrease below which it stops; ``polish`` adds
+        one least-squares model minimiser at the end; ``bounds`` clips
+        candidates."""
+        t0 = time.perf_counter()
+        base = self._search_state()
+        keys = tuple(keys)
+        bounds = bounds or {}
+        lo = np.array([bounds.get(k, (-math.inf, math.inf))[0] for k in keys])
+        hi = np.array([bounds.get(k, (-math.inf, math.inf))[1] for k in keys])
+
+        def evaluate(points):
+            params = []
+            for x in np.asarray(points, dtype=np.float64).reshape(-1, len(keys)):
+                p = dict(base)
+                p.update({k: float(v) for k, v in zip(keys, x)})
+                params.append(p)
+            return self._evaluate_params(params)
+
+        x0 = np.array([base[k] for k in keys])
+        best_x, best_loss, n_evals = _opt.newton_refine(evaluate, x0, np.asarray(step, dtype=np.float64),
+                                                        max_iter=max_iter, xtol=np.asarray(xtol, dtype=np.float64),
+                                                        hmin=np.asarray(hmin, dtype=np.float64), lo=lo, hi=hi,
+                                                        ftol=ftol, polish=polish)
+        self._set_search_state({k: float(v) for k, v in zip(keys, best_x)})
+        self._best_loss = float(best_loss)
+        self._refine_method = "newton"
+        self._refine_nfev = int(n_evals)
+        self._elapsed_refine = time.perf_counter() - t0
+        if verbose:
+            self._print_summary(f"Refine (newton, {n_evals} evals)", self._elapsed_refine)
+        return self
+
     def learned_start(self, net, feature_config) -> dict:
         """(C10, C30, rotation) predicted by the learned initializer from the
         resident Fourier stack, in the search's parameter names."""
@@ -656,27 +698,37 @@ class HipSSBBackend:
         return {"C10_nm": float(c10), "C30": float(c30), "rotation_angle_deg": math.degrees(float(rot))}
 
     _LEARNED_HALF_WIDTHS = {"C10_nm": 75.0, "C30": 7.5e4, "rotation_angle_deg": 2.5}
+    # first Newton stencils: after the global search, and after the learned
+    # start's local search (a box of the half-widths above, 16 trials)
+    _NEWTON_STEP = (10.0, 1.0e4, 0.25)
+    _LEARNED_NEWTON_STEP = (5.0, 5.0e3, 0.15)
     _FIT_BOUNDS = {"C10_nm": (-500.0, 500.0), "C30": (-2.0e5, 2.0e5), "rotation_angle_deg": (-5.0, 5.0)}
 
-    def _local_refine(self, refine: str, keys, radius, rounds: int, batch: int, bounds, seed: int,
-                      xatol: float, verbose: bool) -> None:
+    def _local_refine(self, refine: str, keys, radius, rounds, batch: int, bounds, seed: int,
+                      xatol: float, verbose: bool, newton_step=None) -> None:
         if refine == "nelder-mead":
             self.refine_keys(keys, xatol=xatol, fatol=1e-10, max_iter=400, verbose=verbose)
         elif refine == "quadratic":
-            self.refine_quadratic(keys, radius=radius, rounds=rounds, batch=batch, bounds=bounds, seed=seed,
+            self.refine_quadratic(keys, radius=radius, rounds=rounds or 4, batch=batch, bounds=bounds, seed=seed,
                                   verbose=verbose)
+        elif refine == "newton":
+            self.refine_newton(keys, step=newton_step or self._NEWTON_STEP, max_iter=rounds or 8, bounds=bounds,
+                               verbose=verbose)
         elif refine is not None:
-            raise ValueError(f"refine must be 'nelder-mead', 'quadratic' or None; got {refine!r}")
+            raise ValueError(f"refine must be 'nelder-mead', 'quadratic', 'newton' or None; got {refine!r}")
 
-    def fit_learned(self, net, feature_config, *, trials: int = 16, refine: str | None = "nelder-mead",
-                    xatol: float = 1e-4, rounds: int = 4, half_widths=None, bounds=None, seed: int = 42,
-                    verbose: bool = False):
+    def fit_learned(self, net, feature_config, *, trials: int = 16, refine: str | None = "newton",
+                    xatol: float = 1e-4, rounds: int | None = None, half_widths=None, bounds=None, seed: int = 42,
+                    newton_step=None, verbose: bool = False):
         """Learned start, a ``trials``-candidate batched Sobol search in a box
         around it (clipped to ``bounds``, the global search box), then a local
-        refinement on the exact objective: Nelder-Mead over (C10, C30,
-        rotation) with the reference's coefficients (``refine_keys``), or the
-        batched trust-region refine.  The global batched search
-        (``fit_search``) is the fallback when no trained network is available."""
+        refinement of (C10, C30, rotation) on the exact objective:
+        ``"newton"`` (default: batched finite-difference Newton,
+        ``refine_newton``, at most ``rounds`` batches, default 8),
+        ``"nelder-mead"`` (the reference's coefficients, ``refine_keys``) or
+        ``"quadratic"`` (``refine_quadratic``, ``rounds`` default 4).  The
+        global batched search (``fit_search``) is the fallback when no trained
+        network is available."""
         t0 = time.perf_counter()
         start = self.learned_start(net, feature_config)
         half = dict(self._LEARNED_HALF_WIDTHS)
@@ -698,24 +750,26 @@ class HipSSBBackend:
             self._set_search_state({k: min(max(v, bounds.get(k, (-math.inf, math.inf))[0]),
                                            bounds.get(k, (-math.inf, math.inf))[1]) for k, v in start.items()})
         keys = tuple(start)
-        self._local_refine(refine, keys, [half[k] / 2 for k in keys], rounds, 16, bounds, seed, xatol, verbose)
+        self._local_refine(refine, keys, [half[k] / 2 for k in keys], rounds, 16, bounds, seed, xatol, verbose,
+                           newton_step or self._LEARNED_NEWTON_STEP)
         self._learned_start = start
         self._n_trials = trials
         self._elapsed_optimize = time.perf_counter() - t0
         return self
 
-    def fit_search(self, *, trials: int = 512, refine: str | None = "nelder-mead", xatol: float = 1e-4,
-                   rounds: int = 4, bounds=None, seed: int = 42, verbose: bool = False):
+    def fit_search(self, *, trials: int = 512, refine: str | None = "newton", xatol: float = 1e-4,
+                   rounds: int | None = None, bounds=None, seed: int = 42, newton_step=None, verbose: bool = False):
         """Plain batched fit without the network: a ``trials``-candidate global
         Sobol search over ``bounds`` (C10, C30, rotation), then the same local
-        refinement as ``fit_learned`` from its best point."""
+        refinement as ``fit_learned`` from its best point (a wider first Newton
+        stencil: the global search's best point is farther out)."""
         t0 = time.perf_counter()
         bounds = bounds or dict(self._FIT_BOUNDS)
         self.optimize(aberrations=dict(bounds, C12_nm=0.0, phi12_deg=0.0), n_trials=trials, seed=seed,
                       verbose=False)
         keys = tuple(bounds)
         self._local_refine(refine, keys, [0.05 * (hi - lo) for lo, hi in bounds.values()], rounds, 16, bounds,
-                           seed, xatol, verbose)
+                           seed, xatol, verbose, newton_step)
         self._elapsed_optimize = time.perf_counter() - t0
         return self
 

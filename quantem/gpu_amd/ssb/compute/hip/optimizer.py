@@ -12,11 +12,15 @@ done).  Nothing here computes a loss.
   earlier vertex, convergence when the simplex spread is below ``xatol`` and
   the loss spread below ``fatol``, the 4 initial vertices and the 3 shrunk
   vertices evaluated as batches.
-* ``quadratic_refine`` is the batched local refinement: each round
+* ``quadratic_refine`` is a batched local refinement: each round
   evaluates one batch (the centre, the axis points and a Sobol fill of a
   box), fits a full quadratic model by least squares and moves to its
   minimiser inside the box; it needs a few batches where Nelder-Mead needs
   100+ sequential evaluations.  It has no reference counterpart.
+* ``newton_refine`` is the batched finite-difference Newton refinement
+  (the fits' default): one 10-point stencil per iteration, a trust-region
+  step of the quadratic it interpolates, a stencil floor above the loss's
+  noise.  It has no reference counterpart.
 * ``batched_search`` replaces the reference's Optuna TPE loop
   (``cuda/optimizer.py:59-230``), whose trial sequence cannot be reproduced
   offline: a scrambled Sobol design over the search box (seeded), evaluated
@@ -262,5 +266,142 @@ def quadratic_refine(evaluate: Evaluate, x0, radius, rounds: int = 3, batch: int
                 scale = min(1.0, max(shrink, 2.0 * min(reach, 1.0)))
         centre = xs[int(np.argmin(fs))].copy()
         radius = radius * scale
+    best = int(np.argmin(fs))
+    return np.asarray(xs[best]), float(fs[best]), len(fs)
+
+
+def _stencil(d: int) -> np.ndarray:
+    """Unit offsets of the finite-difference stencil: the centre, +/- each
+    axis, and +i+j for every pair (1 + 2d + d(d-1)/2 points)."""
+    rows = [np.zeros(d)]
+    eye = np.eye(d)
+    rows += [eye[i] for i in range(d)] + [-eye[i] for i in range(d)]
+    rows += [eye[i] + eye[j] for i in range(d) for j in range(i + 1, d)]
+    return np.asarray(rows)
+
+
+def _stencil_model(f: np.ndarray, h: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """Gradient and Hessian at the centre from the stencil losses ``f``
+    (order of ``_stencil``) with axis steps ``h``: central first and second
+    differences on the axes, (f_ij - f_i - f_j + f_0) / (h_i h_j) across."""
+    d = h.size
+    f0, fp, fm = f[0], f[1:1 + d], f[1 + d:1 + 2 * d]
+    g = (fp - fm) / (2.0 * h)
+    H = np.diag((fp - 2.0 * f0 + fm) / (h * h))
+    k = 1 + 2 * d
+    for i in range(d):
+        for j in range(i + 1, d):
+            H[i, j] = H[j, i] = (f[k] - fp[i] - fp[j] + f0) / (h[i] * h[j])
+            k += 1
+    return g, H
+
+
+def newton_refine(evaluate: Evaluate, x0, step, max_iter: int = 8, xtol=None, hmin=None, lo=None, hi=None,
+                  trust: float = 4.0, ftol: float = 0.0, polish: bool = False) -> tuple[np.ndarray, float, int]:
+    """Batched finite-difference Newton refinement from ``x0``; returns
+    (best_x, best_loss, n_evals).
+
+    Each iteration evaluates one batch: the stencil of ``_stencil`` around
+    the centre with axis steps ``h = max(tau * step, hmin)`` (``tau`` starts
+    at 1; ``hmin``, default ``step / 20``, keeps the differences above the
+    loss's own noise).  The model it interpolates, in stencil units
+    ``u = (x - centre) / h`` (so the parameters' scales do not enter its
+    conditioning), gives the Newton step -- Hessian eigenvalues taken by
+    magnitude and floored at 1e-6 of the largest, so a saddle or a nearly
+    flat valley still gives a bounded descent step -- scaled down, keeping
+    its direction, to end inside the box ``|u_i| <= trust``.  The step's end
+    is the next centre; the next batch both checks it and models around it.
+    With rho = actual / predicted decrease: an uphill step (the centre worse
+    than the best point seen) restarts from the best point with ``tau / 4``;
+    rho < 1/4 halves ``tau``; rho > 3/4 on a step that reached the box edge
+    doubles it (at most 4); a step inside half the box shrinks ``tau``
+    towards ``2 |s| / step`` so the stencil follows the distance still to
+    go.  Stops when a step has every ``|s_i| < xtol_i`` (default
+    ``step / 100``), when its predicted decrease is below ``ftol`` (the
+    loss's noise: no later batch could confirm it), or after ``max_iter``
+    batches.  ``polish`` then fits a least-squares quadratic to every point
+    within two stencils of the best and evaluates its minimiser (one more
+    point).  The result is the best point evaluated.  ``lo``/``hi`` clip
+    every point.  It has no reference counterpart.
+    """
+    x0 = np.asarray(x0, dtype=np.float64).reshape(-1)
+    d = x0.size
+    step = np.broadcast_to(np.asarray(step, dtype=np.float64), x0.shape).copy()
+    if np.any(step <= 0):
+        raise ValueError("step must be positive")
+    xtol = step / 100.0 if xtol is None else np.broadcast_to(np.asarray(xtol, dtype=np.float64), x0.shape).copy()
+    hmin = step / 20.0 if hmin is None else np.broadcast_to(np.asarray(hmin, dtype=np.float64), x0.shape).copy()
+    lo = np.full(d, -np.inf) if lo is None else np.asarray(lo, dtype=np.float64)
+    hi = np.full(d, np.inf) if hi is None else np.asarray(hi, dtype=np.float64)
+    unit = _stencil(d)
+    xs: list[np.ndarray] = []
+    fs: list[float] = []
+    centre = np.clip(x0, lo, hi)
+    tau = 1.0
+    last_f = math.inf           # loss at the centre the last step started from
+    pred = 0.0                  # the model's predicted decrease of that step
+    edge = False                # that step reached the trust box
+    for _ in range(max_iter):
+        h = np.maximum(tau * step, hmin)
+        pts = np.clip(centre + unit * h, lo, hi)
+        f = np.asarray(evaluate(pts), dtype=np.float64).reshape(-1)
+        xs.extend(pts)
+        fs.extend(float(v) for v in f)
+        if f[0] > last_f:
+            # the last step went uphill: model again around the best point
+            centre = xs[int(np.argmin(fs))].copy()
+            last_f = min(fs)
+            tau /= 4.0
+            pred = 0.0
+            continue
+        if pred > 0.0:
+            rho = (last_f - float(f[0])) / pred
+            if rho < 0.25:
+                tau /= 2.0
+            elif rho > 0.75 and edge:
+                tau = min(2.0 * tau, 4.0)
+        last_f = float(f[0])
+        hs = (pts[1:1 + d] - pts[1 + d:1 + 2 * d]).diagonal() / 2.0     # steps after clipping
+        if np.any(hs <= 0):
+            break
+        g, H = _stencil_model(f, hs)
+        gu, Hu = g * hs, H * np.outer(hs, hs)                            # stencil units
+        w, V = np.linalg.eigh(Hu)
+        top = float(np.max(np.abs(w)))
+        if not np.isfinite(top) or top == 0.0 or not np.all(np.isfinite(gu)):
+            break
+        w = np.maximum(np.abs(w), 1e-6 * top)
+        gv = V.T @ gu
+        su = -(V @ (gv / w))
+        reach = float(np.max(np.abs(su))) / trust
+        edge = reach >= 1.0
+        if edge:
+            su = su / reach          # keep the direction, end on the box
+        s = su * hs
+        if np.all(np.abs(s) < xtol):
+            break
+        sv = V.T @ su
+        pred = -float(gv @ sv + 0.5 * np.sum(w * sv * sv))
+        if pred < ftol:
+            break
+        if not edge and reach < 0.5:
+            tau = min(tau, 2.0 * float(np.max(np.abs(s) / step)))
+        centre = np.clip(centre + s, lo, hi)
+    if polish and fs:
+        # least-squares quadratic through every point within 2 stencils of the
+        # best one: many points average the loss's noise that the last
+        # stencil's interpolation takes in full; its minimiser costs one more
+        # evaluation and is kept only if it is better
+        b0 = int(np.argmin(fs))
+        h = np.maximum(tau * step, hmin)
+        u = (np.asarray(xs) - xs[b0]) / h
+        near = np.max(np.abs(u), axis=1) <= 2.0
+        su = _quadratic_minimiser(u[near], np.asarray(fs)[near]) if np.count_nonzero(near) >= 2 * len(unit) else None
+        if su is not None and np.all(np.isfinite(su)):
+            su = su / max(1.0, float(np.max(np.abs(su))) / 2.0)
+            xm = np.clip(xs[b0] + su * h, lo, hi)
+            fm = float(np.asarray(evaluate(xm[None, :]), dtype=np.float64).reshape(-1)[0])
+            xs.append(xm)
+            fs.append(fm)
     best = int(np.argmin(fs))
     return np.asarray(xs[best]), float(fs[best]), len(fs)

@@ -99,3 +99,64 @@ def test_quadratic_refine_falls_back_to_the_best_point_and_respects_bounds():
     assert fx <= -0.25 and np.all(np.abs(x) <= 0.6)
     with pytest.raises(ValueError, match="batch"):
         opt.quadratic_refine(evaluate, np.zeros(3), radius=1.0, batch=8)
+
+
+def _quadratic(A, centre, floor=0.0):
+    A = np.asarray(A, dtype=np.float64)
+    c = np.asarray(centre, dtype=np.float64)
+
+    def evaluate(points):
+        P = np.atleast_2d(points) - c
+        return floor + 0.5 * np.einsum("ki,ij,kj->k", P, A, P)
+    return evaluate
+
+
+def test_stencil_model_recovers_gradient_and_hessian_of_a_quadratic():
+    A = np.array([[4.0, 1.9, 0.2], [1.9, 1.0, 0.1], [0.2, 0.1, 2.0]])
+    c = np.array([0.3, -0.7, 1.2])
+    x = np.array([1.0, 0.5, -0.25])
+    h = np.array([0.1, 0.2, 0.05])
+    unit = opt._stencil(3)
+    assert unit.shape == (10, 3)                      # centre, 6 axis points, 3 pairs
+    g, H = opt._stencil_model(_quadratic(A, c)(x + unit * h), h)
+    np.testing.assert_allclose(H, A, rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(g, A @ (x - c), rtol=1e-9, atol=1e-9)
+
+
+def test_newton_refine_lands_on_a_quadratic_minimum_in_few_batches():
+    A = np.array([[4.0, 1.9, 0.2], [1.9, 1.0, 0.1], [0.2, 0.1, 2.0]])
+    c = np.array([0.3, -0.7, 1.2])
+    calls = []
+
+    def evaluate(points):
+        calls.append(len(points))
+        return _quadratic(A, c, 3.0)(points)
+    x, fx, n = opt.newton_refine(evaluate, np.ones(3), [0.1, 0.2, 0.05], xtol=1e-8, hmin=1e-9)
+    np.testing.assert_allclose(x, c, atol=1e-9)
+    assert fx == pytest.approx(3.0, abs=1e-15)
+    assert all(k == 10 for k in calls) and n == sum(calls) and len(calls) <= 4
+
+
+def test_newton_refine_follows_a_badly_scaled_valley_through_noise():
+    # curvatures of the exact SSB loss in (C10 nm, C30, rotation deg) near an
+    # optimum: a C10/C30 valley with condition number ~1e2 in stencil units,
+    # plus 3e-8 of evaluation noise
+    H = np.array([[8.4e-6, 1.56e-9, 0.0], [1.56e-9, 3.55e-13, 0.0], [0.0, 0.0, 4.8e-4]])
+    c = np.array([283.757, 58581.2, -2.6422])
+    rng = np.random.default_rng(0)
+    base = _quadratic(H, c, 0.007944)
+
+    def evaluate(points):
+        return base(points) + 3e-8 * rng.standard_normal(len(np.atleast_2d(points)))
+    x, fx, n = opt.newton_refine(evaluate, [300.645, -20976.9, -2.6386], [10.0, 1e4, 0.25],
+                                 xtol=[0.05, 50.0, 2e-3], hmin=[1.0, 2500.0, 0.1], ftol=1e-8)
+    assert abs(x[0] - c[0]) < 0.5 and abs(x[1] - c[1]) < 3000.0 and abs(x[2] - c[2]) < 0.02
+    assert n <= 80
+
+
+def test_newton_refine_respects_bounds_and_validates_steps():
+    x, fx, _ = opt.newton_refine(_quadratic(np.eye(2), [2.0, -3.0]), np.zeros(2), 0.25, max_iter=10,
+                                 lo=np.array([-1.0, -1.0]), hi=np.array([1.0, 1.0]))
+    np.testing.assert_allclose(x, [1.0, -1.0], atol=1e-6)
+    with pytest.raises(ValueError):
+        opt.newton_refine(_quadratic(np.eye(2), [0.0, 0.0]), np.zeros(2), [0.1, 0.0])
