@@ -186,7 +186,7 @@ def test_newton_refine_reaches_the_converged_optimum(cuda):
     ssb._set_search_state(start)
     ssb.refine_newton()
     got, loss, nfev = ssb._search_state(), ssb._best_loss, ssb._refine_nfev
-    assert ssb._refine_method == "newton" and nfev % 10 == 0 and nfev <= 80
+    assert ssb._refine_method == "newton" and nfev <= 81          # 8 batches of 10 + the polish point
     e_start, e_got = err(phase(start), ref), err(phase(got), ref)
     assert e_start > 5e-4
     assert e_got < 1e-4, (e_got, got, opt_state)
