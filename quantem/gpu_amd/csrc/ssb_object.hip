@@ -431,43 +431,45 @@ k_trotter_full(const float2* __restrict__ Gt, const BfInfo* __restrict__ info, G
 // ------------------------------------------------------------------------
 // Finish: reduce partial planes, inverse FFT, angle.
 // ------------------------------------------------------------------------
-__global__ void __launch_bounds__(256)
-k_reduce_partials(const float2* __restrict__ S_part, const BfInfo* __restrict__ info, int NN, int groups,
-                  float2* __restrict__ S) {
-  const int idx = blockIdx.x * 256 + threadIdx.x;
-  if (idx >= NN) return;
-  const int nbf = info->nbf;
-  Kahan kr, ki;
-  kr.init();
-  ki.init();
-  for (int g = 0; g < groups; ++g) {
-    const float2 v = S_part[(long long)g * NN + idx];
-    kr.add(v.x);
-    ki.add(v.y);
-  }
-  float2 out;
-  if (idx == 0) {
-    out = make_float2(info->dc_re, info->dc_im);
-  } else {
-    const float inv = 1.0f / (float)(nbf > 0 ? nbf : 1);
-    out = make_float2((kr.s - kr.c) * inv, (ki.s - ki.c) * inv);
-  }
-  S[idx] = out;
-}
-
-// inverse FFT along q of column-major S[c][q] (in place), COUNT columns per block
+// Sum of the groups' partial planes (Kahan, in group order), /N_bf, DC :=
+// dc, then the inverse FFT along q of COUNT columns of column-major S[c][q]
 template <int N, int COUNT>
 __global__ void __launch_bounds__(256)
-k_ifft_cols(float2* __restrict__ S, const float2* __restrict__ tw_g) {
+k_reduce_ifft_cols(const float2* __restrict__ S_part, const BfInfo* __restrict__ info, int groups,
+                   const float2* __restrict__ tw_g, float2* __restrict__ S) {
   constexpr int STRIDE = N + 1;
+  constexpr long long NN = (long long)N * N;
   __shared__ float2 buf[COUNT * STRIDE];
   __shared__ float2 tw[N];
   const int tid = threadIdx.x;
   for (int i = tid; i < N; i += 256) tw[i] = tw_g[i];
   const int c0 = blockIdx.x * COUNT;
+  const int nbf = info->nbf;
+  const float inv = 1.0f / (float)(nbf > 0 ? nbf : 1);
   for (int idx = tid; idx < COUNT * N; idx += 256) {
     const int ci = idx / N, r = idx % N;
-    buf[ci * STRIDE + r] = S[(long long)(c0 + ci) * N + r];
+    const long long e = (long long)(c0 + ci) * N + r;
+    Kahan kr, ki;
+    kr.init();
+    ki.init();
+    int g = 0;
+    for (; g + 8 <= groups; g += 8) {      // eight loads in flight, then the adds in group order
+      float2 v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = S_part[(long long)(g + u) * NN + e];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        kr.add(v[u].x);
+        ki.add(v[u].y);
+      }
+    }
+    for (; g < groups; ++g) {
+      const float2 v = S_part[(long long)g * NN + e];
+      kr.add(v.x);
+      ki.add(v.y);
+    }
+    buf[ci * STRIDE + r] = (e == 0) ? make_float2(info->dc_re, info->dc_im)
+                                    : make_float2((kr.s - kr.c) * inv, (ki.s - ki.c) * inv);
   }
   __syncthreads();
   fft_lds<N, COUNT, 256, STRIDE, true>(buf, tw);
@@ -641,24 +643,22 @@ hipError_t launch_trotter_full(int N, const float2* Gt, const BfInfo* info, cons
 
 hipError_t launch_finish_object(int N, const float2* S_part, const BfInfo* info, const float2* tw, float2* S,
                                 float2* obj, float* phase, hipStream_t s) {
-  const int NN = N * N;
-  hipLaunchKernelGGL(k_reduce_partials, dim3((NN + 255) / 256), dim3(256), 0, s, S_part, info, NN,
-                     object_groups(N), S);
+  const int G = object_groups(N);
   switch (N) {
     case 128:
-      hipLaunchKernelGGL((k_ifft_cols<128, 8>), dim3(128 / 8), dim3(256), 0, s, S, tw);
+      hipLaunchKernelGGL((k_reduce_ifft_cols<128, 4>), dim3(128 / 4), dim3(256), 0, s, S_part, info, G, tw, S);
       hipLaunchKernelGGL((k_ifft_rows_phase<128, 8>), dim3(128 / 8), dim3(256), 0, s, S, tw, obj, phase);
       break;
     case 256:
-      hipLaunchKernelGGL((k_ifft_cols<256, 4>), dim3(256 / 4), dim3(256), 0, s, S, tw);
+      hipLaunchKernelGGL((k_reduce_ifft_cols<256, 2>), dim3(256 / 2), dim3(256), 0, s, S_part, info, G, tw, S);
       hipLaunchKernelGGL((k_ifft_rows_phase<256, 4>), dim3(256 / 4), dim3(256), 0, s, S, tw, obj, phase);
       break;
     case 512:
-      hipLaunchKernelGGL((k_ifft_cols<512, 2>), dim3(512 / 2), dim3(256), 0, s, S, tw);
+      hipLaunchKernelGGL((k_reduce_ifft_cols<512, 1>), dim3(512), dim3(256), 0, s, S_part, info, G, tw, S);
       hipLaunchKernelGGL((k_ifft_rows_phase<512, 2>), dim3(512 / 2), dim3(256), 0, s, S, tw, obj, phase);
       break;
     case 1024:
-      hipLaunchKernelGGL((k_ifft_cols<1024, 1>), dim3(1024), dim3(256), 0, s, S, tw);
+      hipLaunchKernelGGL((k_reduce_ifft_cols<1024, 1>), dim3(1024), dim3(256), 0, s, S_part, info, G, tw, S);
       hipLaunchKernelGGL((k_ifft_rows_phase<1024, 1>), dim3(1024), dim3(256), 0, s, S, tw, obj, phase);
       break;
     default:
