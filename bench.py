@@ -165,6 +165,56 @@ def fit_leg(dev, n_sets: int, train_steps: int, search_trials: int, learned_tria
     }
 
 
+def series_leg(data, spec, dev, knobs: dict | None, in_flight: int = 3, recons: int = 30) -> dict:
+    """A series of reconstructions with ``in_flight`` plans on as many
+    streams (default priority), each plan's raw-data stream pass ordered
+    after the previous plan's (``ssb_recon_u16_after``), so one stream pass
+    runs while the other plans finish their BF stages.  Reported beside
+    ``value`` (one reconstruction at a time), never as it; every plan's
+    phase must equal the one-at-a-time result bit for bit."""
+    import torch
+    from quantem.gpu_amd.ssb.compute.hip.engine import SSBPlan, HipSSBEngine
+    from quantem.gpu_amd.ssb.compute.hip import tuner
+    engs = [HipSSBEngine(SSBPlan(spec, device=dev)) for _ in range(in_flight)]
+    for e in engs:
+        for knob, v in (knobs or {}).items():
+            e.tune_set(tuner.KNOBS[knob], int(v))
+    phases = [torch.empty((SCAN, SCAN), dtype=torch.float32, device=dev) for _ in range(in_flight)]
+    main = torch.cuda.current_stream(dev)
+    streams = [main] + [torch.cuda.Stream(device=dev) for _ in range(in_flight - 1)]
+
+    def run(n):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(main)
+        for st in streams[1:]:
+            st.wait_event(e0)
+        for i in range(n):
+            k = i % in_flight
+            with torch.cuda.stream(streams[k]):
+                engs[k].recon_u16(data, C10, 0.0, 0.0, 0.0, 0.0, None, phase_out=phases[k],
+                                  after=engs[(k - 1) % in_flight])
+        for st in streams[1:]:
+            ev = torch.cuda.Event()
+            ev.record(st)
+            main.wait_event(ev)
+        e1.record(main)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / n
+
+    run(2 * in_flight)
+    ms = run(recons)
+    single = torch.empty_like(phases[0])
+    engs[0].recon_u16(data, C10, 0.0, 0.0, 0.0, 0.0, None, phase_out=single)
+    torch.cuda.synchronize()
+    identical = all(torch.equal(single, ph) for ph in phases)
+    for e in engs:
+        e.plan.close()
+    return {"in_flight": in_flight, "recons": recons, "ms_per_recon": ms, "recon_per_s": 1e3 / ms,
+            "phase_identical_to_single": bool(identical)}
+
+
 def decode_leg(data, reps: int = 5, distinct: int = 64) -> dict:
     """GPU decode of bitshuffle-LZ4 chunks (one frame each) into the resident
     uint16 array: ``distinct`` frames of the benchmark input compressed on
@@ -236,6 +286,8 @@ def main() -> int:
     ap.add_argument("--fit-refine", default="newton", choices=["newton", "nelder-mead", "quadratic"])
     ap.add_argument("--fit-rounds", type=int, default=None,
                     help="batches of the newton refine (default 8) or rounds of the quadratic refine (default 4)")
+    ap.add_argument("--no-series", action="store_true",
+                    help="skip the series measurement (3 reconstructions in flight)")
     ap.add_argument("--no-decode", action="store_true",
                     help="skip the bitshuffle-LZ4 decode measurement")
     ap.add_argument("--no-tune", action="store_true",
@@ -360,6 +412,11 @@ def main() -> int:
     else:
         result["cpu_baseline"] = None
     eng.plan.close()
+    if not args.no_series:
+        log("series of reconstructions, 3 in flight")
+        result["series"] = series_leg(data, spec, dev, (tuning or {}).get("config"))
+        log(f"series: {result['series']['ms_per_recon']:.4f} ms/recon, identical "
+            f"{result['series']['phase_identical_to_single']}")
     if not args.no_decode:
         log("bitshuffle-LZ4 decode of the same frames")
         result["decode"] = decode_leg(data)
