@@ -292,6 +292,21 @@ int ssb_bslz4_decode_u16(const uint8_t* comp, long long comp_bytes, const long l
                          uint16_t* out, void* workspace, long long workspace_bytes, int* status_host, void* stream);
 
 /*
+ * Integer detector binning of resident uint16 frames src[frames][rows][cols]
+ * into out[frames][rows/factor][cols/factor]: each output pixel is the
+ * exact sum of its factor x factor block, dead pixels (bad_mask[rows][cols]
+ * nonzero; NULL = none) counted as 0, trailing rows/columns that do not
+ * fill a block trimmed.  out_bits 32: uint32 sums (the widened default of
+ * the reference's binned loads); 16: the low 16 bits, as the reference's
+ * uint64 sum cast back to uint16.  Asynchronous on `stream`.
+ * Replaces _bin_sum (qg/io/backends/cpu/reference.py:27-42, dead pixels
+ * zeroed first at :96-111) and the streaming bin of the CUDA loader
+ * (qg/io/load.py:3009-3026, 3398-3415).
+ */
+int ssb_bin_u16(const uint16_t* src, long long frames, int rows, int cols, int factor, const uint8_t* bad_mask,
+                int out_bits, void* out, void* stream);
+
+/*
  * Tuning knobs of the reconstruction path (no reference counterpart: the
  * reference fixes its launch shapes per scan size, cuda/kernels/fft256.py:1051-1058).
  * QSSB_TUNE_STREAM_WG: workgroups of the raw-data stream pass;

@@ -725,13 +725,21 @@ def gather_bf_columns(data: np.ndarray, rows: np.ndarray, cols: np.ndarray) -> n
     return np.ascontiguousarray(flat[:, rows, cols].T)
 
 
-def bin_sum(data: np.ndarray, factor: int) -> np.ndarray:
+def bin_sum(data: np.ndarray, factor: int, pixel_mask: np.ndarray | None = None, dtype=None) -> np.ndarray:
     """Integer detector binning, ``_bin_sum`` (``qg/io/backends/cpu/reference.py:27-42``):
-    uint64 sums of ``factor x factor`` blocks, incomplete edges trimmed."""
+    uint64 sums of ``factor x factor`` blocks, incomplete edges trimmed.
+    ``pixel_mask`` (nonzero = dead) zeroes those pixels first
+    (``reference.py:96-111``); ``dtype`` casts the sums as ``astype`` does
+    (the CPU loader casts back to the source dtype, keeping the low bits;
+    the CUDA loader widens binned loads to uint32, ``qg/io/load.py:3009-3026``).
+    Default: the uint64 sums."""
+    if pixel_mask is not None:
+        data = np.where(np.asarray(pixel_mask) != 0, 0, data).astype(data.dtype)
     kr, kc = data.shape[-2] // factor, data.shape[-1] // factor
     trimmed = data[..., :kr * factor, :kc * factor]
     shaped = trimmed.reshape(*data.shape[:-2], kr, factor, kc, factor)
-    return shaped.astype(np.uint64).sum(axis=(-3, -1))
+    summed = shaped.astype(np.uint64).sum(axis=(-3, -1))
+    return summed if dtype is None else summed.astype(dtype)
 
 
 __all__ = [name for name in dir() if not name.startswith("_")]

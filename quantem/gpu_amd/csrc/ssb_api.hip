@@ -1029,6 +1029,19 @@ int ssb_bslz4_decode_u16(const uint8_t* comp, long long comp_bytes, const long l
   return QSSB_OK;
 }
 
+int ssb_bin_u16(const uint16_t* src, long long frames, int rows, int cols, int factor, const uint8_t* bad_mask,
+                int out_bits, void* out, void* stream) {
+  if (frames < 0 || rows < 1 || cols < 1) return fail(QSSB_ERR_ARG, "binning needs frames >= 0 and a non-empty detector");
+  if (factor < 1 || factor > 256) return fail(QSSB_ERR_ARG, "det_bin must be in [1, 256]");
+  if (factor > rows || factor > cols) return fail(QSSB_ERR_ARG, "det_bin larger than the detector");
+  if (out_bits != 16 && out_bits != 32) return fail(QSSB_ERR_ARG, "binned output must be 16 or 32 bits");
+  if (frames == 0) return QSSB_OK;
+  if (!src || !out) return fail(QSSB_ERR_ARG, "binning buffers must not be NULL");
+  HIP_TRY(launch_bin_u16(src, frames, rows, cols, factor, bad_mask, out_bits == 32, out, as_stream(stream)),
+          "bin_u16");
+  return QSSB_OK;
+}
+
 long long ssb_bslz4_encode_bound(long long n_elems, int block_bytes) {
   if (n_elems < 1 || block_bytes < 16 || block_bytes > 8192 || block_bytes % 16)
     return fail(QSSB_ERR_ARG, "block_bytes must be a multiple of 16 in [16, 8192] and the chunk non-empty");

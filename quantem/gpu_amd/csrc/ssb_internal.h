@@ -165,6 +165,10 @@ long long bslz4_slots(long long chunk_elems, int block_elems);
 long long bslz4_workspace_bytes(int n_chunks, long long chunk_elems, int block_elems);
 long long bslz4_encode_bound(long long n_elems, int block_bytes);
 long long bslz4_encode_u16(const uint16_t* v, long long n, int block_bytes, uint8_t* dst, long long cap);
+// integer detector binning (ssb_bin.hip): wide = uint32 output, else uint16
+// (low 16 bits); bad = per-pixel dead mask (nonzero = dead) or nullptr
+hipError_t launch_bin_u16(const uint16_t* src, long long frames, int rows, int cols, int factor,
+                          const uint8_t* bad, bool wide, void* out, hipStream_t s);
 hipError_t launch_bslz4_decode_u16(const uint8_t* comp, long long comp_bytes, const long long* chunk_off,
                                    const long long* chunk_len, int n_chunks, long long chunk_elems, int block_elems,
                                    uint16_t* out, void* workspace, hipStream_t s);

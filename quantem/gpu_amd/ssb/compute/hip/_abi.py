@@ -60,6 +60,7 @@ SIGNATURES: dict[str, list] = {
     "ssb_bslz4_encode_u16": [_vp, ctypes.c_longlong, _c_int, _vp, ctypes.c_longlong],
     "ssb_bslz4_decode_u16": [_vp, ctypes.c_longlong, _vp, _vp, _c_int, ctypes.c_longlong, _c_int, _vp, _vp,
                              ctypes.c_longlong, ctypes.POINTER(_c_int), _vp],
+    "ssb_bin_u16": [_vp, ctypes.c_longlong, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _vp],
     "ssb_tune_candidates": [_c_int, _c_int, ctypes.POINTER(_c_int), _c_int],
     "ssb_tune_set": [_vp, _c_int, _c_int],
     "ssb_tune_get": [_vp, _c_int, ctypes.POINTER(_c_int)],

@@ -156,6 +156,30 @@ def test_gather_bf_columns_and_bin_sum_exact():
             np.testing.assert_array_equal(binned[..., i, j], ref)
 
 
+def test_bin_sum_masks_dead_pixels_and_casts_like_the_loaders():
+    """``_bin_sum`` (``qg/io/backends/cpu/reference.py:27-42``, mask at
+    ``:96-111``) by brute force: dead pixels count 0, edges trimmed; the
+    uint16 cast keeps the low 16 bits (four 65535 counts -> 65532), the
+    uint32 output is the exact sum."""
+    rng = np.random.default_rng(5)
+    data = rng.integers(0, 65536, size=(2, 2, 7, 9), dtype=np.uint16)
+    data[0, 0, :2, :2] = 65535
+    mask = rng.random((7, 9)) < 0.2
+    mask[:2, :2] = False
+    for factor in (1, 2, 3):
+        exact = O.bin_sum(data, factor, mask)
+        kr, kc = 7 // factor, 9 // factor
+        assert exact.shape == (2, 2, kr, kc) and exact.dtype == np.uint64
+        for i in range(kr):
+            for j in range(kc):
+                blk = data[..., i * factor:(i + 1) * factor, j * factor:(j + 1) * factor].astype(np.int64)
+                keep = ~mask[i * factor:(i + 1) * factor, j * factor:(j + 1) * factor]
+                np.testing.assert_array_equal(exact[..., i, j], (blk * keep).sum(axis=(-2, -1)))
+        np.testing.assert_array_equal(O.bin_sum(data, factor, mask, np.uint32), exact.astype(np.uint32))
+        np.testing.assert_array_equal(O.bin_sum(data, factor, mask, np.uint16), (exact % 65536).astype(np.uint16))
+    assert int(O.bin_sum(data, 2, None, np.uint16)[0, 0, 0, 0]) == (4 * 65535) % 65536 == 65532
+
+
 # ------------------------------------------------------ geometry and chi
 def test_detector_rotation_is_a_rotation_of_k():
     case = C.make_engine_case(128, 7)
