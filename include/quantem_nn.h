@@ -2,7 +2,7 @@
  * quantem_nn.h -- C ABI of the learned aberration initializer's network
  * runtime in libquantem_ssb_hip.so (gfx950).  These entry points have no
  * counterpart in the reference: the network only seeds the batched
- * aberration search, whose exact SSB objective judges its output.
+ * aberration search, whose exact SSB objective scores its output.
  *
  * All arrays are device pointers; `stream` is a hipStream_t.  Status 0 on
  * success, -1 for bad arguments, -4 for a launch failure.  bf16 values are

@@ -1,7 +1,7 @@
 // Learned aberration initializer: the device side of a small convolutional
 // network trained and run on the GPU.  No reference counterpart exists; the
 // network predicts (C10, C30, rotation) from trotter feature maps and only
-// seeds the batched search, whose exact SSB objective judges the result.
+// seeds the batched search, whose exact SSB objective scores the result.
 //
 // Every layer is a GEMM on the bf16 matrix cores (inputs rounded to bf16,
 // products accumulated in fp32), fed by im2col for the 3x3 stride-2

@@ -7,7 +7,7 @@ around the BF disk as |G|/max|G| and the phase of G(k) conj(G(k_centre))
 (16 channels, 32 x 32) -- to (C10, C30, rotation).  It is trained on the
 weak-phase SSB forward model G_b(Q) = Gamma_b(Q; theta) Phi(Q) + noise in
 the engine's own conventions, generated on the GPU per batch.  Its output
-only seeds a local batched search on the exact SSB objective, which judges
+only seeds a local batched search on the exact SSB objective, which scores
 it; the global batched search stays the fallback.
 
 Architecture: four 3x3 stride-2 convolutions (16->32->64->64->64, ReLU,
