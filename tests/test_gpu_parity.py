@@ -306,6 +306,44 @@ def test_full_size_benchmark_config(cuda):
     assert rms < PHASE_RMS and p999 < PHASE_P999
 
 
+def test_reconstructions_in_flight_match_one_at_a_time(cuda):
+    """A series with three plans on three streams, each plan's stream pass
+    ordered after the previous plan's (``recon_u16(..., after=)``) over two
+    different datasets: every phase is bit-identical to the same plan's
+    one-at-a-time result; a plan cannot be ordered after itself."""
+    torch = cuda
+    cfgs = [synthetic(128, 64, {"C10": 150.0}, seed=31), synthetic(128, 64, {"C10": -80.0}, seed=32)]
+    datas = [torch.from_numpy(h).cuda() for _, h in cfgs]
+    engs = [engine_for_data(torch, cfgs[0][1], semiangle_mrad=20.0, det_sampling=1.25, step=cfgs[0][0].step_A)
+            for _ in range(3)]
+    c10s = (150.0, -80.0)
+    ref = []
+    for d, c10 in zip(datas, c10s):
+        ph = torch.empty((128, 128), dtype=torch.float32, device="cuda")
+        engs[0].recon_u16(d, c10, 0.0, 0.0, 0.0, 0.0, None, phase_out=ph)
+        ref.append(ph)
+    torch.cuda.synchronize()
+    main = torch.cuda.current_stream()
+    streams = [main, torch.cuda.Stream(), torch.cuda.Stream()]
+    start = torch.cuda.Event()
+    start.record(main)
+    for st in streams[1:]:
+        st.wait_event(start)
+    outs = []
+    for i in range(6):
+        k, which = i % 3, i % 2
+        ph = torch.empty((128, 128), dtype=torch.float32, device="cuda")
+        with torch.cuda.stream(streams[k]):
+            engs[k].recon_u16(datas[which], c10s[which], 0.0, 0.0, 0.0, 0.0, None, phase_out=ph,
+                              after=engs[(k - 1) % 3])
+        outs.append((which, ph))
+    torch.cuda.synchronize()
+    for which, ph in outs:
+        assert torch.equal(ph, ref[which])
+    with pytest.raises(ValueError):
+        engs[0].recon_u16(datas[0], 150.0, 0.0, 0.0, 0.0, 0.0, None, after=engs[0])
+
+
 # ------------------------------------------------------------- contract
 def test_errors_follow_reference_types(cuda):
     eng = make_engine(cuda, 128, (16, 16), 8, 0.0197, 20.0, (1.0, 1.0), 0.5)
