@@ -138,53 +138,59 @@ k_bslz4_decode(const uint8_t* __restrict__ comp, const BlockDesc* __restrict__ t
     __syncthreads();                                   // previous block's readers are done
     for (int t = lane; t < cs; t += 64) cin[t] = comp[bd.src + t];
     __syncthreads();
-    // LZ4 block: every lane walks the same sequence headers
-    int ip = 0, op = 0, err = 0;
-    while (true) {
-      if (ip >= cs) { err = kErrStream; break; }
-      const int token = cin[ip++];
-      int lit = token >> 4;
-      if (lit == 15) {
-        int b;
-        do {
-          if (ip >= cs) { err = kErrStream; break; }
-          b = cin[ip++];
-          lit += b;
-        } while (b == 255);
-        if (err) break;
+    // LZ4 block: every lane walks the same sequence headers.  The workgroup
+    // is one wave, whose LDS operations complete in issue order, so the
+    // copies need only a compiler ordering fence between them, not a
+    // barrier (4% faster on detector data than with barriers).  Holding the
+    // block in registers and reading the headers with readlane measured
+    // slower (157 vs 215 GB/s decoded).
+    auto parse = [&](auto byte_at, int& ip, int& op) -> int {
+      while (true) {
+        if (ip >= cs) return kErrStream;
+        const int token = byte_at(ip++);
+        int lit = token >> 4;
+        if (lit == 15) {
+          int b;
+          do {
+            if (ip >= cs) return kErrStream;
+            b = byte_at(ip++);
+            lit += b;
+          } while (b == 255);
+        }
+        if (ip + lit > cs || op + lit > osz) return kErrStream;
+        for (int t = lane; t < lit; t += 64) cout[op + t] = cin[ip + t];
+        ip += lit;
+        op += lit;
+        if (ip == cs) return 0;                          // the last sequence has no match
+        if (ip + 2 > cs) return kErrStream;
+        const int off = byte_at(ip) | (byte_at(ip + 1) << 8);
+        ip += 2;
+        int ml = token & 15;
+        if (ml == 15) {
+          int b;
+          do {
+            if (ip >= cs) return kErrStream;
+            b = byte_at(ip++);
+            ml += b;
+          } while (b == 255);
+        }
+        ml += 4;
+        if (off == 0 || off > op || op + ml > osz) return kErrStream;
+        asm volatile("" ::: "memory");                   // literals before the match reads them
+        if (off >= ml) {                                 // uniform: no overlap, a plain copy
+          for (int t = lane; t < ml; t += 64) cout[op + t] = cout[op - off + t];
+        } else if (off == 1) {                           // a run of one byte
+          const uint8_t r = cout[op - 1];
+          for (int t = lane; t < ml; t += 64) cout[op + t] = r;
+        } else {
+          for (int t = lane; t < ml; t += 64) cout[op + t] = cout[op - off + (t % off)];
+        }
+        op += ml;
+        asm volatile("" ::: "memory");
       }
-      if (ip + lit > cs || op + lit > osz) { err = kErrStream; break; }
-      for (int t = lane; t < lit; t += 64) cout[op + t] = cin[ip + t];
-      ip += lit;
-      op += lit;
-      if (ip == cs) break;                             // the last sequence has no match
-      if (ip + 2 > cs) { err = kErrStream; break; }
-      const int off = cin[ip] | (cin[ip + 1] << 8);
-      ip += 2;
-      int ml = token & 15;
-      if (ml == 15) {
-        int b;
-        do {
-          if (ip >= cs) { err = kErrStream; break; }
-          b = cin[ip++];
-          ml += b;
-        } while (b == 255);
-        if (err) break;
-      }
-      ml += 4;
-      if (off == 0 || off > op || op + ml > osz) { err = kErrStream; break; }
-      __syncthreads();                                 // literals written before the match reads them
-      if (off >= ml) {                                 // uniform: no overlap, a plain copy
-        for (int t = lane; t < ml; t += 64) cout[op + t] = cout[op - off + t];
-      } else if (off == 1) {                           // a run of one byte
-        const uint8_t r = cout[op - 1];
-        for (int t = lane; t < ml; t += 64) cout[op + t] = r;
-      } else {
-        for (int t = lane; t < ml; t += 64) cout[op + t] = cout[op - off + (t % off)];
-      }
-      op += ml;
-      __syncthreads();
-    }
+    };
+    int ip = 0, op = 0;
+    int err = parse([&](int i) -> int { return cin[i]; }, ip, op);
     if (!err && op != osz) err = kErrStream;
     if (err) {
       if (lane == 0) atomicOr(status, err);
