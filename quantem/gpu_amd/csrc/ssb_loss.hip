@@ -409,7 +409,7 @@ hipError_t launch_phase_moments(int N, const float2* Gt, int gcols, const BfInfo
         break;
       case 256:
         QSSB_P1(256, 8, 512);
-        QSSB_P2(256, 4);
+        QSSB_P2(256, 2);
         break;
       case 512:
         QSSB_P1(512, 4, 512);
