@@ -14,12 +14,23 @@ from quantem.gpu_amd.ssb.compute.hip import _abi
 ROOT = Path(__file__).resolve().parents[1]
 
 
-def declared_symbols() -> list[str]:
-    names = []
+_RET = {"int": ctypes.c_int, "long long": ctypes.c_longlong, "const char*": ctypes.c_char_p}
+
+
+def declared() -> dict[str, object]:
+    """Every entry point the headers declare, with the ctypes type of its return value."""
+    out = {}
     for header in sorted((ROOT / "include").glob("*.h")):
         text = re.sub(r"/\*.*?\*/", "", header.read_text(), flags=re.S)
-        names += re.findall(r"^\s*(?:int|const char\s*\*)\s+((?:ssb|qnn)_\w+)\s*\(", text, flags=re.M)
-    return names
+        for ret, name in re.findall(r"^\s*(int|long long|const char\s*\*)\s+((?:ssb|qnn)_\w+)\s*\(", text,
+                                    flags=re.M):
+            assert name not in out, f"{name} declared twice"
+            out[name] = _RET[re.sub(r"\s*\*", "*", re.sub(r"\s+", " ", ret.strip()))]
+    return out
+
+
+def declared_symbols() -> list[str]:
+    return list(declared())
 
 
 @pytest.fixture(scope="module")
@@ -40,9 +51,10 @@ def test_header_declares_the_boundary():
 def test_library_exports_every_declared_symbol(lib):
     missing = [n for n in declared_symbols() if getattr(lib, n, None) is None]
     assert not missing, f"not exported: {missing}"
+    decl = declared()
     for n in _abi.SIGNATURES:
-        if n in declared_symbols():
-            assert getattr(lib, n).restype is ctypes.c_int
+        if n in decl:
+            assert getattr(lib, n).restype is decl[n], n
 
 
 def test_version_and_stage_names(lib):
