@@ -299,7 +299,7 @@ def main() -> int:
                     help="skip the CPU baseline (used for profiler runs)")
     ap.add_argument("--fit-sets", type=int, default=5,
                     help="held-out datasets of the aberration-fit comparison (0 skips it)")
-    ap.add_argument("--fit-train-steps", type=int, default=1500)
+    ap.add_argument("--fit-train-steps", type=int, default=4000)
     ap.add_argument("--fit-search-trials", type=int, default=512)
     ap.add_argument("--fit-trials", type=int, default=16, help="trials of the local search around the learned start")
     ap.add_argument("--fit-refine", default="newton", choices=["newton", "nelder-mead", "quadratic"])

@@ -43,8 +43,11 @@ int qnn_cast_bf16(const float* x, long long n, uint16_t* y, void* stream);
 /* dZ = dY * (Y > 0) (Y NULL: identity), with its bf16 copy dZh. */
 int qnn_relu_bwd(const float* Y, const float* dY, long long n, float* dZ, uint16_t* dZh, void* stream);
 
-/* out[n] = sum_m dZ[m][n] (compensated, fixed order). */
-int qnn_colsum(const float* dZ, int M, int N, float* out, void* stream);
+/* out[n] = sum_m dZ[m][n] (compensated, fixed order: partial sums over
+ * chunks of rows, then over the chunks).  `work` holds at least
+ * qnn_colsum_workspace(M, N) floats of device memory. */
+long long qnn_colsum_workspace(int M, int N);
+int qnn_colsum(const float* dZ, int M, int N, float* out, float* work, long long work_floats, void* stream);
 
 /* loss = mean (pred - target)^2 over n values; grad = 2 (pred - target) / n. */
 int qnn_mse(const float* pred, const float* target, int n, float* grad, float* loss, void* stream);

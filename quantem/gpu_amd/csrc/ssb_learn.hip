@@ -173,19 +173,55 @@ k_relu_bwd(const float* __restrict__ Y, const float* __restrict__ dY, long long 
   }
 }
 
-// bias gradient: column sums of dZ[M][N], one thread per column, fixed order
+// bias gradient: column sums of dZ[M][N] in a fixed order, in two passes.
+// Pass 1: one workgroup per (64-column tile, chunk of kColsumRows rows),
+// 4 row lanes per column, each a compensated sum, combined in lane order;
+// pass 2: the chunks' partial sums per column, compensated, in chunk order.
+constexpr int kColsumRows = 2048;
+
 __global__ void __launch_bounds__(256)
-k_colsum(const float* __restrict__ dZ, int M, int N, float* __restrict__ out) {
+k_colsum_part(const float* __restrict__ dZ, int M, int N, float* __restrict__ part) {
+  __shared__ float sh[4][64];
+  const int lc = threadIdx.x & 63, lr = threadIdx.x >> 6;
+  const int n = blockIdx.x * 64 + lc;
+  const int m0 = blockIdx.y * kColsumRows;
+  const int m1 = min(M, m0 + kColsumRows);
+  float s = 0.0f, c = 0.0f;
+  if (n < N) {
+    for (int m = m0 + lr; m < m1; m += 4) {
+      const float y = dZ[(long long)m * N + n] - c;
+      const float t = s + y;
+      c = (t - s) - y;
+      s = t;
+    }
+  }
+  sh[lr][lc] = s - c;
+  __syncthreads();
+  if (lr == 0 && n < N) {
+    float s2 = 0.0f, c2 = 0.0f;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const float y = sh[r][lc] - c2;
+      const float t = s2 + y;
+      c2 = (t - s2) - y;
+      s2 = t;
+    }
+    part[(long long)blockIdx.y * N + n] = s2 - c2;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_colsum_finish(const float* __restrict__ part, int chunks, int N, float* __restrict__ out) {
   const int n = blockIdx.x * 256 + threadIdx.x;
   if (n >= N) return;
   float s = 0.0f, c = 0.0f;
-  for (int m = 0; m < M; ++m) {
-    const float y = dZ[(long long)m * N + n] - c;
+  for (int k = 0; k < chunks; ++k) {
+    const float y = part[(long long)k * N + n] - c;
     const float t = s + y;
     c = (t - s) - y;
     s = t;
   }
-  out[n] = s;
+  out[n] = s - c;
 }
 
 // mean squared error over B x D and its gradient (2/(B D)) (p - t)
@@ -471,10 +507,19 @@ int qnn_relu_bwd(const float* Y, const float* dY, long long n, float* dZ, uint16
   return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
-int qnn_colsum(const float* dZ, int M, int N, float* out, void* stream) {
-  if (!dZ || !out || M < 1 || N < 1) return -1;
-  hipLaunchKernelGGL(k_colsum, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
-                     dZ, M, N, out);
+long long qnn_colsum_workspace(int M, int N) {
+  if (M < 1 || N < 1) return -1;
+  return (long long)((M + kColsumRows - 1) / kColsumRows) * N;
+}
+
+int qnn_colsum(const float* dZ, int M, int N, float* out, float* work, long long work_floats, void* stream) {
+  if (!dZ || !out || !work || M < 1 || N < 1) return -1;
+  const int chunks = (M + kColsumRows - 1) / kColsumRows;
+  if (chunks > 65535 || work_floats < (long long)chunks * N) return -1;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_colsum_part, dim3((unsigned)((N + 63) / 64), (unsigned)chunks), dim3(256), 0, s, dZ, M, N,
+                     work);
+  hipLaunchKernelGGL(k_colsum_finish, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, work, chunks, N, out);
   return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 

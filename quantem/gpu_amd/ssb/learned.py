@@ -41,7 +41,8 @@ def _lib():
         "qnn_col2im": [vp, ci, ci, ci, ci, vp, vp],
         "qnn_cast_bf16": [vp, i64, vp, vp],
         "qnn_relu_bwd": [vp, vp, i64, vp, vp, vp],
-        "qnn_colsum": [vp, ci, ci, vp, vp],
+        "qnn_colsum_workspace": [ci, ci],
+        "qnn_colsum": [vp, ci, ci, vp, vp, ctypes.c_longlong, vp],
         "qnn_mse": [vp, vp, ci, vp, vp, vp],
         "qnn_adam": [vp, vp, vp, vp, i64, cf, cf, cf, cf, ci, vp, vp],
         "qnn_features_synth": [vp, vp, ci, cf, cf, cf, cf, vp, ctypes.c_uint, vp, vp],
@@ -50,6 +51,7 @@ def _lib():
         fn = getattr(lib, name)
         fn.argtypes = argtypes
         fn.restype = ci
+    lib.qnn_colsum_workspace.restype = i64
     return lib
 
 
@@ -133,6 +135,12 @@ class LearnedInit:
         _call("qnn_gemm_bf16", _p(A), sam, sak, _p(B), sbk, sbn, _p(C), scm, scn, _p(bias), M, N, K,
               1 if relu else 0, 1 if acc else 0, self._s())
 
+    def _colsum(self, dz, M, N, out):
+        """Bias gradient: column sums of dz[M][N] (qnn_colsum, two fixed-order passes)."""
+        need = int(_lib().qnn_colsum_workspace(M, N))
+        ws = self.torch.empty(max(need, 1), dtype=self.torch.float32, device=self.dev)
+        _call("qnn_colsum", _p(dz), M, N, _p(out), _p(ws), need, self._s())
+
     def _gemm_wgrad(self, A, sam, sak, B, sbk, sbn, C, M, N, K):
         """Weight gradient C[M][N] with a long reduction K, split over K."""
         splits = max(1, min(64, K // 1024))
@@ -202,7 +210,7 @@ class LearnedInit:
             dw = torch.empty((fout, fin), dtype=torch.float32, device=self.dev)
             self._gemm_wgrad(dzh, 1, fout, xh, fin, 1, dw, fout, fin, Bn)
             db = torch.empty(fout, dtype=torch.float32, device=self.dev)
-            _call("qnn_colsum", _p(dz), Bn, fout, _p(db), self._s())
+            self._colsum(dz, Bn, fout, db)
             grads[li] = (dw, db)
             dx = torch.empty((Bn, fin), dtype=torch.float32, device=self.dev)
             self._gemm(dzh, fout, 1, self.half[li], fin, 1, dx, fin, 1, None, Bn, fin, fout)
@@ -219,7 +227,7 @@ class LearnedInit:
             dw = torch.empty((cout, K), dtype=torch.float32, device=self.dev)
             self._gemm_wgrad(dzh, 1, cout, cols, K, 1, dw, cout, K, M)
             db = torch.empty(cout, dtype=torch.float32, device=self.dev)
-            _call("qnn_colsum", _p(dz), M, cout, _p(db), self._s())
+            self._colsum(dz, M, cout, db)
             grads[li] = (dw, db)
             if li > 0:
                 dcols = torch.empty((M, K), dtype=torch.float32, device=self.dev)
