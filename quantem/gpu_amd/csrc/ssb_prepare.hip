@@ -72,31 +72,34 @@ k_dp_partial(const uint16_t* __restrict__ data, long long frames, int D, int fra
 
 // Pass 2: fold the frame ranges into exact uint64 sums; mean = f32(sum)/f32(N)
 // with one correctly rounded division (backend.py:438-441).  A workgroup
-// owns 64 pixels; its four waves fold interleaved quarters of the ranges
+// owns 64 pixels; its LANES waves fold interleaved shares of the ranges
 // (each wave reads 256 contiguous bytes per range) and combine in LDS.
-__global__ void __launch_bounds__(256)
+template <int LANES>
+__global__ void __launch_bounds__(64 * LANES)
 k_dp_reduce(const uint32_t* __restrict__ partial, int ranges, int D, long long frames,
             unsigned long long* __restrict__ sums, float* __restrict__ mean) {
-  __shared__ unsigned long long part[4][64];
+  __shared__ unsigned long long part[LANES][64];
   const int lane = threadIdx.x & 63;
   const int quarter = threadIdx.x >> 6;
   const int i = blockIdx.x * 64 + lane;
   unsigned long long s = 0ull;
   if (i < D) {
     int r = quarter;
-    for (; r + 12 < ranges; r += 16) {
+    for (; r + 3 * LANES < ranges; r += 4 * LANES) {
       const uint32_t a = partial[(long long)r * D + i];
-      const uint32_t b = partial[(long long)(r + 4) * D + i];
-      const uint32_t c = partial[(long long)(r + 8) * D + i];
-      const uint32_t d = partial[(long long)(r + 12) * D + i];
+      const uint32_t b = partial[(long long)(r + LANES) * D + i];
+      const uint32_t c = partial[(long long)(r + 2 * LANES) * D + i];
+      const uint32_t d = partial[(long long)(r + 3 * LANES) * D + i];
       s += (unsigned long long)a + b + c + d;
     }
-    for (; r < ranges; r += 4) s += partial[(long long)r * D + i];
+    for (; r < ranges; r += LANES) s += partial[(long long)r * D + i];
   }
   part[quarter][lane] = s;
   __syncthreads();
   if (quarter == 0 && i < D) {
-    const unsigned long long t = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
+    unsigned long long t = 0ull;
+#pragma unroll
+    for (int k = 0; k < LANES; ++k) t += part[k][lane];
     sums[i] = t;
     mean[i] = (float)t / (float)frames;
   }
@@ -540,7 +543,10 @@ hipError_t launch_mean_dp_reduce(long long frames, int D, const uint32_t* partia
                                  unsigned long long* sums, float* mean, hipStream_t s) {
   int ranges, fpr;
   range_split(frames, D, max_ranges, target_wg, ranges, fpr);
-  hipLaunchKernelGGL(k_dp_reduce, dim3((D + 63) / 64), dim3(256), 0, s, partial, ranges, D, frames, sums, mean);
+  // 16 range lanes per pixel column: 1024-thread workgroups keep enough
+  // loads in flight for a reduction that has only D / 64 workgroups
+  hipLaunchKernelGGL((k_dp_reduce<16>), dim3((D + 63) / 64), dim3(1024), 0, s, partial, ranges, D, frames, sums,
+                     mean);
   return hipGetLastError();
 }
 
