@@ -64,3 +64,12 @@ def test_objective_modes_follow_the_reference_contract():
         _check_objective_mode("native")
     with pytest.raises(ValueError, match="objective_mode"):
         _check_objective_mode("fast")
+
+
+def test_bin_frames_checks_its_input_before_any_gpu_call():
+    torch = pytest.importorskip("torch")
+    from quantem.gpu_amd.ssb.compute.hip.binning import bin_frames
+    with pytest.raises(TypeError):
+        bin_frames(torch.zeros((2, 8, 8), dtype=torch.uint16), 2)            # host tensor
+    with pytest.raises(TypeError):
+        bin_frames(np.zeros((2, 8, 8), dtype=np.uint16), 2)

@@ -160,3 +160,19 @@ def test_newton_refine_respects_bounds_and_validates_steps():
     np.testing.assert_allclose(x, [1.0, -1.0], atol=1e-6)
     with pytest.raises(ValueError):
         opt.newton_refine(_quadratic(np.eye(2), [0.0, 0.0]), np.zeros(2), [0.1, 0.0])
+
+
+def test_newton_refine_stops_on_a_small_predicted_decrease():
+    A = np.diag([2.0, 1.0, 4.0])
+    calls = []
+
+    def evaluate(points):
+        calls.append(len(points))
+        return _quadratic(A, [0.1, -0.2, 0.3], 1.0)(points)
+    # the first step (inside the trust box of 4 stencil widths) solves the
+    # quadratic; the second batch, at the minimum, predicts ~0 decrease,
+    # below ftol, and stops.  The polish needs 2 stencils' worth of points
+    # near the best one; the first batch lies farther out, so it is skipped.
+    x, fx, n = opt.newton_refine(evaluate, np.zeros(3), 0.1, max_iter=8, ftol=1e-12, polish=True)
+    np.testing.assert_allclose(x, [0.1, -0.2, 0.3], atol=1e-9)
+    assert calls == [10, 10] and n == 20
