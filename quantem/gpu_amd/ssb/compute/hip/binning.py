@@ -47,6 +47,10 @@ def bin_frames(frames, det_bin: int, pixel_mask=None, dtype=None, out=None):
         out = torch.empty(shape, dtype=dtype, device=src.device)
     elif tuple(out.shape) != shape or out.dtype != dtype or not out.is_contiguous():
         raise ValueError(f"out must be a contiguous {dtype} tensor of shape {shape}")
+    elif not isinstance(out, torch.Tensor) or not out.is_cuda or out.device != src.device:
+        # the kernel writes through out's address: a host or other-GPU
+        # tensor must be refused before any launch
+        raise ValueError(f"out must live on {src.device}; got {out.device}")
     bad = None
     if pixel_mask is not None:
         bad = torch.as_tensor(pixel_mask).to(device=src.device)

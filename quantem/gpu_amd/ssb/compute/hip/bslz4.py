@@ -61,6 +61,9 @@ class Bslz4Chunks:
             out = torch.empty(self.shape, dtype=torch.uint16, device=self.device)
         elif out.dtype != torch.uint16 or out.numel() != self.n * self.chunk_elems or not out.is_contiguous():
             raise ValueError("out must be a contiguous uint16 tensor of the decoded size")
+        elif not out.is_cuda or out.device != self.device:
+            # the decode kernel writes through out's address
+            raise ValueError(f"out must live on {self.device}; got {out.device}")
         status = ctypes.c_int(0)
         _abi.call("ssb_bslz4_decode_u16", _abi.ptr(self.comp), int(self.comp.numel()), _abi.ptr(self.offsets),
                   _abi.ptr(self.sizes), self.n, self.chunk_elems, self.block_bytes, _abi.ptr(out),

@@ -36,6 +36,26 @@ def _torch():
     return torch
 
 
+def check_out(tensor, name: str, numel: int, dtype, device) -> None:
+    """Reject a caller-supplied output the native call would write out of
+    bounds or through a host address: it must be a contiguous ``dtype``
+    tensor on ``device`` holding at least ``numel`` elements.  ``None``
+    (no output wanted) passes."""
+    if tensor is None:
+        return
+    import torch
+    if not isinstance(tensor, torch.Tensor):
+        raise ValueError(f"{name} must be a torch tensor")
+    if tensor.dtype != dtype:
+        raise ValueError(f"{name} must have dtype {dtype}; got {tensor.dtype}")
+    if not tensor.is_cuda or tensor.device != torch.device(device):
+        raise ValueError(f"{name} must live on {device}; got {tensor.device}")
+    if not tensor.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if tensor.numel() < numel:
+        raise ValueError(f"{name} holds {tensor.numel()} elements; {numel} are written")
+
+
 def scan_target(rows: int, cols: int) -> int:
     """Supported square scan size for a rows x cols scan (``cuda/backend.py:238-246``)."""
     return int(_abi.lib().ssb_scan_target(int(rows), int(cols)))
@@ -222,6 +242,7 @@ class HipSSBEngine:
         dt = torch.uint8 if narrow_u8 else torch.uint16
         if out is None:
             out = torch.empty((count, self.n * self.n), dtype=dt, device=self.plan.device)
+        check_out(out, "out", int(count) * self.n * self.n, dt, self.plan.device)
         _abi.call("ssb_bf_columns_u16", self.plan.handle, _abi.ptr(data), int(bf_offset), int(count),
                   1 if narrow_u8 else 0, _abi.ptr(out), self._stream())
         return out
@@ -335,6 +356,7 @@ class HipSSBEngine:
         torch = self.torch
         if not self._has_g:
             raise ValueError("no resident Fourier stack")
+        check_out(phase_out, "phase_out", self.n * self.n, torch.float32, self.plan.device)
         if self._rotation is None:
             self.cache_rotation(0.0)
         obj = torch.empty((self.n, self.n), dtype=torch.complex64, device=self.plan.device)
@@ -358,6 +380,9 @@ class HipSSBEngine:
         """Enqueue one exact-loss evaluation writing a 1-element float32 device tensor."""
         if not self._has_g:
             raise ValueError("no resident Fourier stack")
+        if loss_out is None:
+            raise ValueError("loss_out is required")
+        check_out(loss_out, "loss_out", 1, self.torch.float32, self.plan.device)
         if self._rotation is None:
             self.cache_rotation(0.0)
         _abi.call("ssb_phase_loss", self.plan.handle, float(c10), float(c12), float(phi12), None,
@@ -399,6 +424,7 @@ class HipSSBEngine:
                 raise ValueError("rotations must have one entry per trial")
         if out is None:
             out = torch.empty(t, dtype=torch.float32, device=self.plan.device)
+        check_out(out, "out", t, torch.float32, self.plan.device)
         _abi.call("ssb_batch_phase_loss", self.plan.handle, int(t), m.ctypes.data_as(_abi._dp),
                   a.ctypes.data_as(_abi._dp), r.ctypes.data_as(_abi._dp) if r is not None else None,
                   _abi.ptr(out), self._stream())
@@ -437,6 +463,9 @@ class HipSSBEngine:
         raw-data stream pass this one's stream pass must follow (two engines
         on two streams keep two reconstructions in flight)."""
         self._check_u16(data)
+        nn = self.n * self.n
+        check_out(obj_out, "obj_out", nn, self.torch.complex64, self.plan.device)
+        check_out(phase_out, "phase_out", nn, self.torch.float32, self.plan.device)
         radius = float(bf_radius) if bf_radius is not None else 0.0
         if after is None:
             _abi.call("ssb_recon_u16", self.plan.handle, _abi.ptr(data), float(c10), float(c12),

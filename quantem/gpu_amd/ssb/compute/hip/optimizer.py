@@ -321,8 +321,11 @@ def newton_refine(evaluate: Evaluate, x0, step, max_iter: int = 8, xtol=None, hm
     loss's noise: no later batch could confirm it), or after ``max_iter``
     batches.  ``polish`` then fits a least-squares quadratic to every point
     within two stencils of the best and evaluates its minimiser (one more
-    point).  The result is the best point evaluated.  ``lo``/``hi`` clip
-    every point.  It has no reference counterpart.
+    point).  The result is the best point evaluated.  Every point stays in
+    ``[lo, hi]``: the stencil keeps its symmetric shape (half-width at most
+    half the box) and its centre moves inward when it would cross a bound,
+    so the differences stay central; steps are clipped to the box.  It has
+    no reference counterpart.
     """
     x0 = np.asarray(x0, dtype=np.float64).reshape(-1)
     d = x0.size
@@ -336,36 +339,45 @@ def newton_refine(evaluate: Evaluate, x0, step, max_iter: int = 8, xtol=None, hm
     unit = _stencil(d)
     xs: list[np.ndarray] = []
     fs: list[float] = []
-    centre = np.clip(x0, lo, hi)
+    target = np.clip(x0, lo, hi)   # the point the next batch checks (the last step's end)
     tau = 1.0
-    last_f = math.inf           # loss at the centre the last step started from
+    last_f = math.inf           # loss at the point the last step started from
     pred = 0.0                  # the model's predicted decrease of that step
     edge = False                # that step reached the trust box
+    n_unit = len(unit)
     for _ in range(max_iter):
-        h = np.maximum(tau * step, hmin)
-        pts = np.clip(centre + unit * h, lo, hi)
+        # a symmetric stencil that fits the box: at most half its width per
+        # axis, its centre moved inward when target +- h would cross a bound
+        # (a stencil clipped at a bound would collapse one side onto the
+        # centre); the target itself is then evaluated as one extra point
+        h = np.minimum(np.maximum(tau * step, hmin), (hi - lo) / 2.0)
+        if np.any(h <= 0):
+            break
+        centre = np.clip(target, lo + h, hi - h)
+        pts = centre + unit * h
+        shifted = not np.array_equal(centre, target)
+        if shifted:
+            pts = np.vstack([pts, target[None, :]])
         f = np.asarray(evaluate(pts), dtype=np.float64).reshape(-1)
         xs.extend(pts)
         fs.extend(float(v) for v in f)
-        if f[0] > last_f:
+        f_target = float(f[-1] if shifted else f[0])
+        if f_target > last_f:
             # the last step went uphill: model again around the best point
-            centre = xs[int(np.argmin(fs))].copy()
+            target = xs[int(np.argmin(fs))].copy()
             last_f = min(fs)
             tau /= 4.0
             pred = 0.0
             continue
         if pred > 0.0:
-            rho = (last_f - float(f[0])) / pred
+            rho = (last_f - f_target) / pred
             if rho < 0.25:
                 tau /= 2.0
             elif rho > 0.75 and edge:
                 tau = min(2.0 * tau, 4.0)
-        last_f = float(f[0])
-        hs = (pts[1:1 + d] - pts[1 + d:1 + 2 * d]).diagonal() / 2.0     # steps after clipping
-        if np.any(hs <= 0):
-            break
-        g, H = _stencil_model(f, hs)
-        gu, Hu = g * hs, H * np.outer(hs, hs)                            # stencil units
+        last_f = f_target
+        g, H = _stencil_model(f[:n_unit], h)
+        gu, Hu = g * h, H * np.outer(h, h)                               # stencil units
         w, V = np.linalg.eigh(Hu)
         top = float(np.max(np.abs(w)))
         if not np.isfinite(top) or top == 0.0 or not np.all(np.isfinite(gu)):
@@ -377,16 +389,20 @@ def newton_refine(evaluate: Evaluate, x0, step, max_iter: int = 8, xtol=None, hm
         edge = reach >= 1.0
         if edge:
             su = su / reach          # keep the direction, end on the box
-        s = su * hs
-        if np.all(np.abs(s) < xtol):
+        new_target = np.clip(centre + su * h, lo, hi)
+        moved = new_target - target
+        if np.all(np.abs(moved) < xtol):
             break
-        sv = V.T @ su
-        pred = -float(gv @ sv + 0.5 * np.sum(w * sv * sv))
+
+        def model(u):
+            v = V.T @ u
+            return float(gv @ v + 0.5 * np.sum(w * v * v))
+        pred = model((target - centre) / h) - model((new_target - centre) / h)
         if pred < ftol:
             break
         if not edge and reach < 0.5:
-            tau = min(tau, 2.0 * float(np.max(np.abs(s) / step)))
-        centre = np.clip(centre + s, lo, hi)
+            tau = min(tau, 2.0 * float(np.max(np.abs(moved) / step)))
+        target = new_target
     if polish and fs:
         # least-squares quadratic through every point within 2 stencils of the
         # best one: many points average the loss's noise that the last

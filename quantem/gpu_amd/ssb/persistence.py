@@ -1,31 +1,39 @@
-"""Saved-result persistence for the hip SSB backend: exact reuse of a prior
-fit or reconstruction when nothing that determines it has changed.
+"""Saved SSB results: reuse a prior fit or reconstruction when nothing that
+determines it has changed.
 
-Follows the reference's storage contract (``src/quantem/gpu/ssb/_persistence.py:1-235``,
-used by ``SSB.fit``/``SSB.reconstruct`` at ``src/quantem/gpu/ssb/workflow.py:1043-1170``):
+On-disk format (the interchange format of the reference's saved results,
+``src/quantem/gpu/ssb/_persistence.py:1-235``, used by ``SSB.fit`` and
+``SSB.reconstruct`` at ``src/quantem/gpu/ssb/workflow.py:933-996``,
+``1043-1170``), kept so saved files read the same way:
 
-* ``result_paths``: ``save_to`` is a directory (``ssb-<operation>.npz`` and
-  ``.json`` inside it) or an ``.npz`` path (``.json`` beside it);
-* the array file holds ``object_wave`` only; the JSON holds the schema
-  number, the generation time, git provenance, the input description, the
-  signature and every other result field;
-* writes are atomic (temporary file + rename);
-* ``load_result`` returns ``None`` on any mismatch -- a different schema or
-  signature, a missing or unreadable file -- so the caller recomputes.
+* ``save_to`` names a directory (``ssb-<operation>.npz`` and
+  ``ssb-<operation>.json`` inside it) or an ``.npz`` file (its ``.json``
+  beside it);
+* the ``.npz`` holds one array, ``object_wave``;
+* the ``.json`` holds ``schema`` (1), ``generated_at_utc``, ``git``,
+  ``input``, ``signature`` and ``result`` (every other ``SSBResult`` field).
 
-The signature names everything that determines the result: this package's
-own sources (the HIP kernels included), the data source and shape, the
-instrument parameters, the backend settings and the operation's settings.
-Saved arrays are loaded with ``allow_pickle=False``.
+This module's own design: a ``ResultStore`` per (location, operation)
+publishes both files by writing complete bytes to an exclusive temporary
+file in the target directory, flushing it to disk and renaming it over the
+target, so a reader never sees a partial file.  Metadata is normalised by
+one JSON round trip with sorted keys (numpy values, paths and tuples
+encoded on the way), so a signature compares equal exactly when its
+canonical JSON text does.  The software identity is a digest over per-file
+digests of this package's sources (kernels included); the git state comes
+from one ``git status --porcelain=v2 --branch`` call.  Any problem reading
+a saved pair (missing, unreadable, a different schema or signature) means
+"no reusable result": the caller recomputes.  Arrays load with
+``allow_pickle=False``.
 """
 from __future__ import annotations
 
 import hashlib
+import io
 import json
 import os
 import subprocess
-import tempfile
-from dataclasses import fields
+from dataclasses import dataclass, fields
 from datetime import datetime, timezone
 from pathlib import Path
 from typing import Callable, Literal
@@ -35,171 +43,254 @@ import numpy as np
 from .types import SSBResult
 
 SCHEMA = 1
-_ARTIFACT_FIELDS = {"object_wave", "reused", "saved_path", "metadata"}
-_PACKAGE = Path(__file__).resolve().parents[1]          # quantem/gpu_amd
-_SOURCE_SUFFIXES = {".py", ".hip", ".h"}
+Operation = Literal["fit", "reconstruct"]
+
+_PACKAGE_ROOT = Path(__file__).resolve().parents[1]            # quantem/gpu_amd
+_REPO_ROOT = _PACKAGE_ROOT.parents[1]
+_HASHED_SUFFIXES = (".py", ".hip", ".h")
+# fields of SSBResult that describe the stored artefact rather than the result
+_NOT_IN_METADATA = frozenset({"object_wave", "reused", "saved_path", "metadata"})
+# JSON arrays that SSBResult holds as tuples
+_TUPLE_FIELDS = ("scan_sampling_A", "bf_center")
 
 
-def software_signature() -> dict[str, object]:
-    """Identify the implementation: a digest of the package's sources."""
-    digest = hashlib.sha256()
-    roots = [_PACKAGE, _PACKAGE.parents[1] / "include"]
-    for root in roots:
-        if not root.is_dir():
-            continue
-        for path in sorted(root.rglob("*")):
-            if path.is_file() and path.suffix in _SOURCE_SUFFIXES and "__pycache__" not in path.parts:
-                digest.update(str(path.relative_to(root.parent)).encode("utf-8"))
-                digest.update(path.read_bytes())
-    return {"package": "quantem.gpu_amd", "source_sha256": digest.hexdigest()}
-
-
-def _git_provenance() -> dict[str, object] | None:
-    repository = _PACKAGE.parents[1]
-
-    def git(*arguments: str) -> str | None:
-        try:
-            done = subprocess.run(["git", "-C", str(repository), *arguments], capture_output=True, text=True,
-                                  check=False, timeout=10)
-        except (OSError, subprocess.SubprocessError):
-            return None
-        return done.stdout.strip() if done.returncode == 0 else None
-
-    commit = git("rev-parse", "HEAD")
-    if commit is None:
-        return None
-    status = git("status", "--short", "--untracked-files=all")
-    return {"commit": commit, "branch": git("branch", "--show-current"), "dirty": bool(status)}
-
-
-def json_value(value: object) -> object:
-    """Deterministic JSON-safe form of one value (sorted keys; arrays as lists)."""
-    if isinstance(value, dict):
-        return {str(k): json_value(v) for k, v in sorted(value.items(), key=lambda kv: str(kv[0]))}
-    if isinstance(value, (list, tuple)):
-        return [json_value(v) for v in value]
-    if isinstance(value, Path):
-        return str(value)
+# --------------------------------------------------------------- encoding
+def _encode_extra(value):
+    """``json.dumps`` hook for the non-JSON types metadata may hold."""
     if isinstance(value, np.ndarray):
         return value.tolist()
     if isinstance(value, np.generic):
         return value.item()
-    if value is None or isinstance(value, (bool, int, float, str)):
-        return value
+    if isinstance(value, os.PathLike):
+        return os.fspath(value)
     raise TypeError(f"Cannot save SSB metadata value of type {type(value).__name__}.")
 
 
+def _stringify_keys(value):
+    """Dict keys as strings (JSON's rule), recursively, before sorting."""
+    if isinstance(value, dict):
+        return {str(k): _stringify_keys(v) for k, v in value.items()}
+    if isinstance(value, (list, tuple)):
+        return [_stringify_keys(v) for v in value]
+    return value
+
+
+def canonical_text(value: object) -> str:
+    """Canonical JSON text of ``value``: sorted keys, no insignificant space."""
+    return json.dumps(_stringify_keys(value), sort_keys=True, separators=(",", ":"), default=_encode_extra)
+
+
+def json_value(value: object) -> object:
+    """The JSON-safe form of ``value`` (what saving and reloading gives):
+    dicts with sorted string keys, tuples and arrays as lists, numpy
+    scalars as Python numbers, paths as strings.  Any other type raises
+    ``TypeError``."""
+    return json.loads(canonical_text(value))
+
+
+# ------------------------------------------------------------- provenance
+def _file_digest(path: Path) -> str:
+    h = hashlib.sha256()
+    with path.open("rb") as fh:
+        for block in iter(lambda: fh.read(1 << 20), b""):
+            h.update(block)
+    return h.hexdigest()
+
+
+def software_signature() -> dict[str, object]:
+    """Identity of the implementation: one digest over the sorted
+    ``relative path  file digest`` lines of the package's Python and HIP
+    sources and of the public C headers."""
+    lines = []
+    for root in (_PACKAGE_ROOT, _REPO_ROOT / "include"):
+        if not root.is_dir():
+            continue
+        for path in root.rglob("*"):
+            if path.suffix in _HASHED_SUFFIXES and path.is_file() and "__pycache__" not in path.parts:
+                lines.append(f"{path.relative_to(_REPO_ROOT).as_posix()}  {_file_digest(path)}")
+    manifest = "\n".join(sorted(lines)).encode("utf-8")
+    return {"package": "quantem.gpu_amd", "source_sha256": hashlib.sha256(manifest).hexdigest()}
+
+
+def _git_state() -> dict[str, object] | None:
+    """{commit, branch, dirty} of the checkout this package runs from, or
+    ``None`` outside a git checkout (or without git)."""
+    try:
+        done = subprocess.run(["git", "-C", str(_REPO_ROOT), "status", "--porcelain=v2", "--branch",
+                               "--untracked-files=all"], capture_output=True, text=True, timeout=10)
+    except (OSError, subprocess.SubprocessError):
+        return None
+    if done.returncode != 0:
+        return None
+    header: dict[str, str] = {}
+    dirty = False
+    for line in done.stdout.splitlines():
+        if line.startswith("# branch."):
+            key, _, val = line[len("# branch."):].partition(" ")
+            header[key] = val.strip()
+        elif line.strip():
+            dirty = True
+    commit = header.get("oid")
+    if not commit or commit == "(initial)":
+        return None
+    branch = header.get("head")
+    return {"commit": commit, "branch": None if branch in (None, "(detached)") else branch, "dirty": dirty}
+
+
 def path_signature(path: str | Path) -> dict[str, object]:
-    """Describe a source path (sizes, times, inodes) without reading its payload."""
-    resolved = Path(path).expanduser().resolve()
-    if not resolved.exists():
-        return {"path": str(resolved), "missing": True}
-    paths = [resolved]
-    if resolved.is_dir():
-        paths.extend(sorted(p for p in resolved.rglob("*") if p.is_file()))
-    files = []
-    for item in paths:
-        st = item.stat()
-        files.append({"path": str(item), "size": int(st.st_size), "mtime_ns": int(st.st_mtime_ns),
-                      "ctime_ns": int(st.st_ctime_ns), "device": int(st.st_dev), "inode": int(st.st_ino)})
-    return {"path": str(resolved), "files": files}
+    """What identifies a detector source without reading it: for the path
+    (and, for a directory, every file below it, in sorted order) its size,
+    modification and change times, device and inode."""
+    root = Path(path).expanduser().resolve()
+    if not root.exists():
+        return {"path": str(root), "missing": True}
+    members = [root]
+    if root.is_dir():
+        for dirpath, dirnames, filenames in os.walk(root):
+            dirnames.sort()
+            members.extend(Path(dirpath) / name for name in sorted(filenames))
+        members = [root] + sorted(members[1:])
+
+    def describe(p: Path) -> dict[str, object]:
+        st = p.stat()
+        return {"path": str(p), "size": st.st_size, "mtime_ns": st.st_mtime_ns, "ctime_ns": st.st_ctime_ns,
+                "device": st.st_dev, "inode": st.st_ino}
+    return {"path": str(root), "files": [describe(p) for p in members]}
 
 
-def result_paths(save_to: str | Path, operation: Literal["fit", "reconstruct"]) -> tuple[Path, Path]:
-    """(array path, metadata path) of one saved result."""
-    destination = Path(save_to).expanduser().resolve()
-    if destination.suffix:
-        if destination.suffix.lower() != ".npz":
-            raise ValueError("save_to must be a directory or an .npz file, for example "
-                             "save_to='results/ssb' or save_to='results/ssb-fit.npz'.")
-        return destination, destination.with_suffix(".json")
-    return destination / f"ssb-{operation}.npz", destination / f"ssb-{operation}.json"
+# ----------------------------------------------------------------- storage
+def result_paths(save_to: str | Path, operation: Operation) -> tuple[Path, Path]:
+    """(array file, metadata file) of one saved result."""
+    target = Path(save_to).expanduser().resolve()
+    if not target.suffix:
+        stem = target / f"ssb-{operation}"
+        return stem.with_suffix(".npz"), stem.with_suffix(".json")
+    if target.suffix.lower() == ".npz":
+        return target, target.with_suffix(".json")
+    raise ValueError("save_to must be a directory or an .npz file, for example "
+                     "save_to='results/ssb' or save_to='results/ssb-fit.npz'.")
 
 
-def _atomic_write(path: Path, write: Callable, mode: str) -> None:
-    path.parent.mkdir(parents=True, exist_ok=True)
-    kwargs = {"encoding": "utf-8"} if "b" not in mode else {}
-    tmp = tempfile.NamedTemporaryFile(mode=mode, prefix=f".{path.name}.", dir=path.parent, delete=False, **kwargs)
-    tmp_path = Path(tmp.name)
-    try:
-        with tmp:
-            write(tmp)
-        os.replace(tmp_path, path)
-    finally:
-        tmp_path.unlink(missing_ok=True)
+def _publish(target: Path, payload: bytes) -> None:
+    """Replace ``target`` with ``payload`` in one rename: the bytes go to an
+    exclusive temporary file in the same directory, are flushed to disk,
+    and the file is renamed over the target."""
+    target.parent.mkdir(parents=True, exist_ok=True)
+    for attempt in range(100):
+        tmp = target.parent / f".{target.name}.{os.getpid()}.{attempt}.part"
+        try:
+            fd = os.open(tmp, os.O_WRONLY | os.O_CREAT | os.O_EXCL, 0o644)
+        except FileExistsError:
+            continue
+        try:
+            with os.fdopen(fd, "wb") as fh:
+                fh.write(payload)
+                fh.flush()
+                os.fsync(fh.fileno())
+            os.replace(tmp, target)
+        finally:
+            if tmp.exists():
+                tmp.unlink()
+        return
+    raise OSError(f"no free temporary name beside {target}")
 
 
-def _host_array(value) -> np.ndarray:
-    if hasattr(value, "detach") and hasattr(value, "cpu"):      # torch tensor
-        return value.detach().cpu().numpy()
-    return np.asarray(value)
+def _to_host(array) -> np.ndarray:
+    if hasattr(array, "detach"):                      # torch tensor, possibly on the GPU
+        array = array.detach().cpu().numpy()
+    return np.asarray(array)
 
 
-def save_result(result: SSBResult, *, paths: tuple[Path, Path], signature: dict[str, object],
-                input_metadata: dict[str, object]) -> SSBResult:
-    """Persist one result (atomic .npz + .json) and attach its provenance."""
-    array_path, metadata_path = paths
-    result_fields = {f.name: getattr(result, f.name) for f in fields(result) if f.name not in _ARTIFACT_FIELDS}
-    metadata = {
-        "schema": SCHEMA,
-        "generated_at_utc": datetime.now(timezone.utc).isoformat(),
-        "git": _git_provenance(),
-        "input": json_value(input_metadata),
-        "signature": json_value(signature),
-        "result": json_value(result_fields),
-    }
-    wave = _host_array(result.object_wave)
-    _atomic_write(array_path, lambda f: np.savez(f, object_wave=wave), "wb")
-    _atomic_write(metadata_path, lambda f: (json.dump(metadata, f, indent=2), f.write("\n")), "w")
-    result.reused = False
-    result.saved_path = array_path
-    result.metadata = metadata
-    return result
+@dataclass(frozen=True)
+class ResultStore:
+    """The saved result of one operation at one location."""
 
+    array_path: Path
+    metadata_path: Path
 
-def load_result(*, paths: tuple[Path, Path], signature: dict[str, object], device=None) -> SSBResult | None:
-    """The saved result whose schema and signature match exactly, else ``None``.
+    @classmethod
+    def at(cls, save_to: str | Path, operation: Operation) -> "ResultStore":
+        return cls(*result_paths(save_to, operation))
 
-    ``device``: a torch device to place ``object_wave`` on (default: numpy)."""
-    array_path, metadata_path = paths
-    if not array_path.is_file() or not metadata_path.is_file():
-        return None
-    try:
-        metadata = json.loads(metadata_path.read_text(encoding="utf-8"))
-        if metadata.get("schema") != SCHEMA or metadata.get("signature") != json_value(signature):
+    @property
+    def paths(self) -> tuple[Path, Path]:
+        return self.array_path, self.metadata_path
+
+    def exists(self) -> bool:
+        return self.array_path.exists() or self.metadata_path.exists()
+
+    def save(self, result: SSBResult, signature: dict, input_metadata: dict) -> SSBResult:
+        """Write both files and attach the provenance to ``result``."""
+        payload = {f.name: getattr(result, f.name) for f in fields(result) if f.name not in _NOT_IN_METADATA}
+        metadata = {
+            "schema": SCHEMA,
+            "generated_at_utc": datetime.now(timezone.utc).isoformat(),
+            "git": _git_state(),
+            "input": json_value(input_metadata),
+            "signature": json_value(signature),
+            "result": json_value(payload),
+        }
+        buf = io.BytesIO()
+        np.savez(buf, object_wave=_to_host(result.object_wave))
+        _publish(self.array_path, buf.getvalue())
+        _publish(self.metadata_path, (json.dumps(metadata, indent=2) + "\n").encode("utf-8"))
+        result.reused = False
+        result.saved_path = self.array_path
+        result.metadata = metadata
+        return result
+
+    def load(self, signature: dict, device=None) -> SSBResult | None:
+        """The stored result if its schema and signature match ``signature``
+        exactly; ``None`` otherwise.  ``device``: torch device for
+        ``object_wave`` (default: a NumPy array)."""
+        try:
+            metadata = json.loads(self.metadata_path.read_bytes())
+            if not isinstance(metadata, dict) or metadata.get("schema") != SCHEMA:
+                return None
+            if canonical_text(metadata.get("signature")) != canonical_text(signature):
+                return None
+            stored = dict(metadata["result"])
+            with np.load(self.array_path, allow_pickle=False) as arrays:
+                wave = np.array(arrays["object_wave"])
+        except (OSError, ValueError, KeyError, TypeError):
             return None
-        saved = dict(metadata["result"])
-        with np.load(array_path, allow_pickle=False) as arrays:
-            wave = np.array(arrays["object_wave"])
-    except (OSError, ValueError, KeyError, TypeError):
-        return None
-    if device is not None:
-        import torch
-        wave = torch.from_numpy(wave).to(device)
-    for name in ("scan_sampling_A", "bf_center"):
-        if isinstance(saved.get(name), list):
-            saved[name] = tuple(saved[name])
-    known = {f.name for f in fields(SSBResult)}
-    saved = {k: v for k, v in saved.items() if k in known}
-    return SSBResult(object_wave=wave, reused=True, saved_path=array_path, metadata=metadata, **saved)
+        known = {f.name for f in fields(SSBResult)} - _NOT_IN_METADATA
+        kwargs = {k: v for k, v in stored.items() if k in known}
+        for name in _TUPLE_FIELDS:
+            if isinstance(kwargs.get(name), list):
+                kwargs[name] = tuple(kwargs[name])
+        if device is not None:
+            import torch
+            wave = torch.from_numpy(wave).to(device)
+        return SSBResult(object_wave=wave, reused=True, saved_path=self.array_path, metadata=metadata, **kwargs)
 
 
-def reuse_or_run(operation: Literal["fit", "reconstruct"], save_to: str | Path, signature: dict[str, object],
-                 run: Callable[[], SSBResult], *, input_metadata: dict[str, object], force: bool = False,
-                 device=None, verbose: bool = False) -> SSBResult:
-    """The reuse rule of ``SSB.fit``/``SSB.reconstruct`` (``workflow.py:1069-1103``):
-    load a matching saved result unless ``force``; otherwise run and save."""
-    paths = result_paths(save_to, operation)
+def save_result(result: SSBResult, *, paths: tuple[Path, Path], signature: dict,
+                input_metadata: dict) -> SSBResult:
+    """Persist ``result`` at ``paths`` (see ``result_paths``)."""
+    return ResultStore(*paths).save(result, signature, input_metadata)
+
+
+def load_result(*, paths: tuple[Path, Path], signature: dict, device=None) -> SSBResult | None:
+    """The result saved at ``paths`` if it matches ``signature``, else ``None``."""
+    return ResultStore(*paths).load(signature, device)
+
+
+def reuse_or_run(operation: Operation, save_to: str | Path, signature: dict, run: Callable[[], SSBResult], *,
+                 input_metadata: dict, force: bool = False, device=None, verbose: bool = False) -> SSBResult:
+    """Reuse rule of ``SSB.fit``/``SSB.reconstruct``
+    (``src/quantem/gpu/ssb/workflow.py:1069-1103``): a matching saved result
+    is returned unless ``force``; otherwise ``run()`` and save."""
+    store = ResultStore.at(save_to, operation)
     if not force:
-        reused = load_result(paths=paths, signature=signature, device=device)
-        if reused is not None:
+        hit = store.load(signature, device)
+        if hit is not None:
             if verbose:
-                print(f"Matching SSB result found; loading {paths[0]}")
-            return reused
-        if verbose and any(p.exists() for p in paths):
+                print(f"Matching SSB result found; loading {store.array_path}")
+            return hit
+        if verbose and store.exists():
             print(f"Saved SSB settings changed; running {operation} end to end")
-    result = run()
-    result = save_result(result, paths=paths, signature=signature, input_metadata=input_metadata)
+    result = store.save(run(), signature, input_metadata)
     if verbose:
-        print(f"SSB result saved to {paths[0]}")
+        print(f"SSB result saved to {store.array_path}")
     return result

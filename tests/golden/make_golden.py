@@ -142,6 +142,34 @@ def fixtures() -> dict:
     }
 
 
+FIXTURE_H5 = Path("/root/reference/src/quantem/gpu/swift/Tests/Native4DSTEMIOTests/Fixtures/fixture_data_000001.h5")
+CHUNK_OUT = Path(__file__).resolve().parent / "bslz4_fixture_chunk.bin"
+
+
+def bslz4_fixture_chunk() -> dict:
+    """The one bitshuffle-LZ4 chunk of the reference's native-IO fixture
+    (``src/quantem/gpu/swift/Tests/Native4DSTEMIOTests/Fixtures/fixture_data_000001.h5``):
+    one 64 x 64 uint16 frame whose decoded values are ``pixel % 257``
+    (``Native4DSTEMIOTests.swift:1224-1226``, frame read back at ``:1286``;
+    the bad-pixel zeroing there happens after the decode).  The chunk is
+    located by a plain byte search for its header -- total bytes 8192 and
+    block bytes 8192 as big-endian u64/u32 -- and its length is the header
+    plus one 4-byte block size plus that block; no HDF5 library is used."""
+    import struct
+    raw = FIXTURE_H5.read_bytes()
+    header = struct.pack(">QI", 64 * 64 * 2, 8192)
+    at = raw.find(header)
+    if at < 0 or raw.find(header, at + 1) >= 0:
+        raise RuntimeError("expected exactly one chunk header in the fixture")
+    (block,) = struct.unpack(">I", raw[at + 12:at + 16])
+    chunk = raw[at:at + 16 + block]
+    CHUNK_OUT.write_bytes(chunk)
+    return {"source": "src/quantem/gpu/swift/Tests/Native4DSTEMIOTests/Fixtures/fixture_data_000001.h5",
+            "file": CHUNK_OUT.name, "offset": at, "bytes": len(chunk), "lz4_block_bytes": block,
+            "shape": [64, 64], "dtype": "uint16", "expected": "pixel % 257 (raster order)",
+            "expected_from": "Native4DSTEMIOTests.swift:1224-1226, 1286"}
+
+
 def main() -> None:
     doc = {
         "extract_gqk_arange": extract_gqk_arange(),
@@ -149,6 +177,7 @@ def main() -> None:
         "zero_input": zero_input_known_answer(),
         "wavelength": wavelengths(),
         "fixtures": fixtures(),
+        "bslz4_fixture_chunk": bslz4_fixture_chunk(),
     }
     OUT.write_text(json.dumps(doc, indent=1) + "\n")
     print(f"wrote {OUT} ({OUT.stat().st_size} bytes)")

@@ -73,3 +73,17 @@ def test_library_encoder_output_decodes_with_the_oracle(n, kind):
     assert chunk[:12] == B.encode_chunk(v)[:12]
     if kind == "zeros":
         assert len(chunk) < n // 20
+
+
+def test_reference_fixture_chunk_decodes_to_its_known_frame(golden):
+    """The chunk of the reference's native-IO fixture file decodes to
+    ``pixel % 257`` on 64 x 64 (``Native4DSTEMIOTests.swift:1224-1226``)."""
+    from pathlib import Path
+    meta = golden["bslz4_fixture_chunk"]
+    chunk = (Path(__file__).resolve().parent / "golden" / meta["file"]).read_bytes()
+    assert len(chunk) == meta["bytes"] == 778
+    out = B.decode_chunk(chunk)
+    np.testing.assert_array_equal(out, (np.arange(64 * 64) % 257).astype(np.uint16))
+    # a chunk written by the reference's encoder, not by ours: the oracle's
+    # encoder need not produce the same bytes, only the same frame
+    assert np.array_equal(B.decode_chunk(B.encode_chunk(out)), out)

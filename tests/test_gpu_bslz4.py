@@ -73,3 +73,20 @@ def test_decoded_dataset_reconstructs_like_the_raw_one(cuda):
     eng.recon_u16(raw, *C.MAKE_ENGINE_ABERRATIONS, 0.0, 0.0, 3.0, phase_out=a)
     eng.recon_u16(dec, *C.MAKE_ENGINE_ABERRATIONS, 0.0, 0.0, 3.0, phase_out=b)
     assert cuda.equal(a, b)
+
+
+def test_reference_fixture_chunk_decodes_on_the_gpu(cuda, golden):
+    """The chunk of the reference's native-IO fixture file, decoded on the
+    GPU (alone and as the middle of three chunks), equals ``pixel % 257``."""
+    from pathlib import Path
+
+    from quantem.gpu_amd.ssb.compute.hip.bslz4 import decode_u16
+    meta = golden["bslz4_fixture_chunk"]
+    chunk = (Path(__file__).resolve().parent / "golden" / meta["file"]).read_bytes()
+    expect = (np.arange(64 * 64) % 257).astype(np.uint16).reshape(1, 64, 64)
+    np.testing.assert_array_equal(decode_u16([chunk], (1, 64, 64)).cpu().numpy(), expect)
+    other = _frames("poisson", 2, (64, 64), 7)
+    chunks = [B.encode_chunk(other[0]), chunk, B.encode_chunk(other[1])]
+    out = decode_u16(chunks, (1, 64, 64)).cpu().numpy()
+    np.testing.assert_array_equal(out[1], expect[0])
+    np.testing.assert_array_equal(out[[0, 2]], other)

@@ -73,3 +73,18 @@ def test_bin_frames_checks_its_input_before_any_gpu_call():
         bin_frames(torch.zeros((2, 8, 8), dtype=torch.uint16), 2)            # host tensor
     with pytest.raises(TypeError):
         bin_frames(np.zeros((2, 8, 8), dtype=np.uint16), 2)
+
+
+def test_output_checks_reject_host_short_and_mistyped_tensors():
+    """Caller-supplied outputs are checked before any native call: a host
+    tensor, a wrong dtype, a short or non-contiguous tensor raise ValueError."""
+    torch = pytest.importorskip("torch")
+    from quantem.gpu_amd.ssb.compute.hip.engine import check_out
+    dev = "cuda:0"
+    check_out(None, "out", 4, torch.float32, dev)
+    with pytest.raises(ValueError, match="must live on"):
+        check_out(torch.zeros(4), "out", 4, torch.float32, dev)
+    with pytest.raises(ValueError, match="dtype"):
+        check_out(torch.zeros(4, dtype=torch.float64), "out", 4, torch.float32, dev)
+    with pytest.raises(ValueError, match="torch tensor"):
+        check_out(np.zeros(4, np.float32), "out", 4, torch.float32, dev)

@@ -176,3 +176,34 @@ def test_newton_refine_stops_on_a_small_predicted_decrease():
     x, fx, n = opt.newton_refine(evaluate, np.zeros(3), 0.1, max_iter=8, ftol=1e-12, polish=True)
     np.testing.assert_allclose(x, [0.1, -0.2, 0.3], atol=1e-9)
     assert calls == [10, 10] and n == 20
+
+
+def test_newton_refine_keeps_a_symmetric_stencil_at_a_bound():
+    # minimum just inside the upper bound of x0 (rotation clipped to +-5 deg):
+    # a stencil of half-width 0.25 around 4.95 would cross 5.  Every point
+    # stays inside the box and every batch is a central stencil (the axis
+    # pairs straddle their centre at equal distances), so the model is the
+    # quadratic itself and one step lands on the minimum.
+    A = np.array([[2.0, 0.3], [0.3, 1.0]])
+    c = np.array([4.9, 1.0])
+    batches = []
+
+    def evaluate(points):
+        batches.append(np.array(points, copy=True))
+        return _quadratic(A, c, 0.5)(points)
+    lo, hi = np.array([-5.0, -5.0]), np.array([5.0, 5.0])
+    x, fx, _ = opt.newton_refine(evaluate, [4.95, 0.0], [0.25, 0.25], max_iter=4, xtol=1e-9, hmin=1e-6,
+                                 lo=lo, hi=hi)
+    np.testing.assert_allclose(x, c, atol=1e-9)
+    for pts in batches:
+        assert np.all(pts >= lo) and np.all(pts <= hi)
+        if len(pts) == len(opt._stencil(2)):
+            centre, plus, minus = pts[0], pts[1:3], pts[3:5]
+            np.testing.assert_allclose(plus - centre, centre - minus, atol=1e-12)
+
+
+def test_newton_refine_reaches_a_minimum_outside_the_box_at_the_bound():
+    # constrained minimum on the bound: the result is the bound itself
+    x, fx, _ = opt.newton_refine(_quadratic(np.eye(1), [6.0]), [4.0], [0.5], max_iter=10, lo=np.array([-5.0]),
+                                 hi=np.array([5.0]))
+    assert x[0] == pytest.approx(5.0, abs=1e-9)

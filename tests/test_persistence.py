@@ -89,3 +89,33 @@ def test_path_signature_describes_without_reading(tmp_path):
     assert P.path_signature(tmp_path / "missing")["missing"] is True
     s1 = P.software_signature()
     assert s1 == P.software_signature() and len(s1["source_sha256"]) == 64
+
+
+def test_publish_leaves_only_the_target(tmp_path):
+    store = P.ResultStore.at(tmp_path / "deep" / "dir", "reconstruct")
+    store.save(_result(3), {"k": 1}, {})
+    store.save(_result(4), {"k": 1}, {})                     # replaced in place
+    names = sorted(p.name for p in (tmp_path / "deep" / "dir").iterdir())
+    assert names == ["ssb-reconstruct.json", "ssb-reconstruct.npz"]
+    back = store.load({"k": 1})
+    np.testing.assert_array_equal(back.object_wave, _result(4).object_wave)
+
+
+def test_signature_comparison_is_canonical(tmp_path):
+    # key order, tuples vs lists and numpy scalars do not change a signature
+    store = P.ResultStore.at(tmp_path, "fit")
+    store.save(_result(), {"b": (1, 2), "a": np.float64(0.5)}, {})
+    assert store.load({"a": 0.5, "b": [1, 2]}) is not None
+    assert store.load({"a": 0.5, "b": [1, 3]}) is None
+
+
+def test_git_state_and_directory_signature(tmp_path):
+    state = P._git_state()
+    if state is not None:
+        assert len(state["commit"]) == 40 and isinstance(state["dirty"], bool)
+    (tmp_path / "b").mkdir()
+    (tmp_path / "b" / "z.bin").write_bytes(b"1")
+    (tmp_path / "a.bin").write_bytes(b"22")
+    sig = P.path_signature(tmp_path)
+    paths = [f["path"] for f in sig["files"]]
+    assert paths[0] == str(tmp_path) and paths[1:] == sorted(paths[1:]) and len(paths) == 3
