@@ -33,6 +33,26 @@ extern "C" {
 typedef struct ssb_plan ssb_plan;
 
 /*
+ * Element types of raw detector frames for the *_typed entry points.  The
+ * reference keeps the caller's dtype (cuda/backend.py:195-201) and sums
+ * integer frames in uint64 (signed values sign-extended, modulo 2^64) and
+ * float frames in float64 (cuda/backend.py:430-441); here float frames are
+ * summed in double-float (pairs of float32, error-free transformations).
+ * The Fourier stack converts each sample to float32 (astype(complex64),
+ * cuda/backend.py:570-571).
+ */
+#define QSSB_DTYPE_U8 1
+#define QSSB_DTYPE_U16 2
+#define QSSB_DTYPE_U32 3
+#define QSSB_DTYPE_I8 4
+#define QSSB_DTYPE_I16 5
+#define QSSB_DTYPE_I32 6
+#define QSSB_DTYPE_F32 7
+
+/* Bytes per element of a QSSB_DTYPE_* code; 0 for an unknown code. */
+int ssb_dtype_size(int dtype);
+
+/*
  * Probe disk centre and radius (pixels) from the plan's last mean DP
  * (ssb_mean_dp_u16): detect_bf_radius of qg/detector/compute/cuda/probe.py:12-123
  * (threshold_ratio 0.1 there).  The SSB backend derives
@@ -61,6 +81,12 @@ int ssb_bf_columns_u16(ssb_plan* plan, const uint16_t* data, int bf_offset, int 
                        void* stream);
 
 /*
+ * ssb_bf_columns_u16 for uint8 frames (out is uint8); the reference's
+ * companion export accepts uint8 or uint16 counts only (cuda/engine.py:598-602).
+ */
+int ssb_bf_columns_u8(ssb_plan* plan, const uint8_t* data, int bf_offset, int count, void* out, void* stream);
+
+/*
  * Learned-initializer features of the resident Hermitian stack: for the 8
  * scan-frequency grid points (qrow_host[i], qcol_host[i]), the 32 x 32
  * detector window of G around the rounded BF centre as |G|/max|G| and the
@@ -87,6 +113,15 @@ int ssb_scan_target(int H, int W);
  */
 int ssb_normalize_scan_u16(const uint16_t* src, int H, int W, int det_rows, int det_cols, uint16_t* dst, int T,
                            void* stream);
+
+/*
+ * ssb_normalize_scan_u16 for frames of any QSSB_DTYPE_* element type: the
+ * pad frame is the cropped scan's int64 sum / count truncated toward zero
+ * (integers) or its float64 sum / count rounded to float32 (float32 frames,
+ * summed here in double-float), as cuda/backend.py:253-266.
+ */
+int ssb_normalize_scan_typed(const void* src, int dtype, int H, int W, int det_rows, int det_cols, void* dst,
+                             int T, void* stream);
 
 /* Library version (major*10000 + minor*100 + patch). */
 int ssb_version(void);
@@ -134,6 +169,25 @@ int ssb_mean_dp_u16(ssb_plan* plan, const uint16_t* data, int64_t frames, uint64
                     float* mean_out, void* stream);
 
 /*
+ * ssb_mean_dp_u16 for frames of any QSSB_DTYPE_* element type: exact
+ * uint64 sums for integer types (sums_out, may be NULL), double-float sums
+ * for float32 (sums_out must be NULL), mean = f32(sum)/f32(frames) times
+ * the detector gain when one is set (cuda/backend.py:430-443).
+ */
+int ssb_mean_dp_typed(ssb_plan* plan, const void* data, int dtype, int64_t frames, uint64_t* sums_out,
+                      float* mean_out, void* stream);
+
+/*
+ * Per-pixel detector gain (float32 [det_rows][det_cols], device; NULL
+ * clears it), copied into the plan.  Applied as the reference applies
+ * detector_gain: to the mean DP before the bright-field mask and the probe
+ * radius (cuda/backend.py:220-226, 281-282, 442-443) and to every BF sample
+ * before the scan FFT (cuda/backend.py:574-576).  Takes effect from the next
+ * mean DP / Fourier stack / reconstruction.
+ */
+int ssb_set_detector_gain(ssb_plan* plan, const float* gain, void* stream);
+
+/*
  * Bright-field selection on the device from the plan's mean DP
  * (cuda/backend.py:448-518).  bf_radius <= 0 selects the automatic
  * mean+std radius.  No host synchronisation; see ssb_get_bf.
@@ -170,6 +224,9 @@ int ssb_get_geometry(ssb_plan* plan, float* kx_out, float* ky_out, float* apertu
  * scan axes, Hermitian half-plane kept in the plan.
  */
 int ssb_prepare_g_u16(ssb_plan* plan, const uint16_t* data, void* stream);
+
+/* ssb_prepare_g_u16 for frames of any QSSB_DTYPE_* element type (cuda/backend.py:520-589). */
+int ssb_prepare_g_typed(ssb_plan* plan, const void* data, int dtype, void* stream);
 
 /*
  * Export the resident stack in the reference layout G[b][q_row][q_col <= N/2]
@@ -243,6 +300,11 @@ int ssb_batch_phase_loss(ssb_plan* plan, int n_trials, const double* mags_host, 
 int ssb_recon_u16(ssb_plan* plan, const uint16_t* data, double c10, double c12, double phi12,
                   double rotation_rad, float threshold, float bf_radius, void* obj_out, float* phase_out,
                   void* stream);
+
+/* ssb_recon_u16 for frames of any QSSB_DTYPE_* element type (cuda/backend.py:168-851). */
+int ssb_recon_typed(ssb_plan* plan, const void* data, int dtype, double c10, double c12, double phi12,
+                    double rotation_rad, float threshold, float bf_radius, void* obj_out, float* phase_out,
+                    void* stream);
 
 /*
  * ssb_recon_u16 with its raw-data stream pass ordered after the stream pass

@@ -116,16 +116,26 @@ def normalize_scan(data: np.ndarray) -> np.ndarray:
 # (cuda/backend.py:420-518)
 # --------------------------------------------------------------------------
 
-def mean_dp(data: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
-    """Return (uint64 per-pixel sums, float32 mean DP).
+def mean_dp(data: np.ndarray, detector_gain: np.ndarray | None = None) -> tuple[np.ndarray, np.ndarray]:
+    """Return (per-pixel sums, float32 mean DP).
 
-    ``cuda/backend.py:432-441``: ``frames.sum(axis=0, dtype=uint64)
-    .astype(float32) / N`` -- the quotient is a float32 array divided by an
-    integer, i.e. one correctly rounded float32 division.
+    ``cuda/backend.py:432-443``: ``frames.sum(axis=0, dtype=uint64 for
+    integer samples, float64 otherwise).astype(float32) / N`` -- the quotient
+    is a float32 array divided by an integer, i.e. one correctly rounded
+    float32 division -- then ``* detector_gain`` (float32) when given.
+    Signed integers enter the uint64 sum sign-extended, modulo 2^64 (the
+    int64 sum's two's-complement bits).
     """
     frames = data.reshape(-1, data.shape[-2], data.shape[-1])
-    sums = frames.sum(axis=0, dtype=np.uint64)
+    if np.issubdtype(frames.dtype, np.unsignedinteger):
+        sums = frames.sum(axis=0, dtype=np.uint64)
+    elif np.issubdtype(frames.dtype, np.integer):
+        sums = frames.sum(axis=0, dtype=np.int64).view(np.uint64)
+    else:
+        sums = frames.sum(axis=0, dtype=np.float64)
     mean = sums.astype(np.float32) / np.float32(frames.shape[0])
+    if detector_gain is not None:
+        mean = mean * np.asarray(detector_gain, dtype=np.float32)
     return sums, mean.astype(np.float32)
 
 
@@ -266,18 +276,21 @@ def detect_bf_radius(mean: np.ndarray, threshold_ratio: float = 0.1) -> tuple[tu
 # --------------------------------------------------------------------------
 
 def extract_gqk(data: np.ndarray, rows: np.ndarray, cols: np.ndarray,
-                dtype=np.float64) -> tuple[np.ndarray, complex]:
+                dtype=np.float64, detector_gain: np.ndarray | None = None) -> tuple[np.ndarray, complex]:
     """Hermitian half-plane ``G[b, Qr, 0..N/2]`` and ``dc = mean_b G_b[0,0]``.
 
     ``cuda/backend.py:520-589``: gather BF columns, transpose to
-    ``[b, R_r, R_c]``, cast to complex, unnormalised forward ``fft2``, keep
-    columns ``0..N/2``.  The oracle uses ``rfft2`` (identical columns) at the
-    requested precision.
+    ``[b, R_r, R_c]``, cast to complex64 (samples rounded to float32), times
+    the BF pixels' detector gain in float32 when given (``:574-576``),
+    unnormalised forward ``fft2``, keep columns ``0..N/2``.  The oracle uses
+    ``rfft2`` (identical columns) at the requested precision.
     """
     import scipy.fft as sfft
     H, W = data.shape[:2]
     flat = data.reshape(H * W, data.shape[2], data.shape[3])
-    vbf = flat[:, rows, cols].T.reshape(len(rows), H, W)
+    vbf = flat[:, rows, cols].T.reshape(len(rows), H, W).astype(np.float32)
+    if detector_gain is not None:
+        vbf = vbf * np.asarray(detector_gain, dtype=np.float32)[rows, cols][:, None, None]
     real = np.float32 if dtype == np.float32 else np.float64
     g = sfft.rfft2(vbf.astype(real), axes=(-2, -1), workers=_workers())
     cdt = np.complex64 if dtype == np.float32 else np.complex128
@@ -595,16 +608,29 @@ def corrected_fourier_mean(G_half, dc, g: Geometry, c10, c12, phi12,
     divides the group sum by N_bf.  Summation here is in the oracle's
     precision (pairwise NumPy sums).
     """
+    from concurrent.futures import ThreadPoolExecutor
     dt = g.dtype
     cdt = np.complex64 if dt == np.float32 else np.complex128
     nb, n = G_half.shape[0], G_half.shape[1]
     acc = np.zeros((n, n), dtype=np.complex128)
     step = _bf_chunk(nb, n, dt)
-    for b0 in range(0, nb, step):
+    starts = list(range(0, nb, step))
+
+    def chunk(b0):
         b1 = min(nb, b0 + step)
         gh = gamma_fn(b0, b1) if gamma_fn else gamma_hat_block(g, b0, b1, c10, c12, phi12)
         full = _full_plane(G_half[b0:b1].astype(cdt))
-        acc += (full * np.conj(gh)).sum(axis=0)
+        return (full * np.conj(gh)).sum(axis=0)
+
+    # chunks on a thread pool; partial sums added in chunk order
+    workers = max(1, min(len(starts), _workers()))
+    if workers == 1:
+        for b0 in starts:
+            acc += chunk(b0)
+    else:
+        with ThreadPoolExecutor(max_workers=workers) as ex:
+            for part in ex.map(chunk, starts):
+                acc += part
     acc /= nb
     acc[0, 0] = dc
     return acc.astype(cdt)
@@ -621,22 +647,41 @@ def object_wave(G_half, dc, g: Geometry, c10, c12, phi12) -> np.ndarray:
 
 
 def _per_bf_moments(G_half, dc, g: Geometry, gamma_fn):
+    """(sum_b phi_b, sum_b phi_b^2) over BF chunks.  Chunks run on a thread
+    pool (NumPy's elementwise work and the FFTs release the GIL) and their
+    partial sums are added in chunk order, so the result does not depend on
+    the thread count."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import scipy.fft as sfft
     dt = g.dtype
     cdt = np.complex64 if dt == np.float32 else np.complex128
     nb, n = G_half.shape[0], G_half.shape[1]
-    s1 = np.zeros((n, n), dtype=np.float64)
-    s2 = np.zeros((n, n), dtype=np.float64)
     step = _bf_chunk(nb, n, dt)
-    import scipy.fft as sfft
-    for b0 in range(0, nb, step):
+    starts = list(range(0, nb, step))
+    workers = max(1, min(len(starts), _workers()))
+
+    def chunk(b0):
         b1 = min(nb, b0 + step)
         gh = gamma_fn(b0, b1)
         corr = _full_plane(G_half[b0:b1].astype(cdt)) * np.conj(gh)
         corr[:, 0, 0] = dc
-        ob = sfft.ifft2(corr, axes=(-2, -1), workers=_workers())
+        ob = sfft.ifft2(corr, axes=(-2, -1), workers=1 if workers > 1 else _workers())
         ph = np.angle(ob).astype(dt)
-        s1 += ph.sum(axis=0, dtype=np.float64)
-        s2 += (ph.astype(np.float64) ** 2).sum(axis=0)
+        return ph.sum(axis=0, dtype=np.float64), (ph.astype(np.float64) ** 2).sum(axis=0)
+
+    s1 = np.zeros((n, n), dtype=np.float64)
+    s2 = np.zeros((n, n), dtype=np.float64)
+    if workers == 1:
+        parts = map(chunk, starts)
+        for a, b in parts:
+            s1 += a
+            s2 += b
+        return s1, s2
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        for a, b in ex.map(chunk, starts):
+            s1 += a
+            s2 += b
     return s1, s2
 
 
@@ -700,13 +745,13 @@ class Prepared:
 
 def prepare(data, *, voltage_kV, semiangle_mrad, scan_sampling, det_sampling,
             rotation_rad=0.0, bf_intensity_threshold=0.0, bf_radius=None,
-            dtype=np.float64) -> Prepared:
+            dtype=np.float64, detector_gain=None) -> Prepared:
     """``CudaSSBBackend.__init__`` data path (``cuda/backend.py:168-365``):
     scan normalisation, mean DP, BF mask, G extraction, geometry."""
     data = normalize_scan(np.asarray(data))
-    sums, mean = mean_dp(data)
+    sums, mean = mean_dp(data, detector_gain)
     bf = select_bf(mean, bf_intensity_threshold, bf_radius)
-    G, dc = extract_gqk(data, bf.rows, bf.cols, dtype)
+    G, dc = extract_gqk(data, bf.rows, bf.cols, dtype, detector_gain)
     ds = det_sampling if isinstance(det_sampling, (tuple, list)) else (det_sampling, det_sampling)
     lam = wavelength_angstrom(voltage_kV)
     geom = detector_geometry(bf.rows, bf.cols, bf.center, data.shape[2:], ds, lam,

@@ -28,8 +28,12 @@ struct ssb_plan {
   Optics optics{};
   // device buffers
   uint32_t* dp_partial = nullptr;
+  void* dp_partial_wide = nullptr;   // 64-bit partials of 32-bit samples (allocated on first use)
   unsigned long long* dp_sums = nullptr;
   float* dp_mean = nullptr;
+  float* dp_dcpix = nullptr;         // per-pixel G_b(0,0) contribution, f32(sum) * gain
+  float* gain = nullptr;             // detector gain (NULL = none)
+  int dp_dtype = QSSB_DTYPE_U16;     // sample type of the last mean DP
   int* bf_rows = nullptr;
   int* bf_cols = nullptr;
   // tuning knobs (ssb_tune_set): stream-pass workgroups, gather FFT pairs,
@@ -172,6 +176,13 @@ __global__ void k_import_g(const float2* __restrict__ in, int gcols, int n, int 
 
 }  // namespace
 
+namespace qssb {
+// The thread's last-error record, for entry points in other translation units.
+int report_error(int code, const std::string& msg) { return fail(code, msg); }
+int report_hip_error(hipError_t e, const char* where) { return hip_fail(e, where); }
+void clear_error() { g_last_error.clear(); }
+}  // namespace qssb
+
 static void release_events(ssb_plan* p);
 
 extern "C" {
@@ -237,6 +248,7 @@ int ssb_plan_create(int scan_n, int det_rows, int det_cols, int max_bf, double w
   if (e == hipSuccess) e = dmalloc(&p->dp_partial, (size_t)p->max_ranges * p->D);
   if (e == hipSuccess) e = dmalloc(&p->dp_sums, (size_t)p->D);
   if (e == hipSuccess) e = dmalloc(&p->dp_mean, (size_t)p->D);
+  if (e == hipSuccess) e = dmalloc(&p->dp_dcpix, (size_t)p->D);
   if (e == hipSuccess) e = dmalloc(&p->bf_rows, (size_t)max_bf);
   if (e == hipSuccess) e = dmalloc(&p->bf_cols, (size_t)max_bf);
   if (e == hipSuccess) e = dmalloc(&p->info, 1);
@@ -311,8 +323,11 @@ int ssb_set_scan_frequencies(ssb_plan* p, const float* qrow_host, const float* q
 int ssb_plan_destroy(ssb_plan* p) {
   if (!p) return QSSB_OK;
   dfree(p->dp_partial);
+  dfree(p->dp_partial_wide);
   dfree(p->dp_sums);
   dfree(p->dp_mean);
+  dfree(p->dp_dcpix);
+  dfree(p->gain);
   dfree(p->bf_rows);
   dfree(p->bf_cols);
   if (p->ev_stream) (void)hipEventDestroy(p->ev_stream);
@@ -344,14 +359,43 @@ int ssb_plan_destroy(ssb_plan* p) {
   return QSSB_OK;
 }
 
-int ssb_mean_dp_u16(ssb_plan* p, const uint16_t* data, int64_t frames, uint64_t* sums_out, float* mean_out,
-                    void* stream) {
+static bool dtype_unsigned_int(int dtype) {
+  return dtype == QSSB_DTYPE_U8 || dtype == QSSB_DTYPE_U16 || dtype == QSSB_DTYPE_U32;
+}
+
+// partial-sum buffer of the stream pass for this sample type
+static int dp_partial_for(ssb_plan* p, int dtype, void** out) {
+  if (!dtype_wide_partials(dtype)) {
+    *out = p->dp_partial;
+    return QSSB_OK;
+  }
+  if (!p->dp_partial_wide) {
+    hipError_t e = hipMalloc(&p->dp_partial_wide, (size_t)p->max_ranges * p->D * 8);
+    if (e != hipSuccess) {
+      p->dp_partial_wide = nullptr;
+      return hip_fail(e, "mean DP partial allocation");
+    }
+  }
+  *out = p->dp_partial_wide;
+  return QSSB_OK;
+}
+
+static int mean_dp_typed(ssb_plan* p, const void* data, int dtype, int64_t frames, uint64_t* sums_out,
+                         float* mean_out, void* stream) {
   int rc = check_plan(p);
   if (rc) return rc;
   if (!data || frames < 1) return fail(QSSB_ERR_ARG, "data is NULL or has no frames");
+  if (!dtype_size(dtype)) return fail(QSSB_ERR_ARG, "unknown sample type " + std::to_string(dtype));
+  if (sums_out && dtype == QSSB_DTYPE_F32)
+    return fail(QSSB_ERR_ARG, "float32 frames have no integer sums; pass sums_out = NULL");
+  void* partial = nullptr;
+  rc = dp_partial_for(p, dtype, &partial);
+  if (rc) return rc;
   hipStream_t s = as_stream(stream);
-  HIP_TRY(launch_mean_dp(data, frames, p->D, p->dp_partial, p->max_ranges, p->tune_stream_wg, p->dp_sums, p->dp_mean, s),
+  HIP_TRY(launch_mean_dp(dtype, data, frames, p->D, partial, p->max_ranges, p->tune_stream_wg, p->gain, p->dp_sums,
+                         p->dp_mean, p->dp_dcpix, s),
           "mean_dp");
+  p->dp_dtype = dtype;
   if (sums_out)
     HIP_TRY(hipMemcpyAsync(sums_out, p->dp_sums, (size_t)p->D * 8, hipMemcpyDeviceToDevice, s), "mean_dp copy");
   if (mean_out)
@@ -359,12 +403,43 @@ int ssb_mean_dp_u16(ssb_plan* p, const uint16_t* data, int64_t frames, uint64_t*
   return QSSB_OK;
 }
 
+int ssb_dtype_size(int dtype) { return dtype_size(dtype); }
+
+int ssb_mean_dp_u16(ssb_plan* p, const uint16_t* data, int64_t frames, uint64_t* sums_out, float* mean_out,
+                    void* stream) {
+  return mean_dp_typed(p, data, QSSB_DTYPE_U16, frames, sums_out, mean_out, stream);
+}
+
+int ssb_mean_dp_typed(ssb_plan* p, const void* data, int dtype, int64_t frames, uint64_t* sums_out,
+                      float* mean_out, void* stream) {
+  return mean_dp_typed(p, data, dtype, frames, sums_out, mean_out, stream);
+}
+
+int ssb_set_detector_gain(ssb_plan* p, const float* gain, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  hipStream_t s = as_stream(stream);
+  if (!gain) {
+    if (p->gain) {
+      HIP_TRY(hipStreamSynchronize(s), "set_detector_gain sync");
+      dfree(p->gain);
+      p->gain = nullptr;
+    }
+    return QSSB_OK;
+  }
+  if (!p->gain) HIP_TRY(dmalloc(&p->gain, (size_t)p->D), "detector gain allocation");
+  HIP_TRY(hipMemcpyAsync(p->gain, gain, (size_t)p->D * 4, hipMemcpyDeviceToDevice, s), "set_detector_gain");
+  return QSSB_OK;
+}
+
+static bool exact_dc(const ssb_plan* p) { return dtype_unsigned_int(p->dp_dtype) && !p->gain; }
+
 int ssb_select_bf(ssb_plan* p, float threshold, float bf_radius, void* stream) {
   int rc = check_plan(p);
   if (rc) return rc;
   hipStream_t s = as_stream(stream);
-  HIP_TRY(launch_select_bf(p->dp_mean, p->dp_sums, p->det_rows, p->det_cols, threshold, bf_radius, p->max_bf,
-                           p->bf_rows, p->bf_cols, p->info, s),
+  HIP_TRY(launch_select_bf(p->dp_mean, p->dp_sums, p->dp_dcpix, exact_dc(p), p->det_rows, p->det_cols, threshold,
+                           bf_radius, p->max_bf, p->bf_rows, p->bf_cols, p->info, s),
           "select_bf");
   p->geometry_valid = false;
   p->gcols = 0;
@@ -475,10 +550,11 @@ static int alloc_gt(ssb_plan* p, int gcols, int nbf) {
   return QSSB_OK;
 }
 
-int ssb_prepare_g_u16(ssb_plan* p, const uint16_t* data, void* stream) {
+static int prepare_g_typed(ssb_plan* p, const void* data, int dtype, void* stream) {
   int rc = check_plan(p);
   if (rc) return rc;
   if (!data) return fail(QSSB_ERR_ARG, "data is NULL");
+  if (!dtype_size(dtype)) return fail(QSSB_ERR_ARG, "unknown sample type " + std::to_string(dtype));
   hipStream_t s = as_stream(stream);
   {
     // the resident stack is sized by the current selection: read it back
@@ -492,8 +568,8 @@ int ssb_prepare_g_u16(ssb_plan* p, const uint16_t* data, void* stream) {
   if (rc) return rc;
   rc = alloc_h(p, p->nbf_hint);
   if (rc) return rc;
-  HIP_TRY(launch_gather_rowfft(p->n, data, p->D, p->det_cols, p->bf_rows, p->bf_cols, p->info, p->nbf_hint, p->tw,
-                               p->tune_gather_pairs, p->H, s),
+  HIP_TRY(launch_gather_rowfft(p->n, dtype, data, p->D, p->det_cols, p->bf_rows, p->bf_cols, p->info, p->nbf_hint,
+                               p->tw, p->tune_gather_pairs, p->gain, p->H, s),
           "gather_rowfft");
   HIP_TRY(launch_trotter_slots(p->n, true, p->H, p->info, geom_ptrs(p), p->rowph, p->colph, p->echi, p->optics, p->tw,
                                p->S_part, p->Gt, false, p->tune_trotter, s),
@@ -501,6 +577,14 @@ int ssb_prepare_g_u16(ssb_plan* p, const uint16_t* data, void* stream) {
   p->gcols = p->n / 2 + 1;
   p->g_nbf = p->nbf_hint;
   return QSSB_OK;
+}
+
+int ssb_prepare_g_u16(ssb_plan* p, const uint16_t* data, void* stream) {
+  return prepare_g_typed(p, data, QSSB_DTYPE_U16, stream);
+}
+
+int ssb_prepare_g_typed(ssb_plan* p, const void* data, int dtype, void* stream) {
+  return prepare_g_typed(p, data, dtype, stream);
 }
 
 int ssb_export_g(ssb_plan* p, void* g_out, int nbf, void* stream) {
@@ -647,8 +731,8 @@ int ssb_max_u16(const uint16_t* data, long long count, unsigned int* max_out_hos
   return QSSB_OK;
 }
 
-int ssb_bf_columns_u16(ssb_plan* p, const uint16_t* data, int bf_offset, int count, int narrow_u8, void* out,
-                       void* stream) {
+static int bf_columns(ssb_plan* p, const void* data, bool in_u8, int bf_offset, int count, int narrow_u8, void* out,
+                      void* stream) {
   int rc = check_plan(p);
   if (rc) return rc;
   if (!data || !out) return fail(QSSB_ERR_ARG, "data and output must not be NULL");
@@ -656,10 +740,19 @@ int ssb_bf_columns_u16(ssb_plan* p, const uint16_t* data, int bf_offset, int cou
   if (bf_offset < 0 || count < 1 || bf_offset + count > p->nbf_hint)
     return fail(QSSB_ERR_ARG, "BF range [" + std::to_string(bf_offset) + ", " + std::to_string(bf_offset + count) +
                                   ") is outside the selection of " + std::to_string(p->nbf_hint));
-  HIP_TRY(launch_bf_columns(data, (long long)p->n * p->n, p->D, p->det_cols, p->bf_rows, p->bf_cols, bf_offset,
-                            count, narrow_u8 != 0, out, as_stream(stream)),
+  HIP_TRY(launch_bf_columns(data, in_u8, (long long)p->n * p->n, p->D, p->det_cols, p->bf_rows, p->bf_cols,
+                            bf_offset, count, narrow_u8 != 0, out, as_stream(stream)),
           "bf_columns");
   return QSSB_OK;
+}
+
+int ssb_bf_columns_u16(ssb_plan* p, const uint16_t* data, int bf_offset, int count, int narrow_u8, void* out,
+                       void* stream) {
+  return bf_columns(p, data, false, bf_offset, count, narrow_u8, out, stream);
+}
+
+int ssb_bf_columns_u8(ssb_plan* p, const uint8_t* data, int bf_offset, int count, void* out, void* stream) {
+  return bf_columns(p, data, true, bf_offset, count, 1, out, stream);
 }
 
 int ssb_learn_features(ssb_plan* p, const int* qrow_host, const int* qcol_host, float* out, void* stream) {
@@ -688,10 +781,11 @@ int ssb_scan_target(int H, int W) {
   return longest <= 128 ? 128 : (longest <= 256 ? 256 : (longest <= 512 ? 512 : 1024));
 }
 
-int ssb_normalize_scan_u16(const uint16_t* src, int H, int W, int det_rows, int det_cols, uint16_t* dst, int T,
-                           void* stream) {
+static int normalize_scan_typed(const void* src, int dtype, int H, int W, int det_rows, int det_cols, void* dst,
+                                int T, void* stream) {
   g_last_error.clear();
   if (!src || !dst) return fail(QSSB_ERR_ARG, "source and destination must not be NULL");
+  if (!dtype_size(dtype)) return fail(QSSB_ERR_ARG, "unknown sample type " + std::to_string(dtype));
   if (H < 1 || W < 1 || det_rows < 1 || det_cols < 1) return fail(QSSB_ERR_ARG, "shapes must be positive");
   if (T != ssb_scan_target(H, W))
     return fail(QSSB_ERR_ARG, "target scan size " + std::to_string(T) + " is not the supported size for a " +
@@ -700,18 +794,28 @@ int ssb_normalize_scan_u16(const uint16_t* src, int H, int W, int det_rows, int 
   const long long D = (long long)det_rows * det_cols;
   if (D > (1 << 24)) return fail(QSSB_ERR_UNSUPPORTED, "detector too large");
   hipStream_t s = as_stream(stream);
-  uint32_t* partial = nullptr;
-  uint16_t* mean = nullptr;
-  hipError_t e = dmalloc(&partial, (size_t)normalize_partial_groups(H, T) * D);
-  if (e == hipSuccess) e = dmalloc(&mean, (size_t)D + 8);
-  if (e == hipSuccess) e = launch_normalize_scan(src, H, W, (int)D, dst, T, partial, mean, s);
+  size_t mean_off = 0;
+  const size_t bytes = normalize_workspace_bytes(dtype, H, T, (int)D, &mean_off);
+  uint8_t* ws = nullptr;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&ws), bytes);
+  if (e == hipSuccess) e = launch_normalize_scan_typed(dtype, src, H, W, (int)D, dst, T, ws, ws + mean_off, s);
   hipError_t e2 = hipStreamSynchronize(s);
-  dfree(partial);
-  dfree(mean);
+  dfree(ws);
   if (e != hipSuccess) return hip_fail(e, "normalize_scan");
   if (e2 != hipSuccess) return hip_fail(e2, "normalize_scan sync");
   return QSSB_OK;
 }
+
+int ssb_normalize_scan_u16(const uint16_t* src, int H, int W, int det_rows, int det_cols, uint16_t* dst, int T,
+                           void* stream) {
+  return normalize_scan_typed(src, QSSB_DTYPE_U16, H, W, det_rows, det_cols, dst, T, stream);
+}
+
+int ssb_normalize_scan_typed(const void* src, int dtype, int H, int W, int det_rows, int det_cols, void* dst, int T,
+                             void* stream) {
+  return normalize_scan_typed(src, dtype, H, W, det_rows, det_cols, dst, T, stream);
+}
+
 
 static ChiPoly chi_poly(const ssb_plan* p, const double* mags, const double* angles) {
   static const int kn[14] = {1, 1, 2, 2, 3, 3, 3, 4, 4, 4, 5, 5, 5, 5};
@@ -802,12 +906,16 @@ int ssb_batch_phase_loss(ssb_plan* p, int n_trials, const double* mags_host, con
   return QSSB_OK;
 }
 
-static int recon_u16(ssb_plan* p, const ssb_plan* after, const uint16_t* data, double c10, double c12,
-                     double phi12, double rotation_rad, float threshold, float bf_radius, void* obj_out,
-                     float* phase_out, hipStream_t s) {
+static int recon_typed(ssb_plan* p, const ssb_plan* after, const void* data, int dtype, double c10, double c12,
+                       double phi12, double rotation_rad, float threshold, float bf_radius, void* obj_out,
+                       float* phase_out, hipStream_t s) {
   int rc = check_plan(p);
   if (rc) return rc;
   if (!data) return fail(QSSB_ERR_ARG, "data is NULL");
+  if (!dtype_size(dtype)) return fail(QSSB_ERR_ARG, "unknown sample type " + std::to_string(dtype));
+  void* partial = nullptr;
+  rc = dp_partial_for(p, dtype, &partial);
+  if (rc) return rc;
   if (after == p) return fail(QSSB_ERR_ARG, "a plan cannot be ordered after itself");
   if (after) {
     rc = check_plan(after);
@@ -825,16 +933,19 @@ static int recon_u16(ssb_plan* p, const ssb_plan* after, const uint16_t* data, d
   };
   const long long frames = (long long)p->n * p->n;
   mark(0);
-  HIP_TRY(launch_mean_dp_stream(data, frames, p->D, p->dp_partial, p->max_ranges, p->tune_stream_wg, s), "mean_dp");
+  HIP_TRY(launch_mean_dp_stream(dtype, data, frames, p->D, partial, p->max_ranges, p->tune_stream_wg, s), "mean_dp");
   if (p->ev_stream) {
     HIP_TRY(hipEventRecord(p->ev_stream, s), "stream pass event");
     p->ev_stream_recorded = true;
   }
   mark(1);
-  HIP_TRY(launch_mean_dp_reduce(frames, p->D, p->dp_partial, p->max_ranges, p->tune_stream_wg, p->dp_sums, p->dp_mean, s), "mean_dp");
+  HIP_TRY(launch_mean_dp_reduce(dtype, data, frames, p->D, partial, p->max_ranges, p->tune_stream_wg, p->gain,
+                                p->dp_sums, p->dp_mean, p->dp_dcpix, s),
+          "mean_dp");
+  p->dp_dtype = dtype;
   mark(2);
-  HIP_TRY(launch_select_bf(p->dp_mean, p->dp_sums, p->det_rows, p->det_cols, threshold, bf_radius, p->max_bf,
-                           p->bf_rows, p->bf_cols, p->info, s),
+  HIP_TRY(launch_select_bf(p->dp_mean, p->dp_sums, p->dp_dcpix, exact_dc(p), p->det_rows, p->det_cols, threshold,
+                           bf_radius, p->max_bf, p->bf_rows, p->bf_cols, p->info, s),
           "select_bf");
   mark(3);
   p->rotation = rotation_rad;
@@ -844,8 +955,8 @@ static int recon_u16(ssb_plan* p, const ssb_plan* after, const uint16_t* data, d
   rc = alloc_h(p, p->max_bf);
   if (rc) return rc;
   mark(4);
-  HIP_TRY(launch_gather_rowfft(p->n, data, p->D, p->det_cols, p->bf_rows, p->bf_cols, p->info, p->nbf_hint, p->tw,
-                               p->tune_gather_pairs, p->H, s),
+  HIP_TRY(launch_gather_rowfft(p->n, dtype, data, p->D, p->det_cols, p->bf_rows, p->bf_cols, p->info, p->nbf_hint,
+                               p->tw, p->tune_gather_pairs, p->gain, p->H, s),
           "gather_rowfft");
   mark(5);
   const QuadChi m = quad_chi(p, c10, c12, phi12);
@@ -863,16 +974,23 @@ static int recon_u16(ssb_plan* p, const ssb_plan* after, const uint16_t* data, d
 
 int ssb_recon_u16(ssb_plan* p, const uint16_t* data, double c10, double c12, double phi12, double rotation_rad,
                   float threshold, float bf_radius, void* obj_out, float* phase_out, void* stream) {
-  return recon_u16(p, nullptr, data, c10, c12, phi12, rotation_rad, threshold, bf_radius, obj_out, phase_out,
-                   as_stream(stream));
+  return recon_typed(p, nullptr, data, QSSB_DTYPE_U16, c10, c12, phi12, rotation_rad, threshold, bf_radius, obj_out,
+                     phase_out, as_stream(stream));
+}
+
+int ssb_recon_typed(ssb_plan* p, const void* data, int dtype, double c10, double c12, double phi12,
+                    double rotation_rad, float threshold, float bf_radius, void* obj_out, float* phase_out,
+                    void* stream) {
+  return recon_typed(p, nullptr, data, dtype, c10, c12, phi12, rotation_rad, threshold, bf_radius, obj_out,
+                     phase_out, as_stream(stream));
 }
 
 int ssb_recon_u16_after(ssb_plan* p, const ssb_plan* after, const uint16_t* data, double c10, double c12,
                         double phi12, double rotation_rad, float threshold, float bf_radius, void* obj_out,
                         float* phase_out, void* stream) {
   if (!after) return fail(QSSB_ERR_ARG, "after is NULL");
-  return recon_u16(p, after, data, c10, c12, phi12, rotation_rad, threshold, bf_radius, obj_out, phase_out,
-                   as_stream(stream));
+  return recon_typed(p, after, data, QSSB_DTYPE_U16, c10, c12, phi12, rotation_rad, threshold, bf_radius, obj_out,
+                     phase_out, as_stream(stream));
 }
 
 int ssb_stage_count(void) { return NSTAGES; }

@@ -4,7 +4,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 #include "ssb_device.h"
+#include "quantem_ssb.h"
 
 // Device-side status of the bright-field selection.
 #define QSSB_DEV_NO_CANDIDATES 1
@@ -12,6 +15,12 @@
 #define QSSB_DEV_CAPACITY 3
 
 namespace qssb {
+
+// Record the calling thread's last error (ssb_last_error) and return `code`
+// (or the status of a HIP error); clear it at the start of an entry point.
+int report_error(int code, const std::string& msg);
+int report_hip_error(hipError_t e, const char* where);
+void clear_error();
 
 // Bright-field selection result, resident in device memory so that the
 // pipeline can run without a host round trip.
@@ -63,17 +72,31 @@ struct QuadChi {
   float mxx, mxy, myy;
 };
 
-// target_wg: workgroups of the stream pass (0 = default 1536).
-hipError_t launch_mean_dp(const uint16_t* data, long long frames, int D, uint32_t* partial, int max_ranges,
-                          int target_wg, unsigned long long* sums, float* mean, hipStream_t s);
+// Bytes per sample of a QSSB_DTYPE_* code (0 = unknown); whether its
+// stream-pass partials are 64-bit (uint32, int32, float32 samples).
+int dtype_size(int dtype);
+bool dtype_wide_partials(int dtype);
+
+// Mean DP of `frames` frames of D samples of type `dtype`.  partial holds
+// max_ranges * D partial sums (32-bit, or 64-bit when dtype_wide_partials);
+// target_wg: workgroups of the stream pass (0 = default 1536).  gain (D
+// floats) may be NULL.  sums: exact uint64 sums (integer samples; 0 for
+// float32), mean: f32(sum)/f32(frames) * gain, dcpix: f32(sum) * gain
+// (signed samples: the signed sum).
+hipError_t launch_mean_dp(int dtype, const void* data, long long frames, int D, void* partial, int max_ranges,
+                          int target_wg, const float* gain, unsigned long long* sums, float* mean, float* dcpix,
+                          hipStream_t s);
 // the two passes of launch_mean_dp, separately (stage timing)
-hipError_t launch_mean_dp_stream(const uint16_t* data, long long frames, int D, uint32_t* partial, int max_ranges,
+hipError_t launch_mean_dp_stream(int dtype, const void* data, long long frames, int D, void* partial, int max_ranges,
                                  int target_wg, hipStream_t s);
-hipError_t launch_mean_dp_reduce(long long frames, int D, const uint32_t* partial, int max_ranges, int target_wg,
-                                 unsigned long long* sums, float* mean, hipStream_t s);
-hipError_t launch_select_bf(const float* mean, const unsigned long long* sums, int det_rows, int det_cols,
-                            float threshold, float bf_radius, int max_bf, int* rows, int* cols, BfInfo* info,
-                            hipStream_t s);
+hipError_t launch_mean_dp_reduce(int dtype, const void* data, long long frames, int D, const void* partial,
+                                 int max_ranges, int target_wg, const float* gain, unsigned long long* sums,
+                                 float* mean, float* dcpix, hipStream_t s);
+// exact_dc: dc from the exact unsigned integer sums (no gain); otherwise
+// from a compensated sum of dcpix over the selected pixels.
+hipError_t launch_select_bf(const float* mean, const unsigned long long* sums, const float* dcpix, bool exact_dc,
+                            int det_rows, int det_cols, float threshold, float bf_radius, int max_bf, int* rows,
+                            int* cols, BfInfo* info, hipStream_t s);
 // Probe disk centre and radius of a float32 mean DP; out[4] = {status, row, col, radius}.
 hipError_t launch_detect_bf_radius(const float* mean, int R, int C, df ratio, int* out, hipStream_t s);
 hipError_t launch_bf_geometry(const int* rows, const int* cols, const BfInfo* info, int max_bf, df recip_y,
@@ -83,12 +106,14 @@ hipError_t launch_bf_geometry(const int* rows, const int* cols, const BfInfo* in
 // gather_pairs_candidates(N) (0 = gather_pairs_default(N)).
 int gather_pairs_default(int N);
 int gather_pairs_candidates(int N, int* out);   // out[2]; returns the count
-hipError_t launch_gather_rowfft(int N, const uint16_t* data, int D, int det_cols, const int* rows,
+// Samples of type dtype become float32 times gain[pixel] (gain may be NULL).
+hipError_t launch_gather_rowfft(int N, int dtype, const void* data, int D, int det_cols, const int* rows,
                                 const int* cols, const BfInfo* info, int nbf_hint, const float2* tw, int pairs,
-                                float2* H, hipStream_t s);
-// Exact BF columns out[b][s] (uint16, or uint8 when narrow) for BF pixels
-// [bf_offset, bf_offset + count) of the list rows/cols.
-hipError_t launch_bf_columns(const uint16_t* data, long long frames, int D, int det_cols, const int* rows,
+                                const float* gain, float2* H, hipStream_t s);
+// Exact BF columns out[b][s] for BF pixels [bf_offset, bf_offset + count)
+// of the list rows/cols: uint8 frames -> uint8 (in_u8), uint16 frames ->
+// uint16, or uint8 when narrow.
+hipError_t launch_bf_columns(const void* data, bool in_u8, long long frames, int D, int det_cols, const int* rows,
                              const int* cols, int bf_offset, int count, bool narrow, void* out, hipStream_t s);
 // max of n uint16 values into out[0] (partial holds max_u16_parts() uints).
 hipError_t launch_max_u16(const uint16_t* data, long long n, unsigned int* partial, unsigned int* out,
@@ -100,6 +125,11 @@ int max_u16_parts();
 hipError_t launch_normalize_scan(const uint16_t* src, int H, int W, int D, uint16_t* dst, int T,
                                  uint32_t* partial, uint16_t* mean, hipStream_t s);
 int normalize_partial_groups(int H, int T);
+// The same for any QSSB_DTYPE_* sample type; partial and mean as laid out by
+// normalize_workspace_bytes (mean at *mean_offset of the workspace).
+hipError_t launch_normalize_scan_typed(int dtype, const void* src, int H, int W, int D, void* dst, int T,
+                                       void* partial, void* mean, hipStream_t s);
+size_t normalize_workspace_bytes(int dtype, int H, int T, int D, size_t* mean_offset);
 
 // Constants of the isotropic aperture evaluated from d = Rout^2 - |u|^2.
 struct ApIso {

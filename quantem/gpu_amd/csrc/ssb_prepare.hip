@@ -6,6 +6,8 @@
 //   mean DP / mask     src/quantem/gpu/ssb/compute/cuda/backend.py:420-518
 //   geometry           src/quantem/gpu/ssb/compute/cuda/engine.py:784-848
 //   G extraction       src/quantem/gpu/ssb/compute/cuda/backend.py:520-589
+#include <type_traits>
+
 #include "ssb_internal.h"
 
 namespace qssb {
@@ -13,11 +15,14 @@ namespace qssb {
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 // ------------------------------------------------------------------------
-// Mean DP, pass 1: per frame-range uint32 partial sums.
+// Mean DP, pass 1: per frame-range partial sums.
 // grid = (pixel tiles of 256*VEC, frame ranges).  Every workgroup streams a
 // contiguous 256*VEC-pixel slice of `frames_per_range` consecutive frames,
 // so each wave issues 16-byte coalesced non-temporal loads.  A range holds
-// at most 65536 frames, so uint32 cannot overflow (65536 * 65535 < 2^32).
+// at most 65536 frames, so the 32-bit partials of 8- and 16-bit samples
+// cannot overflow (65536 * 65535 < 2^32, 65536 * 32768 = 2^31); 32-bit
+// samples keep 64-bit partials (signed ones sign-extended, modulo 2^64, as
+// the reference's uint64 accumulator), float32 samples double-float pairs.
 // ------------------------------------------------------------------------
 template <int VEC>
 __global__ void __launch_bounds__(256)
@@ -70,38 +75,145 @@ k_dp_partial(const uint16_t* __restrict__ data, long long frames, int D, int fra
   }
 }
 
-// Pass 2: fold the frame ranges into exact uint64 sums; mean = f32(sum)/f32(N)
-// with one correctly rounded division (backend.py:438-441).  A workgroup
+// Partial-sum arithmetic per sample type: P is the stored partial.
+template <typename T> struct DpElem;
+template <> struct DpElem<uint8_t> { typedef uint32_t P; };
+template <> struct DpElem<int8_t> { typedef int32_t P; };
+template <> struct DpElem<uint16_t> { typedef uint32_t P; };
+template <> struct DpElem<int16_t> { typedef int32_t P; };
+template <> struct DpElem<uint32_t> { typedef unsigned long long P; };
+template <> struct DpElem<int32_t> { typedef unsigned long long P; };
+template <> struct DpElem<float> { typedef float2 P; };
+
+template <typename P> __device__ __forceinline__ void dp_zero(P& a) { a = 0; }
+template <> __device__ __forceinline__ void dp_zero<float2>(float2& a) { a = make_float2(0.f, 0.f); }
+
+template <typename T, typename P>
+__device__ __forceinline__ void dp_add(P& a, T v) {
+  if constexpr (std::is_same<P, float2>::value) {
+    // cascaded summation: the rounding error of every add goes to .y
+    const df t = two_sum(a.x, v);
+    a.x = t.hi;
+    a.y += t.lo;
+  } else if constexpr (std::is_same<T, int32_t>::value) {
+    a += (unsigned long long)(long long)v;
+  } else {
+    a += (P)v;
+  }
+}
+
+// Generic stream pass for the non-uint16 sample types: VEC samples of 16
+// bytes per thread and frame when the detector row allows it, else one.
+template <typename T, int VEC>
+__global__ void __launch_bounds__(256)
+k_dp_partial_t(const T* __restrict__ data, long long frames, int D, int frames_per_range,
+               typename DpElem<T>::P* __restrict__ partial) {
+  typedef typename DpElem<T>::P P;
+  const int pix0 = (blockIdx.x * 256 + threadIdx.x) * VEC;
+  const long long f0 = (long long)blockIdx.y * frames_per_range;
+  long long f1 = f0 + frames_per_range;
+  if (f1 > frames) f1 = frames;
+  if (pix0 >= D) return;
+  P acc[VEC];
+#pragma unroll
+  for (int e = 0; e < VEC; ++e) dp_zero(acc[e]);
+  const T* p = data + f0 * (long long)D + pix0;
+  for (long long f = f0; f < f1; ++f) {
+    if constexpr (VEC * sizeof(T) == 16) {
+      union {
+        u32x4 v;
+        T e[VEC];
+      } u;
+      u.v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) dp_add<T, P>(acc[e], u.e[e]);
+    } else {
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) dp_add<T, P>(acc[e], p[e]);
+    }
+    p += D;
+  }
+  P* out = partial + (long long)blockIdx.y * D + pix0;
+#pragma unroll
+  for (int e = 0; e < VEC; ++e) out[e] = acc[e];
+}
+
+// Pass 2: fold the frame ranges into exact uint64 sums (integer samples) or
+// a double-float sum (float32 samples); mean = f32(sum)/f32(N) with one
+// correctly rounded division (backend.py:438-441), times the detector gain
+// when one is set (backend.py:442-443; x * 1.0f == x, so no gain leaves the
+// mean unchanged).  dcpix[i] = f32(sum) * gain is pixel i's contribution to
+// G_b(0,0), used for dc when the integer path cannot be exact.  A workgroup
 // owns 64 pixels; its LANES waves fold interleaved shares of the ranges
 // (each wave reads 256 contiguous bytes per range) and combine in LDS.
-template <int LANES>
+template <typename P> struct DpSum { typedef unsigned long long S; };
+template <> struct DpSum<float2> { typedef float2 S; };
+
+template <typename P>
+__device__ __forceinline__ void dp_fold(typename DpSum<P>::S& s, P v) {
+  if constexpr (std::is_same<P, float2>::value) {
+    const df t = df_add(df{s.x, s.y}, df{v.x, v.y});
+    s = make_float2(t.hi, t.lo);
+  } else if constexpr (std::is_same<P, int32_t>::value) {
+    s += (unsigned long long)(long long)v;
+  } else {
+    s += (unsigned long long)v;
+  }
+}
+
+template <int LANES, typename P, bool SIGNED>
 __global__ void __launch_bounds__(64 * LANES)
-k_dp_reduce(const uint32_t* __restrict__ partial, int ranges, int D, long long frames,
-            unsigned long long* __restrict__ sums, float* __restrict__ mean) {
-  __shared__ unsigned long long part[LANES][64];
+k_dp_reduce(const P* __restrict__ partial, int ranges, int D, long long frames, const float* __restrict__ gain,
+            unsigned long long* __restrict__ sums, float* __restrict__ mean, float* __restrict__ dcpix) {
+  typedef typename DpSum<P>::S S;
+  __shared__ S part[LANES][64];
   const int lane = threadIdx.x & 63;
   const int quarter = threadIdx.x >> 6;
   const int i = blockIdx.x * 64 + lane;
-  unsigned long long s = 0ull;
+  S s;
+  dp_zero(s);
   if (i < D) {
     int r = quarter;
     for (; r + 3 * LANES < ranges; r += 4 * LANES) {
-      const uint32_t a = partial[(long long)r * D + i];
-      const uint32_t b = partial[(long long)(r + LANES) * D + i];
-      const uint32_t c = partial[(long long)(r + 2 * LANES) * D + i];
-      const uint32_t d = partial[(long long)(r + 3 * LANES) * D + i];
-      s += (unsigned long long)a + b + c + d;
+      const P a = partial[(long long)r * D + i];
+      const P b = partial[(long long)(r + LANES) * D + i];
+      const P c = partial[(long long)(r + 2 * LANES) * D + i];
+      const P d = partial[(long long)(r + 3 * LANES) * D + i];
+      dp_fold<P>(s, a);
+      dp_fold<P>(s, b);
+      dp_fold<P>(s, c);
+      dp_fold<P>(s, d);
     }
-    for (; r < ranges; r += LANES) s += partial[(long long)r * D + i];
+    for (; r < ranges; r += LANES) dp_fold<P>(s, partial[(long long)r * D + i]);
   }
   part[quarter][lane] = s;
   __syncthreads();
   if (quarter == 0 && i < D) {
-    unsigned long long t = 0ull;
+    S t;
+    dp_zero(t);
 #pragma unroll
-    for (int k = 0; k < LANES; ++k) t += part[k][lane];
-    sums[i] = t;
-    mean[i] = (float)t / (float)frames;
+    for (int k = 0; k < LANES; ++k) {
+      if constexpr (std::is_same<P, float2>::value) {
+        const df u = df_add(df{t.x, t.y}, df{part[k][lane].x, part[k][lane].y});
+        t = make_float2(u.hi, u.lo);
+      } else {
+        t += part[k][lane];
+      }
+    }
+    const float g = gain ? gain[i] : 1.0f;
+    float total;
+    if constexpr (std::is_same<P, float2>::value) {
+      total = t.x + t.y;
+      sums[i] = 0ull;
+    } else {
+      total = (float)t;
+      sums[i] = t;
+    }
+    mean[i] = (total / (float)frames) * g;
+    float dcv;
+    if constexpr (SIGNED) dcv = (float)(long long)t;
+    else dcv = total;
+    dcpix[i] = dcv * g;
   }
 }
 
@@ -222,6 +334,7 @@ __device__ __forceinline__ float count_over_pi(float n) {
 template <bool CACHE>
 __global__ void __launch_bounds__(MASK_THREADS)
 k_select_bf(const float* __restrict__ mean_g, const unsigned long long* __restrict__ sums,
+            const float* __restrict__ dcpix, int exact_dc,
             int det_rows, int det_cols, float threshold, float bf_radius, int max_bf,
             int* __restrict__ bf_rows, int* __restrict__ bf_cols, BfInfo* __restrict__ info) {
   __shared__ float sh_f[2 * MASK_WAVES];
@@ -329,18 +442,22 @@ k_select_bf(const float* __restrict__ mean_g, const unsigned long long* __restri
   unsigned long long mask_bits = 0ull;
   long long any_cand = 0, dcs = 0;
   int local = 0;
+  Kahan kd;
+  kd.init();
   FOR_PIX_RC {
     if (mean[i] > cut) {
       any_cand += 1;
       if (dist2_f32((float)r_, (float)c_, center_r, center_c) <= r2) {
         mask_bits |= 1ull << (i - p0);
         local += 1;
-        dcs += (long long)sums[i];
+        if (exact_dc) dcs += (long long)sums[i];
+        else kd.add(dcpix[i]);
       }
     }
   }
   long long dummy = 0;
   block_sum3_i64(any_cand, dcs, dummy, sh_i);
+  const float dc_float = exact_dc ? 0.0f : block_sum_comp(kd.s, -kd.c, sh_f);
   // exclusive scan of `local` over the block: wave scan + wave offsets
   int incl = local;
   for (int off = 1; off < 64; off <<= 1) {
@@ -379,12 +496,16 @@ k_select_bf(const float* __restrict__ mean_g, const unsigned long long* __restri
     info->center_c = center_c;
     info->radius2 = r2;
     float dc = 0.0f;
-    if (nb > 0) {
+    if (nb > 0 && exact_dc) {
       // G_b(0,0) is the exact integer column sum; dc = mean_b G_b(0,0).
       const unsigned long long tot = (unsigned long long)dcs;
       const unsigned long long qd = tot / (unsigned long long)nb;
       const unsigned long long rm = tot % (unsigned long long)nb;
       dc = (float)qd + (float)rm / (float)nb;
+    } else if (nb > 0) {
+      // gain-weighted, signed or float samples: compensated float32 sum of
+      // the per-pixel G_b(0,0) values
+      dc = dc_float / (float)nb;
     }
     info->dc_re = dc;
     info->dc_im = 0.0f;
@@ -425,17 +546,18 @@ k_bf_geometry(const int* __restrict__ bf_rows, const int* __restrict__ bf_cols,
 // [b][R_r][slot], slot 0 packing the purely real bins 0 and N/2 as
 // (X[0], X[N/2]); slots 1..N/2-1 hold X[k].
 // ------------------------------------------------------------------------
-template <int N, int PAIRS>
+template <int N, int PAIRS, typename T>
 __global__ void __launch_bounds__(256)
-k_gather_rowfft(const uint16_t* __restrict__ data, int D, int det_cols,
+k_gather_rowfft(const T* __restrict__ data, int D, int det_cols,
                 const int* __restrict__ bf_rows, const int* __restrict__ bf_cols,
                 const BfInfo* __restrict__ info, const float2* __restrict__ tw_g,
-                float2* __restrict__ H) {
+                const float* __restrict__ gain, float2* __restrict__ H) {
   constexpr int STRIDE = N + 1;
   constexpr int CH = 2 * PAIRS;
   __shared__ float2 z[PAIRS * STRIDE];
   __shared__ float2 tw[N];
   __shared__ int offs[CH];
+  __shared__ float gsc[CH];
   const int tid = threadIdx.x;
   for (int i = tid; i < N; i += 256) tw[i] = tw_g[i];
   const int nbf = info->nbf;
@@ -453,26 +575,31 @@ k_gather_rowfft(const uint16_t* __restrict__ data, int D, int det_cols,
     __syncthreads();
     if (tid < CH) {
       const int b = b0 + tid;
-      offs[tid] = (b < nbf) ? bf_rows[b] * det_cols + bf_cols[b] : -1;
+      const int o = (b < nbf) ? bf_rows[b] * det_cols + bf_cols[b] : -1;
+      offs[tid] = o;
+      gsc[tid] = (gain && o >= 0) ? gain[o] : 1.0f;
     }
     __syncthreads();
     // gather: lane -> BF pixel within the chunk, lane group -> frame; all
-    // of a thread's loads are issued before any is consumed
+    // of a thread's loads are issued before any is consumed.  Samples become
+    // float32 (astype(complex64)) and are scaled by the pixel's gain
+    // (backend.py:570-576; a unit gain leaves them unchanged).
     {
       constexpr int FR = 256 / CH;
       constexpr int NL = N / FR;
       const int j = tid % CH;
       const int off = offs[j];
+      const float gj = gsc[j];
       const int c0 = tid / CH;
       float* zf = reinterpret_cast<float*>(z);
-      const uint16_t* base = data + (long long)rr * N * D + (off >= 0 ? off : 0);
-      unsigned short raw[NL];
+      const T* base = data + (long long)rr * N * D + (off >= 0 ? off : 0);
+      T raw[NL];
 #pragma unroll
       for (int m = 0; m < NL; ++m) raw[m] = base[(long long)(c0 + m * FR) * D];
 #pragma unroll
       for (int m = 0; m < NL; ++m) {
         const int c = c0 + m * FR;
-        zf[2 * ((j >> 1) * STRIDE + c) + (j & 1)] = (off >= 0) ? (float)raw[m] : 0.0f;
+        zf[2 * ((j >> 1) * STRIDE + c) + (j & 1)] = (off >= 0) ? (float)raw[m] * gj : 0.0f;
       }
     }
     __syncthreads();
@@ -511,9 +638,10 @@ k_gather_rowfft(const uint16_t* __restrict__ data, int D, int det_cols,
 // Frame ranges of the stream pass: about target_wg workgroups in all (the
 // default 1536 = 6 per CU was measured best for a 2 GiB stream on MI355X;
 // the tuner may choose another), at most max_ranges, and at most 65536
-// frames per range so the uint32 partial sums cannot overflow.
-static void range_split(long long frames, int D, int max_ranges, int target_wg, int& ranges, int& fpr) {
-  const long long tiles = (D % 8 == 0) ? (D / 8 + 255) / 256 : (D + 255) / 256;
+// frames per range so the 32-bit partial sums cannot overflow.  `vec` is
+// the pixels per thread of the stream kernel.
+static void range_split(long long frames, int D, int vec, int max_ranges, int target_wg, int& ranges, int& fpr) {
+  const long long tiles = (D / vec + 255) / 256;
   long long r = (target_wg > 0 ? target_wg : 1536) / (tiles > 0 ? tiles : 1);
   const long long min_r = (frames + 65535) / 65536;
   if (r < min_r) r = min_r;
@@ -524,37 +652,101 @@ static void range_split(long long frames, int D, int max_ranges, int target_wg, 
   ranges = (int)((frames + fpr - 1) / fpr);
 }
 
-hipError_t launch_mean_dp_stream(const uint16_t* data, long long frames, int D, uint32_t* partial, int max_ranges,
-                                 int target_wg, hipStream_t s) {
-  int ranges, fpr;
-  range_split(frames, D, max_ranges, target_wg, ranges, fpr);
-  const bool vec = (D % 8 == 0) && ((reinterpret_cast<uintptr_t>(data) & 15u) == 0);
-  if (vec) {
-    dim3 grid((D / 8 + 255) / 256, ranges);
-    hipLaunchKernelGGL(k_dp_partial<8>, grid, dim3(256), 0, s, data, frames, D, fpr, partial);
+int dtype_size(int dtype) {
+  switch (dtype) {
+    case QSSB_DTYPE_U8: case QSSB_DTYPE_I8: return 1;
+    case QSSB_DTYPE_U16: case QSSB_DTYPE_I16: return 2;
+    case QSSB_DTYPE_U32: case QSSB_DTYPE_I32: case QSSB_DTYPE_F32: return 4;
+    default: return 0;
+  }
+}
+
+bool dtype_wide_partials(int dtype) {
+  return dtype == QSSB_DTYPE_U32 || dtype == QSSB_DTYPE_I32 || dtype == QSSB_DTYPE_F32;
+}
+
+// pixels per thread of the stream pass for this dtype, detector and base address
+static int stream_vec(int dtype, int D, const void* data) {
+  const int es = dtype_size(dtype);
+  const int v = 16 / es;
+  const bool aligned = (reinterpret_cast<uintptr_t>(data) & 15u) == 0;
+  return (D % v == 0 && aligned) ? v : 1;
+}
+
+template <typename T>
+static void launch_partial_t(const void* data, long long frames, int D, int vec, int ranges, int fpr, void* partial,
+                             hipStream_t s) {
+  typedef typename DpElem<T>::P P;
+  constexpr int V = 16 / sizeof(T);
+  if (vec == V) {
+    dim3 grid((D / V + 255) / 256, ranges);
+    hipLaunchKernelGGL((k_dp_partial_t<T, V>), grid, dim3(256), 0, s, reinterpret_cast<const T*>(data), frames, D,
+                       fpr, reinterpret_cast<P*>(partial));
   } else {
     dim3 grid((D + 255) / 256, ranges);
-    hipLaunchKernelGGL(k_dp_partial<1>, grid, dim3(256), 0, s, data, frames, D, fpr, partial);
+    hipLaunchKernelGGL((k_dp_partial_t<T, 1>), grid, dim3(256), 0, s, reinterpret_cast<const T*>(data), frames, D,
+                       fpr, reinterpret_cast<P*>(partial));
+  }
+}
+
+hipError_t launch_mean_dp_stream(int dtype, const void* data, long long frames, int D, void* partial, int max_ranges,
+                                 int target_wg, hipStream_t s) {
+  const int vec = stream_vec(dtype, D, data);
+  int ranges, fpr;
+  range_split(frames, D, vec, max_ranges, target_wg, ranges, fpr);
+  switch (dtype) {
+    case QSSB_DTYPE_U16:
+      if (vec == 8) {
+        dim3 grid((D / 8 + 255) / 256, ranges);
+        hipLaunchKernelGGL(k_dp_partial<8>, grid, dim3(256), 0, s, reinterpret_cast<const uint16_t*>(data), frames,
+                           D, fpr, reinterpret_cast<uint32_t*>(partial));
+      } else {
+        dim3 grid((D + 255) / 256, ranges);
+        hipLaunchKernelGGL(k_dp_partial<1>, grid, dim3(256), 0, s, reinterpret_cast<const uint16_t*>(data), frames,
+                           D, fpr, reinterpret_cast<uint32_t*>(partial));
+      }
+      break;
+    case QSSB_DTYPE_U8: launch_partial_t<uint8_t>(data, frames, D, vec, ranges, fpr, partial, s); break;
+    case QSSB_DTYPE_I8: launch_partial_t<int8_t>(data, frames, D, vec, ranges, fpr, partial, s); break;
+    case QSSB_DTYPE_I16: launch_partial_t<int16_t>(data, frames, D, vec, ranges, fpr, partial, s); break;
+    case QSSB_DTYPE_U32: launch_partial_t<uint32_t>(data, frames, D, vec, ranges, fpr, partial, s); break;
+    case QSSB_DTYPE_I32: launch_partial_t<int32_t>(data, frames, D, vec, ranges, fpr, partial, s); break;
+    case QSSB_DTYPE_F32: launch_partial_t<float>(data, frames, D, vec, ranges, fpr, partial, s); break;
+    default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_mean_dp_reduce(long long frames, int D, const uint32_t* partial, int max_ranges, int target_wg,
-                                 unsigned long long* sums, float* mean, hipStream_t s) {
+hipError_t launch_mean_dp_reduce(int dtype, const void* data, long long frames, int D, const void* partial,
+                                 int max_ranges, int target_wg, const float* gain, unsigned long long* sums,
+                                 float* mean, float* dcpix, hipStream_t s) {
+  const int vec = stream_vec(dtype, D, data);
   int ranges, fpr;
-  range_split(frames, D, max_ranges, target_wg, ranges, fpr);
+  range_split(frames, D, vec, max_ranges, target_wg, ranges, fpr);
   // 16 range lanes per pixel column: 1024-thread workgroups keep enough
   // loads in flight for a reduction that has only D / 64 workgroups
-  hipLaunchKernelGGL((k_dp_reduce<16>), dim3((D + 63) / 64), dim3(1024), 0, s, partial, ranges, D, frames, sums,
-                     mean);
+  const dim3 grid((D + 63) / 64), block(1024);
+#define QSSB_REDUCE(P, SG)                                                                                  \
+  hipLaunchKernelGGL((k_dp_reduce<16, P, SG>), grid, block, 0, s, reinterpret_cast<const P*>(partial), ranges, \
+                     D, frames, gain, sums, mean, dcpix)
+  switch (dtype) {
+    case QSSB_DTYPE_U8: case QSSB_DTYPE_U16: QSSB_REDUCE(uint32_t, false); break;
+    case QSSB_DTYPE_I8: case QSSB_DTYPE_I16: QSSB_REDUCE(int32_t, true); break;
+    case QSSB_DTYPE_U32: QSSB_REDUCE(unsigned long long, false); break;
+    case QSSB_DTYPE_I32: QSSB_REDUCE(unsigned long long, true); break;
+    case QSSB_DTYPE_F32: QSSB_REDUCE(float2, false); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef QSSB_REDUCE
   return hipGetLastError();
 }
 
-hipError_t launch_mean_dp(const uint16_t* data, long long frames, int D, uint32_t* partial, int max_ranges,
-                          int target_wg, unsigned long long* sums, float* mean, hipStream_t s) {
-  hipError_t e = launch_mean_dp_stream(data, frames, D, partial, max_ranges, target_wg, s);
+hipError_t launch_mean_dp(int dtype, const void* data, long long frames, int D, void* partial, int max_ranges,
+                          int target_wg, const float* gain, unsigned long long* sums, float* mean, float* dcpix,
+                          hipStream_t s) {
+  hipError_t e = launch_mean_dp_stream(dtype, data, frames, D, partial, max_ranges, target_wg, s);
   if (e != hipSuccess) return e;
-  return launch_mean_dp_reduce(frames, D, partial, max_ranges, target_wg, sums, mean, s);
+  return launch_mean_dp_reduce(dtype, data, frames, D, partial, max_ranges, target_wg, gain, sums, mean, dcpix, s);
 }
 
 // ------------------------------------------------------------------------
@@ -696,15 +888,15 @@ hipError_t launch_detect_bf_radius(const float* mean, int R, int C, df ratio, in
   return hipGetLastError();
 }
 
-hipError_t launch_select_bf(const float* mean, const unsigned long long* sums, int det_rows, int det_cols,
-                            float threshold, float bf_radius, int max_bf, int* rows, int* cols,
-                            BfInfo* info, hipStream_t s) {
+hipError_t launch_select_bf(const float* mean, const unsigned long long* sums, const float* dcpix, bool exact_dc,
+                            int det_rows, int det_cols, float threshold, float bf_radius, int max_bf, int* rows,
+                            int* cols, BfInfo* info, hipStream_t s) {
   if (det_rows * det_cols <= SELECT_CACHE_PIX)
-    hipLaunchKernelGGL(k_select_bf<true>, dim3(1), dim3(MASK_THREADS), 0, s, mean, sums, det_rows, det_cols,
-                       threshold, bf_radius, max_bf, rows, cols, info);
+    hipLaunchKernelGGL(k_select_bf<true>, dim3(1), dim3(MASK_THREADS), 0, s, mean, sums, dcpix, (int)exact_dc,
+                       det_rows, det_cols, threshold, bf_radius, max_bf, rows, cols, info);
   else
-    hipLaunchKernelGGL(k_select_bf<false>, dim3(1), dim3(MASK_THREADS), 0, s, mean, sums, det_rows, det_cols,
-                       threshold, bf_radius, max_bf, rows, cols, info);
+    hipLaunchKernelGGL(k_select_bf<false>, dim3(1), dim3(MASK_THREADS), 0, s, mean, sums, dcpix, (int)exact_dc,
+                       det_rows, det_cols, threshold, bf_radius, max_bf, rows, cols, info);
   return hipGetLastError();
 }
 
@@ -728,9 +920,9 @@ int gather_pairs_candidates(int N, int* out) {
   }
 }
 
-hipError_t launch_gather_rowfft(int N, const uint16_t* data, int D, int det_cols, const int* rows,
+hipError_t launch_gather_rowfft(int N, int dtype, const void* data, int D, int det_cols, const int* rows,
                                 const int* cols, const BfInfo* info, int nbf_hint, const float2* tw, int pairs,
-                                float2* H, hipStream_t s) {
+                                const float* gain, float2* H, hipStream_t s) {
   if (pairs <= 0) pairs = gather_pairs_default(N);
   auto grid_for = [&](int ch) {
     long long items = (long long)((nbf_hint + ch - 1) / ch) * N;
@@ -738,22 +930,42 @@ hipError_t launch_gather_rowfft(int N, const uint16_t* data, int D, int det_cols
     if (items > 16384) items = 16384;
     return dim3((unsigned)items);
   };
-#define QSSB_GATHER(NN, PP)                                                                             \
-  if (N == NN && pairs == PP) {                                                                         \
-    hipLaunchKernelGGL((k_gather_rowfft<NN, PP>), grid_for(2 * PP), dim3(256), 0, s, data, D, det_cols, \
-                       rows, cols, info, tw, H);                                                        \
-    return hipGetLastError();                                                                           \
+#define QSSB_GATHER(NN, PP, T)                                                                              \
+  if (N == NN && pairs == PP) {                                                                             \
+    hipLaunchKernelGGL((k_gather_rowfft<NN, PP, T>), grid_for(2 * PP), dim3(256), 0, s,                     \
+                       reinterpret_cast<const T*>(data), D, det_cols, rows, cols, info, tw, gain, H);        \
+    return hipGetLastError();                                                                               \
   }
-  QSSB_GATHER(128, 16)
-  QSSB_GATHER(128, 8)
-  QSSB_GATHER(256, 8)
-  QSSB_GATHER(256, 16)
-  QSSB_GATHER(512, 8)
-  QSSB_GATHER(512, 4)
-  QSSB_GATHER(1024, 4)
-  QSSB_GATHER(1024, 2)
-#undef QSSB_GATHER
+  if (dtype == QSSB_DTYPE_U16) {
+    QSSB_GATHER(128, 16, uint16_t)
+    QSSB_GATHER(128, 8, uint16_t)
+    QSSB_GATHER(256, 8, uint16_t)
+    QSSB_GATHER(256, 16, uint16_t)
+    QSSB_GATHER(512, 8, uint16_t)
+    QSSB_GATHER(512, 4, uint16_t)
+    QSSB_GATHER(1024, 4, uint16_t)
+    QSSB_GATHER(1024, 2, uint16_t)
+    return hipErrorInvalidValue;
+  }
+  // other sample types: the default workgroup shape of each scan size
+  pairs = gather_pairs_default(N);
+#define QSSB_GATHER_T(T)        \
+  QSSB_GATHER(128, 16, T)       \
+  QSSB_GATHER(256, 8, T)        \
+  QSSB_GATHER(512, 8, T)        \
+  QSSB_GATHER(1024, 4, T)       \
   return hipErrorInvalidValue;
+  switch (dtype) {
+    case QSSB_DTYPE_U8: { QSSB_GATHER_T(uint8_t) }
+    case QSSB_DTYPE_I8: { QSSB_GATHER_T(int8_t) }
+    case QSSB_DTYPE_I16: { QSSB_GATHER_T(int16_t) }
+    case QSSB_DTYPE_U32: { QSSB_GATHER_T(uint32_t) }
+    case QSSB_DTYPE_I32: { QSSB_GATHER_T(int32_t) }
+    case QSSB_DTYPE_F32: { QSSB_GATHER_T(float) }
+    default: return hipErrorInvalidValue;
+  }
+#undef QSSB_GATHER_T
+#undef QSSB_GATHER
 }
 
 // ------------------------------------------------------------------------
@@ -845,6 +1057,154 @@ int normalize_partial_groups(int H, int T) {
   return (h + kWindowRows - 1) / kWindowRows;
 }
 
+// Scan normalisation of the other sample types: per 64-row group int64
+// sums (integers) or double-float sums (float32) of the window, then the
+// pad frame sum / count truncated toward zero (the float64 quotient of an
+// int64 sum below 2^53 truncates to the same integer) or rounded to float32.
+template <typename T> struct WinSum { typedef long long S; };
+template <> struct WinSum<float> { typedef float2 S; };
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_window_partial_t(const T* __restrict__ src, int W, int D, int r0, int c0, int h, int w,
+                   typename WinSum<T>::S* __restrict__ partial) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= D) return;
+  const int ra = blockIdx.y * kWindowRows;
+  const int rb = min(h, ra + kWindowRows);
+  if constexpr (std::is_same<T, float>::value) {
+    float hi = 0.0f, lo = 0.0f;
+    for (int r = ra; r < rb; ++r) {
+      const T* row = src + ((long long)(r0 + r) * W + c0) * D + p;
+      for (int c = 0; c < w; ++c) {
+        const df t = two_sum(hi, row[(long long)c * D]);
+        hi = t.hi;
+        lo += t.lo;
+      }
+    }
+    partial[(long long)blockIdx.y * D + p] = make_float2(hi, lo);
+  } else {
+    long long acc = 0;
+    for (int r = ra; r < rb; ++r) {
+      const T* row = src + ((long long)(r0 + r) * W + c0) * D + p;
+      for (int c = 0; c < w; ++c) acc += (long long)row[(long long)c * D];
+    }
+    partial[(long long)blockIdx.y * D + p] = acc;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_window_mean_t(const typename WinSum<T>::S* __restrict__ partial, int groups, int D, long long count,
+                T* __restrict__ mean) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= D) return;
+  if constexpr (std::is_same<T, float>::value) {
+    df s{0.0f, 0.0f};
+    for (int g = 0; g < groups; ++g) {
+      const float2 v = partial[(long long)g * D + p];
+      s = df_add(s, df{v.x, v.y});
+    }
+    const df q = df_div(s, df{(float)count, (float)(count - (long long)(float)count)});
+    mean[p] = q.hi + q.lo;
+  } else {
+    long long s = 0;
+    for (int g = 0; g < groups; ++g) s += partial[(long long)g * D + p];
+    mean[p] = (T)(s / count);
+  }
+}
+
+// Copy (or pad) one frame of `fbytes` bytes per grid row, in 16-byte words
+// when every frame is 16-byte aligned, else in elements of `es` bytes.
+template <int ES>
+__global__ void __launch_bounds__(256)
+k_pad_copy_bytes(const uint8_t* __restrict__ src, int W, long long fbytes, int r0, int c0, int h, int w, int T,
+                 int pad_top, int pad_left, const uint8_t* __restrict__ mean, uint8_t* __restrict__ dst) {
+  const long long frame = blockIdx.y + (long long)blockIdx.z * 65535;
+  if (frame >= (long long)T * T) return;
+  const int i = (int)(frame / T), j = (int)(frame % T);
+  const int si = i - pad_top, sj = j - pad_left;
+  const bool inside = si >= 0 && si < h && sj >= 0 && sj < w;
+  const uint8_t* from = inside ? src + ((long long)(r0 + si) * W + (c0 + sj)) * fbytes : mean;
+  uint8_t* to = dst + frame * fbytes;
+  const long long e = ((long long)blockIdx.x * 256 + threadIdx.x) * ES;
+  if (e >= fbytes) return;
+  if constexpr (ES == 16) {
+    *reinterpret_cast<u32x4*>(to + e) = *reinterpret_cast<const u32x4*>(from + e);
+  } else if constexpr (ES == 4) {
+    *reinterpret_cast<uint32_t*>(to + e) = *reinterpret_cast<const uint32_t*>(from + e);
+  } else if constexpr (ES == 2) {
+    *reinterpret_cast<uint16_t*>(to + e) = *reinterpret_cast<const uint16_t*>(from + e);
+  } else {
+    to[e] = from[e];
+  }
+}
+
+template <typename T>
+static hipError_t normalize_typed(const void* src_v, int H, int W, int D, void* dst_v, int T_, void* partial_v,
+                                  void* mean_v, hipStream_t s) {
+  typedef typename WinSum<T>::S S;
+  const int r0 = H > T_ ? (H - T_) / 2 : 0;
+  const int c0 = W > T_ ? (W - T_) / 2 : 0;
+  const int h = H < T_ ? H : T_;
+  const int w = W < T_ ? W : T_;
+  const int pad_top = (T_ - h) / 2, pad_left = (T_ - w) / 2;
+  const T* src = reinterpret_cast<const T*>(src_v);
+  T* mean = reinterpret_cast<T*>(mean_v);
+  if (h < T_ || w < T_) {
+    const int groups = (h + kWindowRows - 1) / kWindowRows;
+    S* partial = reinterpret_cast<S*>(partial_v);
+    hipLaunchKernelGGL(k_window_partial_t<T>, dim3((D + 255) / 256, groups), dim3(256), 0, s, src, W, D, r0, c0, h,
+                       w, partial);
+    hipLaunchKernelGGL(k_window_mean_t<T>, dim3((D + 255) / 256), dim3(256), 0, s, partial, groups, D,
+                       (long long)h * w, mean);
+  }
+  const long long frames = (long long)T_ * T_;
+  const long long fbytes = (long long)D * sizeof(T);
+  dim3 grid(1, (unsigned)(frames < 65535 ? frames : 65535), (unsigned)((frames + 65534) / 65535));
+  auto al = [](const void* q, int a) { return (reinterpret_cast<uintptr_t>(q) % a) == 0; };
+  const uint8_t* sb = reinterpret_cast<const uint8_t*>(src_v);
+  const uint8_t* mb = reinterpret_cast<const uint8_t*>(mean_v);
+  uint8_t* db = reinterpret_cast<uint8_t*>(dst_v);
+  if (fbytes % 16 == 0 && al(sb, 16) && al(db, 16) && al(mb, 16)) {
+    grid.x = (unsigned)((fbytes / 16 + 255) / 256);
+    hipLaunchKernelGGL(k_pad_copy_bytes<16>, grid, dim3(256), 0, s, sb, W, fbytes, r0, c0, h, w, T_, pad_top,
+                       pad_left, mb, db);
+  } else {
+    constexpr int ES = sizeof(T);
+    grid.x = (unsigned)((fbytes / ES + 255) / 256);
+    hipLaunchKernelGGL(k_pad_copy_bytes<ES>, grid, dim3(256), 0, s, sb, W, fbytes, r0, c0, h, w, T_, pad_top,
+                       pad_left, mb, db);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_normalize_scan_typed(int dtype, const void* src, int H, int W, int D, void* dst, int T,
+                                       void* partial, void* mean, hipStream_t s) {
+  switch (dtype) {
+    case QSSB_DTYPE_U16:
+      return launch_normalize_scan(reinterpret_cast<const uint16_t*>(src), H, W, D, reinterpret_cast<uint16_t*>(dst),
+                                   T, reinterpret_cast<uint32_t*>(partial), reinterpret_cast<uint16_t*>(mean), s);
+    case QSSB_DTYPE_U8: return normalize_typed<uint8_t>(src, H, W, D, dst, T, partial, mean, s);
+    case QSSB_DTYPE_I8: return normalize_typed<int8_t>(src, H, W, D, dst, T, partial, mean, s);
+    case QSSB_DTYPE_I16: return normalize_typed<int16_t>(src, H, W, D, dst, T, partial, mean, s);
+    case QSSB_DTYPE_U32: return normalize_typed<uint32_t>(src, H, W, D, dst, T, partial, mean, s);
+    case QSSB_DTYPE_I32: return normalize_typed<int32_t>(src, H, W, D, dst, T, partial, mean, s);
+    case QSSB_DTYPE_F32: return normalize_typed<float>(src, H, W, D, dst, T, partial, mean, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// bytes of the partial sums and of the pad frame of launch_normalize_scan_typed
+size_t normalize_workspace_bytes(int dtype, int H, int T, int D, size_t* mean_offset) {
+  const int groups = normalize_partial_groups(H, T);
+  const size_t per = dtype == QSSB_DTYPE_U16 ? 4 : 8;
+  size_t part = (size_t)groups * D * per;
+  part = (part + 255) & ~(size_t)255;
+  if (mean_offset) *mean_offset = part;
+  return part + (size_t)D * dtype_size(dtype) + 256;
+}
+
 // ------------------------------------------------------------------------
 // Exact BF columns, detector-major (SSBEngine.write_exact_bf_source,
 // cuda/engine.py:577-701): out[b][s] = data[s][bf_rows[b]][bf_cols[b]] for
@@ -853,11 +1213,11 @@ int normalize_partial_groups(int H, int T) {
 // (BF, scan) passes through LDS so both the frame reads and the column
 // writes are contiguous runs.
 // ------------------------------------------------------------------------
-template <typename OUT>
+template <typename IN, typename OUT>
 __global__ void __launch_bounds__(256)
-k_bf_columns(const uint16_t* __restrict__ data, long long frames, int D, int det_cols, const int* __restrict__ rows,
+k_bf_columns(const IN* __restrict__ data, long long frames, int D, int det_cols, const int* __restrict__ rows,
              const int* __restrict__ cols, int bf_offset, int count, OUT* __restrict__ out) {
-  __shared__ uint16_t tile[16][257];
+  __shared__ IN tile[16][257];
   __shared__ int lin[16];
   const int tid = threadIdx.x;
   const int b0 = blockIdx.y * 16;
@@ -904,15 +1264,21 @@ k_max_final(const unsigned int* __restrict__ partial, int parts, unsigned int* _
   if (threadIdx.x == 0) out[0] = max(max(sm[0], sm[1]), max(sm[2], sm[3]));
 }
 
-hipError_t launch_bf_columns(const uint16_t* data, long long frames, int D, int det_cols, const int* rows,
+hipError_t launch_bf_columns(const void* data, bool in_u8, long long frames, int D, int det_cols, const int* rows,
                              const int* cols, int bf_offset, int count, bool narrow, void* out, hipStream_t s) {
   const dim3 grid((unsigned)((frames + 255) / 256), (unsigned)((count + 15) / 16));
-  if (narrow)
-    hipLaunchKernelGGL(k_bf_columns<uint8_t>, grid, dim3(256), 0, s, data, frames, D, det_cols, rows, cols,
-                       bf_offset, count, reinterpret_cast<uint8_t*>(out));
+  if (in_u8)
+    hipLaunchKernelGGL((k_bf_columns<uint8_t, uint8_t>), grid, dim3(256), 0, s,
+                       reinterpret_cast<const uint8_t*>(data), frames, D, det_cols, rows, cols, bf_offset, count,
+                       reinterpret_cast<uint8_t*>(out));
+  else if (narrow)
+    hipLaunchKernelGGL((k_bf_columns<uint16_t, uint8_t>), grid, dim3(256), 0, s,
+                       reinterpret_cast<const uint16_t*>(data), frames, D, det_cols, rows, cols, bf_offset, count,
+                       reinterpret_cast<uint8_t*>(out));
   else
-    hipLaunchKernelGGL(k_bf_columns<uint16_t>, grid, dim3(256), 0, s, data, frames, D, det_cols, rows, cols,
-                       bf_offset, count, reinterpret_cast<uint16_t*>(out));
+    hipLaunchKernelGGL((k_bf_columns<uint16_t, uint16_t>), grid, dim3(256), 0, s,
+                       reinterpret_cast<const uint16_t*>(data), frames, D, det_cols, rows, cols, bf_offset, count,
+                       reinterpret_cast<uint16_t*>(out));
   return hipGetLastError();
 }
 

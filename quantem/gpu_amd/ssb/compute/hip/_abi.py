@@ -76,6 +76,14 @@ SIGNATURES: dict[str, list] = {
     "ssb_detect_bf_radius": [_vp, _c_double, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int),
                              ctypes.POINTER(_c_int), _vp],
     "ssb_normalize_scan_u16": [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _vp],
+    "ssb_dtype_size": [_c_int],
+    "ssb_mean_dp_typed": [_vp, _vp, _c_int, _i64, _vp, _vp, _vp],
+    "ssb_set_detector_gain": [_vp, _vp, _vp],
+    "ssb_prepare_g_typed": [_vp, _vp, _c_int, _vp],
+    "ssb_recon_typed": [_vp, _vp, _c_int, _c_double, _c_double, _c_double, _c_double, _c_float, _c_float, _vp,
+                        _vp, _vp],
+    "ssb_normalize_scan_typed": [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _vp],
+    "ssb_bf_columns_u8": [_vp, _vp, _c_int, _c_int, _vp, _vp],
 }
 
 _lib: ctypes.CDLL | None = None

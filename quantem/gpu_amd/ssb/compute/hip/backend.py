@@ -101,13 +101,10 @@ class HipSSBBackend:
             raise ValueError(
                 "gqk_storage='full' was removed from the SSB runtime path. Use the default exact "
                 "Hermitian storage; low-level tests may construct canonical full-plane references directly.")
-        if detector_gain is not None:
-            raise NotImplementedError("detector_gain is not supported by the hip SSB backend yet.")
-
         dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         if dev.index is None:
             dev = torch.device("cuda", torch.cuda.current_device())
-        data = self._as_device_u16(data, dev)
+        data = self._as_device_data(data, dev)
         if data.ndim == 3:
             if scan_shape is None:
                 n = int(data.shape[0])
@@ -121,6 +118,13 @@ class HipSSBBackend:
             data = data.reshape(scan_shape[0], scan_shape[1], data.shape[1], data.shape[2])
         elif data.ndim != 4:
             raise ValueError("data must be 3D or 4D.")
+        gain = None
+        if detector_gain is not None:
+            gain = torch.as_tensor(np.asarray(detector_gain.cpu() if isinstance(detector_gain, torch.Tensor)
+                                              else detector_gain, dtype=np.float32)).to(dev)
+            if tuple(gain.shape) != tuple(data.shape[2:]):
+                raise ValueError("detector_gain shape must match detector shape; "
+                                 f"got {tuple(gain.shape)}, expected {tuple(data.shape[2:])}.")
         self._source_shape = tuple(int(v) for v in data.shape)
         data = normalize_scan(data.contiguous())
 
@@ -134,7 +138,7 @@ class HipSSBBackend:
                                       "use 128, 256, 512 or 1024")
         self._detected_bf_radius: float | None = None
         if det_sampling is None:
-            radius = self._detect_radius(data, dev, det_gpts, scan_gpts, wavelength)
+            radius = self._detect_radius(data, dev, det_gpts, scan_gpts, wavelength, gain)
             self._detected_bf_radius = float(radius)
             det_sampling = (2 * semiangle) / radius
             det_sampling = (det_sampling, det_sampling)
@@ -164,6 +168,9 @@ class HipSSBBackend:
                         self.scan_sampling[0], self.scan_sampling[1])
         self._plan = SSBPlan(spec, device=dev)
         self._engine = HipSSBEngine(self._plan)
+        self.detector_gain = gain
+        if gain is not None:
+            self._engine.set_detector_gain(gain)
         self._engine.mean_dp(data)
         bf = self._engine.select_bf(self.bf_intensity_threshold, bf_radius)
         self._engine.prepare_g(data)
@@ -191,22 +198,34 @@ class HipSSBBackend:
 
     # ------------------------------------------------------------- helpers
     @staticmethod
-    def _as_device_u16(data, dev):
+    def _as_device_data(data, dev):
+        """Raw frames on the GPU in a sample type the native library reads:
+        uint8/16/32, int8/16/32 and float32 as they are (the reference keeps
+        the source dtype, ``cuda/backend.py:195-201``); bool as uint8;
+        float16/float64 as float32 (the reference's float64 mean DP then sums
+        the float32-rounded samples)."""
         import torch
-        if isinstance(data, torch.Tensor):
-            if data.dtype != torch.uint16:
-                raise NotImplementedError("the hip SSB backend reads uint16 detector counts; "
-                                          f"got {data.dtype}")
-            return data.to(dev)
-        arr = np.asarray(data)
-        if arr.dtype != np.uint16:
-            raise NotImplementedError(f"the hip SSB backend reads uint16 detector counts; got {arr.dtype}")
-        return torch.from_numpy(np.ascontiguousarray(arr)).to(dev)
+        from .engine import dtype_code
+        if not isinstance(data, torch.Tensor):
+            arr = np.asarray(data)
+            if arr.dtype == np.bool_:
+                arr = arr.astype(np.uint8)
+            elif arr.dtype in (np.float16, np.float64):
+                arr = arr.astype(np.float32)
+            data = torch.from_numpy(np.ascontiguousarray(arr))
+        elif data.dtype == torch.bool:
+            data = data.to(torch.uint8)
+        elif data.dtype in (torch.float16, torch.bfloat16, torch.float64):
+            data = data.to(dev).to(torch.float32)
+        dtype_code(data.dtype)
+        return data.to(dev)
 
     @staticmethod
-    def _detect_radius(data, dev, det_gpts, scan_gpts, wavelength) -> int:
+    def _detect_radius(data, dev, det_gpts, scan_gpts, wavelength, gain=None) -> int:
         spec = PlanSpec(scan_gpts[0], det_gpts[0], det_gpts[1], 1, wavelength, 1e-3, 1e-3, 1e-3, 1.0, 1.0)
         eng = HipSSBEngine(SSBPlan(spec, device=dev))
+        if gain is not None:
+            eng.set_detector_gain(gain)
         eng.mean_dp(data)
         (_, _), radius = eng.detect_bf_radius()
         eng.plan.close()
@@ -424,10 +443,9 @@ class HipSSBBackend:
         if data.ndim != 4 or tuple(int(v) for v in data.shape[:2]) != self.scan_shape:
             raise ValueError("BF-source data must match the engine scan shape; "
                              f"got {tuple(data.shape)}, expected {self.scan_shape} + detector.")
-        if data.dtype == torch.uint8:
-            raise NotImplementedError("uint8 sources are not supported by the hip SSB backend yet.")
         data = data.to(self._plan.device).contiguous()
-        max_value = eng.max_count(data)
+        # uint8 counts always fit the narrow companion; their maximum is metadata only
+        max_value = int(data.max().item()) if data.dtype == torch.uint8 else eng.max_count(data)
         narrow = max_value <= 255
         out_dtype = np.dtype(np.uint8 if narrow else np.uint16)
         path = pathlib.Path(path_stem).with_suffix(".u8" if narrow else ".u16").expanduser().resolve()

@@ -56,18 +56,33 @@ def check_out(tensor, name: str, numel: int, dtype, device) -> None:
         raise ValueError(f"{name} holds {tensor.numel()} elements; {numel} are written")
 
 
+# QSSB_DTYPE_* codes of the sample types the native library reads
+_DTYPE_CODES = {"uint8": 1, "uint16": 2, "uint32": 3, "int8": 4, "int16": 5, "int32": 6, "float32": 7}
+
+
+def dtype_code(dtype) -> int:
+    """QSSB_DTYPE_* code of a torch sample dtype; NotImplementedError for others."""
+    name = str(dtype).replace("torch.", "")
+    if name not in _DTYPE_CODES:
+        raise NotImplementedError(f"the hip SSB backend reads uint8, uint16, uint32, int8, int16, int32 or float32 "
+                                  f"detector frames; got {dtype}")
+    return _DTYPE_CODES[name]
+
+
 def scan_target(rows: int, cols: int) -> int:
     """Supported square scan size for a rows x cols scan (``cuda/backend.py:238-246``)."""
     return int(_abi.lib().ssb_scan_target(int(rows), int(cols)))
 
 
 def normalize_scan(data):
-    """Crop/pad a resident uint16 ``[H, W, k_r, k_c]`` scan to the supported
-    square size with the mean-frame padding of ``cuda/backend.py:229-269``
-    (returns ``data`` itself when it is already supported)."""
+    """Crop/pad a resident ``[H, W, k_r, k_c]`` scan (any supported sample
+    type) to the supported square size with the mean-frame padding of
+    ``cuda/backend.py:229-269`` (returns ``data`` itself when it is already
+    supported)."""
     torch = _torch()
-    if not isinstance(data, torch.Tensor) or data.dtype != torch.uint16 or not data.is_cuda:
-        raise ValueError("data must be a torch.uint16 tensor on the GPU")
+    if not isinstance(data, torch.Tensor) or not data.is_cuda:
+        raise ValueError("data must be a torch tensor on the GPU")
+    code = dtype_code(data.dtype)
     if data.ndim != 4:
         raise ValueError("data must be 4D (scan_row, scan_col, k_row, k_col)")
     H, W, kr, kc = (int(v) for v in data.shape)
@@ -75,10 +90,10 @@ def normalize_scan(data):
     if (H, W) == (T, T):
         return data
     src = data.contiguous()
-    out = torch.empty((T, T, kr, kc), dtype=torch.uint16, device=data.device)
+    out = torch.empty((T, T, kr, kc), dtype=data.dtype, device=data.device)
     with torch.cuda.device(data.device):
         stream = _abi.stream_handle(torch.cuda.current_stream(data.device))
-        _abi.call("ssb_normalize_scan_u16", _abi.ptr(src), H, W, kr, kc, _abi.ptr(out), T, stream)
+        _abi.call("ssb_normalize_scan_typed", _abi.ptr(src), code, H, W, kr, kc, _abi.ptr(out), T, stream)
     return out
 
 
@@ -161,10 +176,12 @@ class HipSSBEngine:
     def _stream(self):
         return _abi.stream_handle(self.torch.cuda.current_stream(self.plan.device))
 
-    def _check_u16(self, data):
+    def _check_data(self, data) -> int:
+        """Validate resident raw frames; returns their QSSB_DTYPE_* code."""
         torch = self.torch
-        if not isinstance(data, torch.Tensor) or data.dtype != torch.uint16:
-            raise ValueError("data must be a torch.uint16 tensor on the GPU")
+        if not isinstance(data, torch.Tensor):
+            raise ValueError("data must be a torch tensor on the GPU")
+        code = dtype_code(data.dtype)
         if not data.is_cuda or data.device != self.plan.device:
             raise ValueError(f"data must live on {self.plan.device}")
         if not data.is_contiguous():
@@ -175,18 +192,44 @@ class HipSSBEngine:
                 f"data must have shape ({s.scan_n}, {s.scan_n}, {s.det_rows}, {s.det_cols}); "
                 f"got {tuple(data.shape)}"
             )
+        return code
+
+    def _check_u16(self, data):
+        if self._check_data(data) != _DTYPE_CODES["uint16"]:
+            raise ValueError("data must be a torch.uint16 tensor on the GPU")
 
     # ------------------------------------------------------------ preparation
     def mean_dp(self, data):
-        """(uint64 sums, float32 mean DP) of resident uint16 data."""
+        """(uint64 sums, float32 mean DP) of resident raw frames; the sums are
+        None for float32 frames (summed in double-float).  The mean carries
+        the detector gain when one is set."""
         torch = self.torch
-        self._check_u16(data)
+        code = self._check_data(data)
         s = self.plan.spec
-        sums = torch.empty((s.det_rows, s.det_cols), dtype=torch.uint64, device=self.plan.device)
+        sums = None
+        if code != _DTYPE_CODES["float32"]:
+            sums = torch.empty((s.det_rows, s.det_cols), dtype=torch.uint64, device=self.plan.device)
         mean = torch.empty((s.det_rows, s.det_cols), dtype=torch.float32, device=self.plan.device)
-        _abi.call("ssb_mean_dp_u16", self.plan.handle, _abi.ptr(data), s.scan_n * s.scan_n,
+        _abi.call("ssb_mean_dp_typed", self.plan.handle, _abi.ptr(data), code, s.scan_n * s.scan_n,
                   _abi.ptr(sums), _abi.ptr(mean), self._stream())
         return sums, mean
+
+    def set_detector_gain(self, gain) -> None:
+        """Per-pixel detector gain ``[det_rows, det_cols]`` (float32; None
+        clears it), applied as ``cuda/backend.py:220-226``, ``442-443``,
+        ``574-576`` apply ``detector_gain``."""
+        torch = self.torch
+        s = self.plan.spec
+        if gain is None:
+            _abi.call("ssb_set_detector_gain", self.plan.handle, None, self._stream())
+            return
+        g = gain if isinstance(gain, torch.Tensor) else torch.as_tensor(np.asarray(gain, dtype=np.float32))
+        g = g.to(device=self.plan.device, dtype=torch.float32).contiguous()
+        if tuple(g.shape) != (s.det_rows, s.det_cols):
+            raise ValueError("detector_gain shape must match detector shape; "
+                             f"got {tuple(g.shape)}, expected {(s.det_rows, s.det_cols)}.")
+        _abi.call("ssb_set_detector_gain", self.plan.handle, _abi.ptr(g), self._stream())
+        self._gain = g             # keep the source alive until the copy has run
 
     def select_bf(self, threshold: float = 0.0, bf_radius: float | None = None) -> BrightFieldState:
         """Device BF selection from the last mean DP; reads the result back."""
@@ -236,15 +279,23 @@ class HipSSBEngine:
         return int(out.value)
 
     def bf_columns(self, data, bf_offset: int, count: int, narrow_u8: bool = False, out=None):
-        """Exact detector-major BF columns ``[count, N*N]`` (uint16, or uint8)."""
+        """Exact detector-major BF columns ``[count, N*N]`` of uint16 frames
+        (uint16, or uint8 when ``narrow_u8``) or uint8 frames (uint8)."""
         torch = self.torch
-        self._check_u16(data)
-        dt = torch.uint8 if narrow_u8 else torch.uint16
+        code = self._check_data(data)
+        if code not in (_DTYPE_CODES["uint8"], _DTYPE_CODES["uint16"]):
+            raise TypeError(f"Exact BF-source export supports uint8 or uint16 counts; got {data.dtype}.")
+        u8_in = code == _DTYPE_CODES["uint8"]
+        dt = torch.uint8 if (narrow_u8 or u8_in) else torch.uint16
         if out is None:
             out = torch.empty((count, self.n * self.n), dtype=dt, device=self.plan.device)
         check_out(out, "out", int(count) * self.n * self.n, dt, self.plan.device)
-        _abi.call("ssb_bf_columns_u16", self.plan.handle, _abi.ptr(data), int(bf_offset), int(count),
-                  1 if narrow_u8 else 0, _abi.ptr(out), self._stream())
+        if u8_in:
+            _abi.call("ssb_bf_columns_u8", self.plan.handle, _abi.ptr(data), int(bf_offset), int(count),
+                      _abi.ptr(out), self._stream())
+        else:
+            _abi.call("ssb_bf_columns_u16", self.plan.handle, _abi.ptr(data), int(bf_offset), int(count),
+                      1 if narrow_u8 else 0, _abi.ptr(out), self._stream())
         return out
 
     def set_bf(self, rows, cols, center, dc) -> None:
@@ -314,8 +365,8 @@ class HipSSBEngine:
         return tuple(out)
 
     def prepare_g(self, data) -> None:
-        self._check_u16(data)
-        _abi.call("ssb_prepare_g_u16", self.plan.handle, _abi.ptr(data), self._stream())
+        code = self._check_data(data)
+        _abi.call("ssb_prepare_g_typed", self.plan.handle, _abi.ptr(data), code, self._stream())
         self._has_g = True
 
     def import_g(self, g_qk) -> None:
@@ -457,18 +508,22 @@ class HipSSBEngine:
     def recon_u16(self, data, c10: float, c12: float, phi12: float, rotation_rad: float = 0.0,
                   threshold: float = 0.0, bf_radius: float | None = None, obj_out=None, phase_out=None,
                   after: "HipSSBEngine | None" = None):
-        """End-to-end plain SSB from resident raw data (no host round trip).
+        """End-to-end plain SSB from resident raw frames of any supported
+        sample type (no host round trip; the name keeps the uint16 fast path
+        it was first written for).
 
         ``after``: another engine whose most recent ``after``-ordered recon's
         raw-data stream pass this one's stream pass must follow (two engines
         on two streams keep two reconstructions in flight)."""
-        self._check_u16(data)
+        code = self._check_data(data)
+        if after is not None and code != _DTYPE_CODES["uint16"]:
+            raise ValueError("recon_u16(after=...) reads uint16 frames")
         nn = self.n * self.n
         check_out(obj_out, "obj_out", nn, self.torch.complex64, self.plan.device)
         check_out(phase_out, "phase_out", nn, self.torch.float32, self.plan.device)
         radius = float(bf_radius) if bf_radius is not None else 0.0
         if after is None:
-            _abi.call("ssb_recon_u16", self.plan.handle, _abi.ptr(data), float(c10), float(c12),
+            _abi.call("ssb_recon_typed", self.plan.handle, _abi.ptr(data), code, float(c10), float(c12),
                       float(phi12), float(rotation_rad), float(threshold), radius,
                       _abi.ptr(obj_out), _abi.ptr(phase_out), self._stream())
         else:

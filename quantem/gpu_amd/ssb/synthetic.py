@@ -163,8 +163,20 @@ def generate(cfg: SyntheticConfig, device: str = "cpu", batch: int = 4096,
         out[s:e] = (amp.real**2 + amp.imag**2) * (cfg.dose / norm)
     gen = torch.Generator(device=dev)
     gen.manual_seed(cfg.seed + 1)
-    counts = torch.poisson(out, generator=gen).clamp_(0, 65535)
-    data = counts.to(torch.int32).to(torch.uint16).reshape(cfg.scan, cfg.scan, n, n)
+    if out.numel() < 2 ** 31:
+        counts = torch.poisson(out, generator=gen).clamp_(0, 65535)
+        data = counts.to(torch.int32).to(torch.uint16).reshape(cfg.scan, cfg.scan, n, n)
+    else:
+        # torch's Poisson sampler launches one grid over the whole tensor;
+        # above 2^31 elements (scan 512 x detector 128) sample in slabs of
+        # frames, in order, from the same generator
+        data = torch.empty((total, n, n), dtype=torch.uint16, device=dev)
+        slab = (2 ** 30) // (n * n)
+        for s in range(0, total, slab):
+            e = min(total, s + slab)
+            data[s:e] = torch.poisson(out[s:e], generator=gen).clamp_(0, 65535).to(torch.int32).to(torch.uint16)
+        del out
+        data = data.reshape(cfg.scan, cfg.scan, n, n)
     if device == "cpu":
         data = data.numpy()
     if return_phase:
