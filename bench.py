@@ -10,8 +10,14 @@ benchmark configuration (300 kV, 20 mrad, 1.25 mrad/px, defocus C10 = 200 A,
 
 Prints one JSON line (see the repository contract): value = recon/s,
 roofline of the dominant kernel (the mean-DP stream) from hipEvents recorded
-inside the timed region, and the CPU baseline (the FP32 NumPy/SciPy
-restatement timed on this host on one full reconstruction).
+inside the timed region, the whole-reconstruction roofline, GPU latencies at
+the R1 (raw frames -> phase) and R2 (resident Fourier stack -> phase)
+boundaries, the batched search legs (phase-image variance and exact
+objective, direct and hipGraph replay, uneven scans in one launch), the fit
+leg (learned start vs plain search), and the CPU baseline: the NumPy/SciPy
+restatement timed on this host after one warm-up run, p50/p95/max of at
+least five runs, R1 and R2, float32 beside float64, on the 64²x64² (padded
+to 128²) CPU-path workload and on this 256²x128² workload.
 """
 from __future__ import annotations
 
@@ -33,6 +39,11 @@ C10 = 200.0                      # 20 nm defocus in the engine's numeric (Angstr
 TRAFFIC_FILE = REPO / "profiles" / "mean_dp_stream_traffic.txt"
 
 
+def _abi_lib():
+    from quantem.gpu_amd.ssb.compute.hip import _abi
+    return _abi.lib()
+
+
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
@@ -50,21 +61,65 @@ def read_traffic() -> float | None:
     return None
 
 
-def cpu_baseline(data_host, cfg) -> dict:
-    """Time the FP32 NumPy/SciPy restatement of the same reconstruction on this host."""
+def percentiles(ms: list[float]) -> dict:
+    """p50 / p95 / max by nearest rank (rank = ceil(p/100 * n)), in ms."""
+    s = sorted(ms)
+    n = len(s)
+
+    def rank(p):
+        return s[max(0, min(n - 1, math.ceil(p / 100.0 * n) - 1))]
+
+    return {"p50_ms": rank(50), "p95_ms": rank(95), "max_ms": s[-1], "runs": n}
+
+
+def _time_runs(fn, runs: int) -> dict:
+    fn()                                            # one untimed warm-up
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append((time.perf_counter() - t0) * 1e3)
+    return percentiles(ts)
+
+
+def cpu_baseline(cases, runs: int = 5) -> dict:
+    """The oracle's NumPy/SciPy restatement of the reconstruction, timed on
+    this host's cores: FP32 (the reference algorithm as shipped, "port") and
+    the FP64 oracle beside it, at R1 (resident u16 frames -> phase: scan
+    normalisation, mean DP, BF selection, gather, scan FFT, corrected Fourier
+    sum, inverse FFT, angle) and R2 (prepared Fourier stack -> phase), one
+    untimed warm-up then ``runs`` timed runs each, p50/p95/max.  ``cases``
+    maps a configuration name to (host u16 frames, SyntheticConfig, C10);
+    ``value`` is FP32 R1 of the configuration measured on the GPU."""
     import numpy as np
     from oracle import ssb_oracle as O
 
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-    t0 = time.perf_counter()
-    p = O.prepare(data_host, voltage_kV=300, semiangle_mrad=cfg.semiangle_mrad,
-                  scan_sampling=cfg.step_A, det_sampling=cfg.det_sampling_mrad, dtype=np.float32)
-    obj = O.object_wave(p.G, p.dc, p.geom, C10, 0.0, 0.0)
-    _ = np.angle(obj)
-    dt = time.perf_counter() - t0
-    return {"value": 1.0 / dt, "unit": "recon/s", "cores": cores, "kind": "port",
-            "sample": f"1 full reconstruction of the same 256x256x128x128 u16 input "
-                      f"({dt:.2f} s; FP32 NumPy/SciPy restatement, scipy.fft workers={cores})"}
+    cores = O._workers()
+    out = {}
+    for name, (host, cfg, c10) in cases.items():
+        row = {}
+        for label, dt in (("fp32", np.float32), ("fp64", np.float64)):
+            def r1(dt=dt):
+                p = O.prepare(host, voltage_kV=300, semiangle_mrad=cfg.semiangle_mrad, scan_sampling=cfg.step_A,
+                              det_sampling=cfg.det_sampling_mrad, dtype=dt)
+                return np.angle(O.object_wave(p.G, p.dc, p.geom, c10, 0.0, 0.0))
+            row[f"R1_{label}"] = _time_runs(r1, runs)
+            p = O.prepare(host, voltage_kV=300, semiangle_mrad=cfg.semiangle_mrad, scan_sampling=cfg.step_A,
+                          det_sampling=cfg.det_sampling_mrad, dtype=dt)
+            row[f"R2_{label}"] = _time_runs(lambda p=p: np.angle(O.object_wave(p.G, p.dc, p.geom, c10, 0.0, 0.0)),
+                                            runs)
+            row["n_bf"] = int(p.bf.rows.size)
+            del p
+            log(f"CPU {name} {label}: R1 p50 {row[f'R1_{label}']['p50_ms']:.1f} ms, "
+                f"R2 p50 {row[f'R2_{label}']['p50_ms']:.1f} ms")
+        out[name] = row
+    main = out["config2"]["R1_fp32"]
+    return {"value": 1e3 / main["p50_ms"], "unit": "recon/s", "cores": cores, "kind": "port",
+            "sample": f"scan 256x256 x detector 128x128 u16 (the GPU workload) and scan 64x64 x detector 64x64 "
+                      f"(padded to 128x128); R1 and R2, FP32 restatement and FP64 oracle, 1 warm-up + {runs} "
+                      f"timed runs each, scipy.fft and BF chunks on {cores} threads; value = 1/p50 of FP32 R1 "
+                      f"at 256x256",
+            "runs": out}
 
 
 def _phase_of(ssb, state):
@@ -276,6 +331,136 @@ def decode_leg(data, reps: int = 5, distinct: int = 64) -> dict:
             "bit_exact": ok, "binning": binning}
 
 
+def _event_times(fn, n: int, stream) -> list[float]:
+    import torch
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n)]
+    for a, b in evs:
+        a.record(stream)
+        fn()
+        b.record(stream)
+    torch.cuda.synchronize()
+    return [a.elapsed_time(b) for a, b in evs]
+
+
+def gpu_latency(eng, data, step, n: int) -> dict:
+    """Per-call hipEvent latency of R1 (resident u16 frames -> phase, the
+    ``value`` workload) and R2 (prepared resident Fourier stack -> phase, the
+    reference's published boundary), p50/p95/max over ``n`` calls each."""
+    import torch
+    stream = torch.cuda.current_stream()
+    r1 = percentiles(_event_times(step, n, stream))
+    eng.mean_dp(data)
+    eng.select_bf(0.0, None)
+    eng.prepare_g(data)
+    eng.cache_rotation(0.0)
+    phase = torch.empty((SCAN, SCAN), dtype=torch.float32, device=data.device)
+    warm = lambda: eng.reconstruct_object(C10, 0.0, 0.0, phase_out=phase)
+    for _ in range(3):
+        warm()
+    r2 = percentiles(_event_times(warm, n, stream))
+    return {"R1": r1, "R2": r2,
+            "R2_bytes": 8.0 * eng.num_bf * SCAN * (SCAN // 2 + 1) + 4.0 * SCAN * SCAN}
+
+
+def gpu_latency_small(data, cfg, dev, n: int) -> dict:
+    """R1 latency of the scan 64 x 64, detector 64 x 64 configuration (padded
+    to 128 x 128 on the device first, as the backend does)."""
+    import torch
+    from quantem.gpu_amd.ssb.compute.hip.engine import HipSSBEngine, PlanSpec, SSBPlan, normalize_scan
+    padded = normalize_scan(data)
+    spec = PlanSpec(128, 64, 64, 4096, cfg.wavelength, cfg.semiangle_mrad * 1e-3, cfg.det_sampling_mrad * 1e-3,
+                    cfg.det_sampling_mrad * 1e-3, cfg.step_A, cfg.step_A)
+    e = HipSSBEngine(SSBPlan(spec, device=dev))
+    phase = torch.empty((128, 128), dtype=torch.float32, device=dev)
+    fn = lambda: e.recon_u16(normalize_scan(data), 0.0, 0.0, 0.0, 0.0, 0.0, None, phase_out=phase)
+    for _ in range(3):
+        fn()
+    out = {"workload": "scan 64x64 x detector 64x64 u16, zero aberrations, padded to 128x128 (pad included)",
+           "R1": percentiles(_event_times(fn, n, torch.cuda.current_stream()))}
+    e.plan.close()
+    del padded
+    return out
+
+
+def trials_leg(dev, data, cfg, n_trials: int = 512, reps: int = 5, seed: int = 3) -> dict:
+    """Batched aberration search (scan 256 x 256, detector 128 x 128): 512
+    trials with C10 U(+-500 A), C30 U(+-2e5 A), rotation U(+-5 deg), each
+    evaluated with the phase-image-variance objective in one persistent launch
+    replayed from a hipGraph; beside it the same trials with the exact
+    objective (the per-trial kernels, eager and graph-replayed), and the
+    uneven-scan mix (scans 128, 192 -> 256 and 256 at detector 128, plus the
+    benchmark dataset) in one launch.  A search unit is one (trial, BF pixel,
+    Hermitian-half Q) evaluation."""
+    import numpy as np
+    import torch
+    from quantem.gpu_amd.ssb.compute.hip.backend import HipSSBBackend
+    from quantem.gpu_amd.ssb.compute.hip.trials import TrialBatch, params_to_vectors, sobol_design
+    from quantem.gpu_amd.ssb.synthetic import SyntheticConfig, generate
+
+    def backend(d, c):
+        return HipSSBBackend(d, semiangle=c.semiangle_mrad, scan_sampling=c.step_A, det_sampling=c.det_sampling_mrad,
+                             voltage_kV=c.voltage_kV)
+
+    ranges = {"C10_nm": (-500.0, 500.0), "C30": (-2.0e5, 2.0e5), "rotation_angle_deg": (-5.0, 5.0)}
+    params = sobol_design(ranges, n_trials, seed)
+    mags, angs, rot = params_to_vectors(params)
+    main = backend(data, cfg)
+    eng = main._engine
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    tb = TrialBatch([eng], n_trials)
+    tb.set(mags, angs, rot)
+    ms_graph = timed(lambda: tb.phase_variance(use_graph=True))
+    ms_direct = timed(lambda: tb.phase_variance(use_graph=False))
+    var = tb.phase_variance().cpu().numpy()[0]
+    tb.close()
+    nbf = eng.num_bf
+    units = float(n_trials) * nbf * SCAN * (SCAN // 2 + 1)
+    out_exact = torch.empty(n_trials, dtype=torch.float32, device=data.device)
+    ms_exact = timed(lambda: eng.batch_phase_loss(mags, angs, rot, out=out_exact))
+    ms_exact_graph = timed(lambda: eng.batch_phase_loss(mags, angs, rot, out=out_exact, graph=True))
+    best = params[int(np.argmax(var))]
+    # uneven scans in one launch: 128, 192 (padded to 256) and 256 at detector 128
+    extra = []
+    for scan, s_ in ((128, 41), (192, 42)):
+        c = SyntheticConfig(scan=scan, det=DET, aberrations={"C10": 120.0, "C30": -5.0e4}, rotation_deg=1.0,
+                            dose=5e4, seed=s_)
+        extra.append(backend(generate(c, device="cuda"), c))
+    engines = [b._engine for b in extra] + [eng]
+    mix = TrialBatch(engines, n_trials)
+    mix.set(mags, angs, rot)
+    ms_mix = timed(lambda: mix.phase_variance(use_graph=True))
+    mix_units = float(n_trials) * sum(e.num_bf * e.n * (e.n // 2 + 1) for e in engines)
+    mix.close()
+    for b in extra:
+        b.close()
+    main.close()
+    return {
+        "workload": f"{n_trials} trials (C10 U(+-500 A), C30 U(+-2e5 A), rotation U(+-5 deg)), scan 256x256, "
+                    f"detector 128x128, N_bf {nbf}",
+        "phase_variance": {"objective": "phase-image variance var_R angle(O_t) (maximised)",
+                           "ms_per_batch_graph": ms_graph, "ms_per_batch_direct": ms_direct,
+                           "trials_per_s": n_trials / (ms_graph * 1e-3), "units": units,
+                           "units_per_s": units / (ms_graph * 1e-3), "best": best},
+        "exact": {"objective": "mean_R var_b phi_b (the reference's default fit objective)",
+                  "ms_per_batch_eager": ms_exact, "ms_per_batch_graph": ms_exact_graph,
+                  "trials_per_s": n_trials / (ms_exact_graph * 1e-3)},
+        "uneven_mix": {"scans": [128, 256, 256], "source_scans": [128, 192, 256], "ms_per_batch_graph": ms_mix,
+                       "units_per_s": mix_units / (ms_mix * 1e-3)},
+    }
+
+
 def copy_probe() -> int:
     """Five launches of an elementwise copy reading and writing 1 GiB each: the
     reference kernel of known traffic for the FETCH_SIZE/WRITE_SIZE counters."""
@@ -297,6 +482,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true",
                     help="skip the CPU baseline (used for profiler runs)")
+    ap.add_argument("--cpu-runs", type=int, default=5, help="timed CPU-baseline runs after one warm-up")
     ap.add_argument("--fit-sets", type=int, default=5,
                     help="held-out datasets of the aberration-fit comparison (0 skips it)")
     ap.add_argument("--fit-train-steps", type=int, default=4000)
@@ -311,6 +497,9 @@ def main() -> int:
                     help="skip the bitshuffle-LZ4 decode measurement")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch shapes instead of autotuning them before timing")
+    ap.add_argument("--no-trials", action="store_true", help="skip the batched-search leg")
+    ap.add_argument("--trials-only", action="store_true",
+                    help="only the batched-search leg (used for profiler runs)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB float32 copy kernel (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -342,6 +531,9 @@ def main() -> int:
     lam = electron_wavelength_angstrom(300e3)
     spec = PlanSpec(SCAN, DET, DET, 4096, lam, cfg.semiangle_mrad * 1e-3,
                     cfg.det_sampling_mrad * 1e-3, cfg.det_sampling_mrad * 1e-3, cfg.step_A, cfg.step_A)
+    if args.trials_only:
+        print(json.dumps({"trials": trials_leg(dev, data, cfg)}), flush=True)
+        return 0
     eng = HipSSBEngine(SSBPlan(spec, device=dev))
     phase = torch.empty((SCAN, SCAN), dtype=torch.float32, device=dev)
 
@@ -367,8 +559,11 @@ def main() -> int:
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     # timed region: hipEvents only around the dominant kernel (the mean-DP
-    # stream, boundaries 0 and 1 of the recon's stages), on the plan's stream
-    eng.timing_begin(args.steps, boundaries=(0, 1))
+    # stream pass: the boundaries before and after its stage), on the plan's stream
+    lib = _abi_lib()
+    names = [lib.ssb_stage_name(i).decode() for i in range(int(lib.ssb_stage_count()))]
+    k_stream = names.index("mean_dp_stream")
+    eng.timing_begin(args.steps, boundaries=(k_stream, k_stream + 1))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     e0.record(stream)
@@ -424,12 +619,19 @@ def main() -> int:
         "stages_ms_per_recon": {k: v / stages["calls"] for k, v in stages.items() if k != "calls"},
         "tuning": tuning,
     }
+    log("per-call latency: R1 and R2 (warm resident Fourier stack)")
+    result["latency"] = gpu_latency(eng, data, step, args.steps)
+    cfg1 = SyntheticConfig(scan=64, det=64, dose=1e4, seed=64)
+    data1 = generate(cfg1, device="cuda")
+    result["latency"]["config1"] = gpu_latency_small(data1, cfg1, dev, args.steps)
     if not args.no_cpu_baseline:
-        log("timing the CPU baseline (FP32 NumPy/SciPy restatement, one reconstruction)")
-        host = data.cpu().numpy()
-        result["cpu_baseline"] = cpu_baseline(host, cfg)
+        log("timing the CPU baseline (FP32 NumPy/SciPy restatement and FP64 oracle, R1 and R2)")
+        cases = {"config2": (data.cpu().numpy(), cfg, C10), "config1": (data1.cpu().numpy(), cfg1, 0.0)}
+        result["cpu_baseline"] = cpu_baseline(cases, runs=args.cpu_runs)
+        del cases
     else:
         result["cpu_baseline"] = None
+    del data1
     eng.plan.close()
     if not args.no_series:
         log("series of reconstructions, 3 in flight")
@@ -439,6 +641,11 @@ def main() -> int:
     if not args.no_decode:
         log("bitshuffle-LZ4 decode of the same frames")
         result["decode"] = decode_leg(data)
+    if not args.no_trials:
+        log("batched aberration search: 512 trials in one launch")
+        result["trials"] = trials_leg(dev, data, cfg)
+        log(f"trials: phase variance {result['trials']['phase_variance']['ms_per_batch_graph']:.2f} ms / 512, "
+            f"exact {result['trials']['exact']['ms_per_batch_graph']:.2f} ms / 512")
     del data
     if args.fit_sets > 0:
         log("aberration fit: learned start + batched refine vs plain batched search")

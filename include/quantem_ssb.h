@@ -291,6 +291,57 @@ int ssb_batch_phase_loss(ssb_plan* plan, int n_trials, const double* mags_host, 
                          const double* rotation_host, float* losses_out, void* stream);
 
 /*
+ * ssb_batch_phase_loss as a captured hipGraph: the per-trial kernels read
+ * their rotation and aberration coefficients from a plan-owned device table,
+ * so the sequence is captured once (again when the count, the pattern of
+ * higher-order trials or losses_out change) and replayed for each new batch
+ * after the table upload, on the plan's own stream ordered after and before
+ * `stream`.  Losses are bit-identical to ssb_batch_phase_loss (every trial
+ * recomputes its geometry).  Synchronises `stream` once for the upload.
+ * Replaces the per-trial loop of cuda/engine.py:1813-1836.
+ */
+int ssb_batch_phase_loss_graph(ssb_plan* plan, int n_trials, const double* mags_host, const double* angles_host,
+                               const double* rotation_host, float* losses_out, void* stream);
+
+/*
+ * Batched aberration search over one or more datasets: a handle over
+ * n_plans plans (at most 16; each with a resident Hermitian Fourier stack
+ * from ssb_prepare_g_*, isotropic detector sampling, one wavelength) and up
+ * to max_trials trial aberration sets.  Every trial is evaluated on every
+ * dataset in one persistent launch whose work queue spans all datasets
+ * (scan sizes may differ), replacing the reference's per-trial loop
+ * (cuda/engine.py:1813-1836 driven by cuda/optimizer.py:59-191).  The
+ * objective is the phase-image variance var_R angle(O_t(R)) of the
+ * Fourier-sum object O_t = IFFT2(S_t), S_t = mean_b G_b conj(Gamma^_b(theta_t))
+ * with the DC bin set to dc (the object path of cuda/engine.py:1047-1107 per
+ * trial): a contrast measure that the correct aberrations maximise.  It is
+ * not the reference's exact objective (ssb_batch_phase_loss).
+ */
+typedef struct ssb_trials ssb_trials;
+int ssb_trials_create(ssb_plan* const* plans, int n_plans, int max_trials, ssb_trials** out);
+int ssb_trials_destroy(ssb_trials* trials);
+
+/*
+ * Set n_trials aberration sets from host arrays in the layout of
+ * ssb_batch_phase_loss (mags_host[14 t ..], angles_host[14 t ..]; odd-order
+ * terms C21, C23, C41, C43, C45 must be zero) with per-trial scan rotations
+ * rotation_host[t] (rad; NULL = 0).  Uploads them to the device and
+ * synchronises `stream`.
+ */
+int ssb_trials_set(ssb_trials* trials, int n_trials, const double* mags_host, const double* angles_host,
+                   const double* rotation_host, void* stream);
+
+/*
+ * Phase-image variance of every (plan, trial) into out[p * max_trials + t]
+ * (device float32).  use_graph = 0 enqueues the kernels on `stream`;
+ * use_graph = 1 captures them once into a hipGraph (again when n_trials or
+ * `out` change) and replays it on the handle's own stream, ordered after
+ * and before `stream`: new trials need only ssb_trials_set.  Both give the
+ * same bits.
+ */
+int ssb_trials_phase_variance(ssb_trials* trials, float* out, int use_graph, void* stream);
+
+/*
  * End-to-end plain SSB from resident raw data without host round trips:
  * mean DP, bright-field selection, geometry, gather + scan FFT fused with the
  * trotter reduction, inverse FFT and angle.  Equivalent to
@@ -374,7 +425,10 @@ int ssb_bin_u16(const uint16_t* src, long long frames, int rows, int cols, int f
  * QSSB_TUNE_STREAM_WG: workgroups of the raw-data stream pass;
  * QSSB_TUNE_GATHER_PAIRS: complex FFTs (two BF pixels each) per gather
  * workgroup; QSSB_TUNE_TROTTER: tile-shape variant of the fused column FFT +
- * trotter.  ssb_tune_candidates writes the allowed values for a scan size
+ * trotter; QSSB_TUNE_SUPERSET: 1 (default) the stream pass of uint16 frames
+ * also copies a sampled bright-field box of every frame compactly for the
+ * gather, 2 the gather reads the frames.  ssb_tune_candidates writes the
+ * allowed values for a scan size
  * (the first is the default) and returns their count; ssb_tune_set accepts
  * one of them or 0 (default); ssb_tune_get reports the value in use.  The
  * knobs change launch shapes only: the mean DP and BF selection are
@@ -384,6 +438,7 @@ int ssb_bin_u16(const uint16_t* src, long long frames, int rows, int cols, int f
 #define QSSB_TUNE_STREAM_WG 0
 #define QSSB_TUNE_GATHER_PAIRS 1
 #define QSSB_TUNE_TROTTER 2
+#define QSSB_TUNE_SUPERSET 3
 int ssb_tune_candidates(int scan_n, int knob, int* values_host, int capacity);
 int ssb_tune_set(ssb_plan* plan, int knob, int value);
 int ssb_tune_get(ssb_plan* plan, int knob, int* value_host);

@@ -713,6 +713,20 @@ def phase_and_loss_full(G_half, dc, g: Geometry, mags, angles) -> tuple[np.ndarr
     return phase.astype(g.dtype), loss
 
 
+def phase_variance(G_half, dc, g: Geometry, mags, angles) -> float:
+    """Phase-image variance ``var_R angle(O(R))`` of the Fourier-sum object
+    ``O = ifft2(mean_b G_b conj(Gamma^_b))`` (DC set to dc) for a 14-coefficient
+    aberration vector: the object path of ``cuda/engine.py:1047-1107`` with
+    ``gamma_mul_pk_onthefly_full`` (``cuda/kernels/common.py:484-532``),
+    reduced to the contrast measure the batched search maximises (no
+    reference counterpart; the reference's default objective is
+    ``phase_and_loss``'s)."""
+    s = corrected_fourier_mean(G_half, dc, g, 0.0, 0.0, 0.0,
+                               gamma_fn=lambda b0, b1: gamma_hat_block_full(g, b0, b1, mags, angles))
+    ph = np.angle(np.fft.ifft2(s.astype(np.complex128)))
+    return float(np.mean(ph * ph) - np.mean(ph) ** 2)
+
+
 def result_phase(G_half, dc, g: Geometry, aberrations: dict) -> tuple[np.ndarray, str]:
     """Phase returned by ``result()`` (``cuda/backend.py:771-851``).
 

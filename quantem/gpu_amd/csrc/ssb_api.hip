@@ -11,77 +11,13 @@
 
 using namespace qssb;
 
-struct ssb_plan {
-  int n = 0;
-  int det_rows = 0, det_cols = 0, D = 0;
-  int max_bf = 0;
-  int max_groups = 0;
-  int max_ranges = 256;
-  int nbf_hint = 0;
-  int gcols = 0;            // columns of the resident stack (0 = none)
-  int g_nbf = 0;
-  double wavelength = 0, semiangle = 0, ang_y = 0, ang_x = 0;
-  double sampling_row = 0, sampling_col = 0;
-  double recip_y = 0, recip_x = 0;
-  bool geometry_valid = false;
-  double rotation = 0;
-  Optics optics{};
-  // device buffers
-  uint32_t* dp_partial = nullptr;
-  void* dp_partial_wide = nullptr;   // 64-bit partials of 32-bit samples (allocated on first use)
-  unsigned long long* dp_sums = nullptr;
-  float* dp_mean = nullptr;
-  float* dp_dcpix = nullptr;         // per-pixel G_b(0,0) contribution, f32(sum) * gain
-  float* gain = nullptr;             // detector gain (NULL = none)
-  int dp_dtype = QSSB_DTYPE_U16;     // sample type of the last mean DP
-  int* bf_rows = nullptr;
-  int* bf_cols = nullptr;
-  // tuning knobs (ssb_tune_set): stream-pass workgroups, gather FFT pairs,
-  // trotter tile shape; 0 = built-in default
-  int tune_stream_wg = 0;
-  int tune_gather_pairs = 0;
-  int tune_trotter = 0;
-  // recorded after the raw-data stream pass of each recon (ssb_recon_u16_after)
-  hipEvent_t ev_stream = nullptr;
-  bool ev_stream_recorded = false;
-  BfInfo* info = nullptr;
-  float* kx = nullptr;
-  float* kxl = nullptr;
-  float* ky = nullptr;
-  float* kyl = nullptr;
-  float* ak = nullptr;
-  float* qx = nullptr;
-  float* qxl = nullptr;
-  float* qy = nullptr;
-  float* qyl = nullptr;
-  float2* tw = nullptr;
-  float2* H = nullptr;
-  float2* S_part = nullptr;
-  float2* S = nullptr;
-  float2* rowph = nullptr;
-  float2* colph = nullptr;
-  float2* echi = nullptr;
-  float2* Gt = nullptr;
-  size_t Gt_bytes = 0;
-  int H_cap = 0;              // BF planes the row-transformed stack H holds
-  // exact per-BF phase path (allocated on first use)
-  float2* T = nullptr;        // column-transformed planes of one BF chunk
-  float2* M1 = nullptr;       // phase moments per BF split, (sum, compensation)
-  float2* M2 = nullptr;
-  float* var_part = nullptr;  // per-block variance sums
-  float* loss = nullptr;      // scalar loss when the caller passes none
-  // stage timing
-  std::vector<hipEvent_t> events;   // (NSTAGES + 1) per recorded call
-  int timing_calls = 0;
-  int timing_max = 0;
-  int timing_mask = 0;        // stage boundaries recorded (bit i = boundary i)
-};
+#include "ssb_plan.h"
 
 namespace {
 // Stages of ssb_recon_u16, in launch order.
-const char* const kStageNames[] = {"mean_dp_stream", "mean_dp_reduce", "select_bf", "geometry",
+const char* const kStageNames[] = {"superset", "mean_dp_stream", "mean_dp_reduce", "select_bf", "geometry",
                                    "gather_rowfft", "colfft_trotter", "finish_object"};
-constexpr int NSTAGES = 7;
+constexpr int NSTAGES = 8;
 }
 
 namespace {
@@ -354,6 +290,15 @@ int ssb_plan_destroy(ssb_plan* p) {
   dfree(p->M2);
   dfree(p->var_part);
   dfree(p->loss);
+  dfree(p->xt);
+  dfree(p->sup_part);
+  dfree(p->sup_box);
+  dfree(p->sup_C);
+  if (p->xexec) (void)hipGraphExecDestroy(p->xexec);
+  if (p->xgraph) (void)hipGraphDestroy(p->xgraph);
+  if (p->gev_in) (void)hipEventDestroy(p->gev_in);
+  if (p->gev_out) (void)hipEventDestroy(p->gev_out);
+  if (p->gstream) (void)hipStreamDestroy(p->gstream);
   release_events(p);
   delete p;
   return QSSB_OK;
@@ -906,6 +851,126 @@ int ssb_batch_phase_loss(ssb_plan* p, int n_trials, const double* mags_host, con
   return QSSB_OK;
 }
 
+// Enqueue the captured form of a batch: per trial the geometry (every
+// trial, from the device table), then the phasors and the exact loss of
+// ssb_batch_phase_loss reading their coefficients from the table.
+static hipError_t enqueue_exact_batch(ssb_plan* p, int n, float* losses_out, hipStream_t s) {
+  hipError_t e = hipSuccess;
+  for (int t = 0; t < n && e == hipSuccess; ++t) {
+    const ExactTrial& h = p->xt_host[t];
+    const ExactTrial* d = p->xt + t;
+    e = launch_bf_geometry(p->bf_rows, p->bf_cols, p->info, p->max_bf, to_df(p->recip_y), to_df(p->recip_x),
+                           h.cs[0], h.cs[1], p->optics, p->kx, p->kxl, p->ky, p->kyl, p->ak, s, d->cs);
+    if (e != hipSuccess) break;
+    if (h.higher) {
+      e = launch_phase_moments(p->n, p->Gt, p->gcols, p->info, p->g_nbf, geom_ptrs(p), p->rowph, p->colph, p->echi,
+                               &h.cp, p->optics, p->tw, p->T, p->M1, p->M2, p->var_part, nullptr, losses_out + t, s,
+                               &d->cp);
+    } else {
+      e = launch_phasors(p->n, p->info, p->nbf_hint, geom_ptrs(p), h.m, p->rowph, p->colph, p->echi, s, &d->m);
+      if (e == hipSuccess)
+        e = launch_phase_moments(p->n, p->Gt, p->gcols, p->info, p->g_nbf, geom_ptrs(p), p->rowph, p->colph,
+                                 p->echi, nullptr, p->optics, p->tw, p->T, p->M1, p->M2, p->var_part, nullptr,
+                                 losses_out + t, s);
+    }
+  }
+  return e;
+}
+
+int ssb_batch_phase_loss_graph(ssb_plan* p, int n_trials, const double* mags_host, const double* angles_host,
+                               const double* rotation_host, float* losses_out, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (n_trials < 1) return fail(QSSB_ERR_ARG, "n_trials must be positive");
+  if (!mags_host || !angles_host || !losses_out) return fail(QSSB_ERR_ARG, "trial arrays must not be NULL");
+  if (!p->gcols) return fail(QSSB_ERR_STATE, "no resident Fourier stack");
+  hipStream_t s = as_stream(stream);
+  rc = alloc_loss(p);
+  if (rc) return rc;
+  if (p->xt_cap < n_trials) {
+    HIP_TRY(hipStreamSynchronize(s), "batch graph sync");
+    if (p->gstream) HIP_TRY(hipStreamSynchronize(p->gstream), "batch graph sync");
+    dfree(p->xt);
+  dfree(p->sup_part);
+  dfree(p->sup_box);
+  dfree(p->sup_C);
+    p->xt = nullptr;
+    p->xt_cap = 0;
+    HIP_TRY(dmalloc(&p->xt, (size_t)n_trials), "batch table allocation");
+    p->xt_cap = n_trials;
+    if (p->xexec) (void)hipGraphExecDestroy(p->xexec);
+    if (p->xgraph) (void)hipGraphDestroy(p->xgraph);
+    p->xexec = nullptr;
+    p->xgraph = nullptr;
+  }
+  if (!p->gstream) {
+    HIP_TRY(hipStreamCreateWithFlags(&p->gstream, hipStreamNonBlocking), "batch graph stream");
+    HIP_TRY(hipEventCreateWithFlags(&p->gev_in, hipEventDisableTiming), "batch graph event");
+    HIP_TRY(hipEventCreateWithFlags(&p->gev_out, hipEventDisableTiming), "batch graph event");
+  }
+  // the previous replay may still read the device table: order the upload after it
+  HIP_TRY(hipStreamWaitEvent(s, p->gev_out, 0), "batch graph order");
+  std::vector<int> pattern(n_trials);
+  p->xt_host.assign(n_trials, ExactTrial{});
+  for (int t = 0; t < n_trials; ++t) {
+    const double* m = mags_host + (size_t)t * 14;
+    const double* a = angles_host + (size_t)t * 14;
+    ExactTrial& x = p->xt_host[t];
+    const double rot = rotation_host ? rotation_host[t] : p->rotation;
+    x.cs[0] = to_df(std::cos(-rot));
+    x.cs[1] = to_df(std::sin(-rot));
+    bool higher = false;
+    for (int i = 2; i < 14; ++i) higher = higher || (m[i] != 0.0);
+    x.higher = higher ? 1 : 0;
+    x.m = quad_chi(p, m[0], m[1], a[1]);
+    x.cp = chi_poly(p, m, a);
+    pattern[t] = x.higher;
+  }
+  HIP_TRY(hipMemcpyAsync(p->xt, p->xt_host.data(), sizeof(ExactTrial) * n_trials, hipMemcpyHostToDevice, s),
+          "batch table upload");
+  HIP_TRY(hipStreamSynchronize(s), "batch table upload");   // the host table may change after return
+  if (!p->xexec || pattern != p->xt_pattern || p->xgraph_out != losses_out) {
+    if (p->xexec) (void)hipGraphExecDestroy(p->xexec);
+    if (p->xgraph) (void)hipGraphDestroy(p->xgraph);
+    p->xexec = nullptr;
+    p->xgraph = nullptr;
+    HIP_TRY(hipStreamBeginCapture(p->gstream, hipStreamCaptureModeThreadLocal), "batch capture");
+    const hipError_t ee = enqueue_exact_batch(p, n_trials, losses_out, p->gstream);
+    const hipError_t ec = hipStreamEndCapture(p->gstream, &p->xgraph);
+    if (ee != hipSuccess) return hip_fail(ee, "batch capture enqueue");
+    if (ec != hipSuccess) return hip_fail(ec, "batch end capture");
+    HIP_TRY(hipGraphInstantiate(&p->xexec, p->xgraph, nullptr, nullptr, 0), "batch graph instantiate");
+    p->xt_pattern = pattern;
+    p->xgraph_out = losses_out;
+  }
+  HIP_TRY(hipEventRecord(p->gev_in, s), "batch graph launch");
+  HIP_TRY(hipStreamWaitEvent(p->gstream, p->gev_in, 0), "batch graph launch");
+  HIP_TRY(hipGraphLaunch(p->xexec, p->gstream), "batch graph launch");
+  HIP_TRY(hipEventRecord(p->gev_out, p->gstream), "batch graph launch");
+  HIP_TRY(hipStreamWaitEvent(s, p->gev_out, 0), "batch graph launch");
+  // the batch leaves the last trial's geometry: restore the cached rotation's
+  return ensure_geometry(p, s);
+}
+
+// Superset sample sums, box and the compact per-frame copy (allocated on first use).
+static int alloc_superset(ssb_plan* p) {
+  if (p->sup_C) return QSSB_OK;
+  const size_t frames = (size_t)p->n * p->n;
+  hipError_t e = dmalloc(&p->sup_part, (size_t)superset_part_count() * p->D);
+  if (e == hipSuccess) e = dmalloc(&p->sup_box, 1);
+  if (e == hipSuccess) e = dmalloc(&p->sup_C, frames * kSupMaxPixels);
+  if (e != hipSuccess) {
+    dfree(p->sup_part);
+    dfree(p->sup_box);
+    dfree(p->sup_C);
+    p->sup_part = nullptr;
+    p->sup_box = nullptr;
+    p->sup_C = nullptr;
+    return hip_fail(e, "superset allocation");
+  }
+  return QSSB_OK;
+}
+
 static int recon_typed(ssb_plan* p, const ssb_plan* after, const void* data, int dtype, double c10, double c12,
                        double phi12, double rotation_rad, float threshold, float bf_radius, void* obj_out,
                        float* phase_out, hipStream_t s) {
@@ -932,43 +997,64 @@ static int recon_typed(ssb_plan* p, const ssb_plan* after, const void* data, int
     if (ev && ((p->timing_mask >> i) & 1)) (void)hipEventRecord(ev[i], s);
   };
   const long long frames = (long long)p->n * p->n;
+  // bright-field superset (uint16 frames of whole 16-byte detector-row groups):
+  // sampled box, copied compactly by the stream pass for the gather
+  const bool sup = dtype == QSSB_DTYPE_U16 && p->tune_superset != 2 && p->det_cols % 8 == 0 &&
+                   (reinterpret_cast<uintptr_t>(data) & 15u) == 0;
+  if (sup) {
+    rc = alloc_superset(p);
+    if (rc) return rc;
+  }
   mark(0);
-  HIP_TRY(launch_mean_dp_stream(dtype, data, frames, p->D, partial, p->max_ranges, p->tune_stream_wg, s), "mean_dp");
+  if (sup)
+    HIP_TRY(launch_superset(reinterpret_cast<const uint16_t*>(data), frames, p->det_rows, p->det_cols,
+                            p->sup_part, p->sup_box, s),
+            "superset");
+  mark(1);
+  if (sup)
+    HIP_TRY(launch_mean_dp_stream_sup(reinterpret_cast<const uint16_t*>(data), frames, p->D, p->det_cols,
+                                      reinterpret_cast<uint32_t*>(partial), p->max_ranges, p->tune_stream_wg,
+                                      p->sup_box, p->sup_C, s),
+            "mean_dp");
+  else
+    HIP_TRY(launch_mean_dp_stream(dtype, data, frames, p->D, partial, p->max_ranges, p->tune_stream_wg, s),
+            "mean_dp");
   if (p->ev_stream) {
     HIP_TRY(hipEventRecord(p->ev_stream, s), "stream pass event");
     p->ev_stream_recorded = true;
   }
-  mark(1);
+  mark(2);
   HIP_TRY(launch_mean_dp_reduce(dtype, data, frames, p->D, partial, p->max_ranges, p->tune_stream_wg, p->gain,
                                 p->dp_sums, p->dp_mean, p->dp_dcpix, s),
           "mean_dp");
   p->dp_dtype = dtype;
-  mark(2);
-  HIP_TRY(launch_select_bf(p->dp_mean, p->dp_sums, p->dp_dcpix, exact_dc(p), p->det_rows, p->det_cols, threshold,
-                           bf_radius, p->max_bf, p->bf_rows, p->bf_cols, p->info, s),
-          "select_bf");
   mark(3);
+  HIP_TRY(launch_select_bf(p->dp_mean, p->dp_sums, p->dp_dcpix, exact_dc(p), p->det_rows, p->det_cols, threshold,
+                           bf_radius, p->max_bf, p->bf_rows, p->bf_cols, p->info, s, sup ? p->sup_box : nullptr),
+          "select_bf");
+  mark(4);
   p->rotation = rotation_rad;
   rc = ensure_geometry(p, s);
   if (rc) return rc;
   p->gcols = 0;
   rc = alloc_h(p, p->max_bf);
   if (rc) return rc;
-  mark(4);
-  HIP_TRY(launch_gather_rowfft(p->n, dtype, data, p->D, p->det_cols, p->bf_rows, p->bf_cols, p->info, p->nbf_hint,
-                               p->tw, p->tune_gather_pairs, p->gain, p->H, s),
-          "gather_rowfft");
   mark(5);
+  HIP_TRY(launch_gather_rowfft(p->n, dtype, data, p->D, p->det_cols, p->bf_rows, p->bf_cols, p->info, p->nbf_hint,
+                               p->tw, p->tune_gather_pairs, p->gain, p->H, s, sup ? p->sup_box : nullptr,
+                               sup ? p->sup_C : nullptr),
+          "gather_rowfft");
+  mark(6);
   const QuadChi m = quad_chi(p, c10, c12, phi12);
   HIP_TRY(launch_phasors(p->n, p->info, p->nbf_hint, geom_ptrs(p), m, p->rowph, p->colph, p->echi, s), "phasors");
   HIP_TRY(launch_trotter_slots(p->n, true, p->H, p->info, geom_ptrs(p), p->rowph, p->colph, p->echi, p->optics, p->tw, p->S_part,
                                nullptr, true, p->tune_trotter, s),
           "colfft_trotter");
-  mark(6);
+  mark(7);
   HIP_TRY(launch_finish_object(p->n, p->S_part, p->info, p->tw, p->S,
                                reinterpret_cast<float2*>(obj_out), phase_out, s),
           "finish_object");
-  mark(7);
+  mark(8);
   return QSSB_OK;
 }
 
@@ -1073,6 +1159,11 @@ int ssb_tune_candidates(int scan_n, int knob, int* values_host, int capacity) {
       n = trotter_variants(scan_n);
       for (int i = 0; i < n; ++i) vals[i] = i;
       break;
+    case QSSB_TUNE_SUPERSET:
+      vals[0] = 1;
+      vals[1] = 2;
+      n = 2;
+      break;
     default:
       return fail(QSSB_ERR_ARG, "unknown tuning knob " + std::to_string(knob));
   }
@@ -1094,6 +1185,7 @@ int ssb_tune_set(ssb_plan* p, int knob, int value) {
   switch (knob) {
     case QSSB_TUNE_STREAM_WG: p->tune_stream_wg = value; break;
     case QSSB_TUNE_GATHER_PAIRS: p->tune_gather_pairs = value; break;
+    case QSSB_TUNE_SUPERSET: p->tune_superset = value; break;
     default: p->tune_trotter = value; break;
   }
   return QSSB_OK;
@@ -1109,6 +1201,7 @@ int ssb_tune_get(ssb_plan* p, int knob, int* value_host) {
       *value_host = p->tune_gather_pairs ? p->tune_gather_pairs : gather_pairs_default(p->n);
       break;
     case QSSB_TUNE_TROTTER: *value_host = p->tune_trotter; break;
+    case QSSB_TUNE_SUPERSET: *value_host = p->tune_superset ? p->tune_superset : 1; break;
     default: return fail(QSSB_ERR_ARG, "unknown tuning knob " + std::to_string(knob));
   }
   return QSSB_OK;

@@ -36,6 +36,10 @@ namespace {
 
 constexpr int kMaxBlockBytes = 8192;                                  // 4096 uint16 elements
 constexpr int kMaxCompBytes = kMaxBlockBytes + kMaxBlockBytes / 255 + 16;  // LZ4 worst case
+// Compressed blocks up to this size are staged in LDS (detector frames
+// compress 10-20x); larger ones are parsed from global memory.  The small
+// staging buffer leaves room for about twice as many workgroups per CU.
+constexpr int kStageBytes = 2048;
 
 struct BlockDesc {
   long long src;   // byte offset of the block's payload in the compressed buffer
@@ -120,7 +124,7 @@ k_bslz4_index(const uint8_t* __restrict__ comp, long long comp_bytes, const long
 __global__ void __launch_bounds__(64)
 k_bslz4_decode(const uint8_t* __restrict__ comp, const BlockDesc* __restrict__ table, long long n_desc,
                uint16_t* __restrict__ out, int* __restrict__ status) {
-  __shared__ uint8_t cin[kMaxCompBytes];
+  __shared__ uint8_t cin[kStageBytes];
   __shared__ __attribute__((aligned(16))) uint8_t cout[kMaxBlockBytes];
   const int lane = threadIdx.x;
   for (long long d = blockIdx.x; d < n_desc; d += gridDim.x) {
@@ -135,8 +139,11 @@ k_bslz4_decode(const uint8_t* __restrict__ comp, const BlockDesc* __restrict__ t
     }
     const int cs = bd.csize;
     const int osz = bd.elems * 2;
+    const bool staged = cs <= kStageBytes;             // uniform
+    const uint8_t* gsrc = comp + bd.src;
     __syncthreads();                                   // previous block's readers are done
-    for (int t = lane; t < cs; t += 64) cin[t] = comp[bd.src + t];
+    if (staged)
+      for (int t = lane; t < cs; t += 64) cin[t] = gsrc[t];
     __syncthreads();
     // LZ4 block: every lane walks the same sequence headers.  The workgroup
     // is one wave, whose LDS operations complete in issue order, so the
@@ -158,7 +165,7 @@ k_bslz4_decode(const uint8_t* __restrict__ comp, const BlockDesc* __restrict__ t
           } while (b == 255);
         }
         if (ip + lit > cs || op + lit > osz) return kErrStream;
-        for (int t = lane; t < lit; t += 64) cout[op + t] = cin[ip + t];
+        for (int t = lane; t < lit; t += 64) cout[op + t] = byte_at(ip + t);
         ip += lit;
         op += lit;
         if (ip == cs) return 0;                          // the last sequence has no match
@@ -190,7 +197,8 @@ k_bslz4_decode(const uint8_t* __restrict__ comp, const BlockDesc* __restrict__ t
       }
     };
     int ip = 0, op = 0;
-    int err = parse([&](int i) -> int { return cin[i]; }, ip, op);
+    int err = staged ? parse([&](int i) -> int { return cin[i]; }, ip, op)
+                     : parse([&](int i) -> int { return gsrc[i]; }, ip, op);
     if (!err && op != osz) err = kErrStream;
     if (err) {
       if (lane == 0) atomicOr(status, err);

@@ -32,8 +32,18 @@ struct BfInfo {
   float radius2;
   float dc_re;
   float dc_im;
-  int pad;
+  int sup_ok;   // every BF pixel lies in the compact superset box (SupBox) of this recon
 };
+
+// Superset of the bright field for one reconstruction: a detector box
+// [r0, r0 + bh) x [c0, c0 + bw) estimated from sampled frames before the
+// stream pass, which then also writes every frame's box compactly
+// (C[frame][row - r0][col - c0], bh * bw uint16 per frame).  bw == 0: none.
+struct SupBox {
+  int r0, c0, bh, bw;
+};
+// Largest box (pixels) the compact copy holds.
+constexpr int kSupMaxPixels = 2048;
 
 // Aberration set in the reference's numeric convention (Angstrom, radians).
 struct Theta {
@@ -93,23 +103,38 @@ hipError_t launch_mean_dp_reduce(int dtype, const void* data, long long frames, 
                                  int max_ranges, int target_wg, const float* gain, unsigned long long* sums,
                                  float* mean, float* dcpix, hipStream_t s);
 // exact_dc: dc from the exact unsigned integer sums (no gain); otherwise
-// from a compensated sum of dcpix over the selected pixels.
+// from a compensated sum of dcpix over the selected pixels.  box (may be
+// NULL): info->sup_ok = every selected pixel inside it.
 hipError_t launch_select_bf(const float* mean, const unsigned long long* sums, const float* dcpix, bool exact_dc,
                             int det_rows, int det_cols, float threshold, float bf_radius, int max_bf, int* rows,
-                            int* cols, BfInfo* info, hipStream_t s);
+                            int* cols, BfInfo* info, hipStream_t s, const SupBox* box = nullptr);
+// Superset box from kSupSamples frames spread over the scan (uint16
+// frames; part holds kSupSplits * D uint32).
+hipError_t launch_superset(const uint16_t* data, long long frames, int det_rows, int det_cols, uint32_t* part,
+                           SupBox* box, hipStream_t s);
+int superset_part_count();
+// The uint16 stream pass that also writes each frame's superset box into C
+// (frames * kSupMaxPixels uint16); needs det_cols % 8 == 0 and 16-byte
+// aligned data.
+hipError_t launch_mean_dp_stream_sup(const uint16_t* data, long long frames, int D, int det_cols, uint32_t* partial,
+                                     int max_ranges, int target_wg, const SupBox* box, uint16_t* C, hipStream_t s);
 // Probe disk centre and radius of a float32 mean DP; out[4] = {status, row, col, radius}.
 hipError_t launch_detect_bf_radius(const float* mean, int R, int C, df ratio, int* out, hipStream_t s);
+// rot_dev (device, may be NULL): {cos_a, sin_a} read in place of the arguments.
 hipError_t launch_bf_geometry(const int* rows, const int* cols, const BfInfo* info, int max_bf, df recip_y,
                               df recip_x, df cos_a, df sin_a, const Optics& o, float* kx, float* kxl, float* ky,
-                              float* kyl, float* ak, hipStream_t s);
+                              float* kyl, float* ak, hipStream_t s, const df* rot_dev = nullptr);
 // pairs: complex FFTs (2 BF pixels each) per workgroup, one of
 // gather_pairs_candidates(N) (0 = gather_pairs_default(N)).
 int gather_pairs_default(int N);
 int gather_pairs_candidates(int N, int* out);   // out[2]; returns the count
 // Samples of type dtype become float32 times gain[pixel] (gain may be NULL).
+// With box and C (uint16 frames): when info->sup_ok the samples come from
+// the compact superset copy C instead of the frames (the same values).
 hipError_t launch_gather_rowfft(int N, int dtype, const void* data, int D, int det_cols, const int* rows,
                                 const int* cols, const BfInfo* info, int nbf_hint, const float2* tw, int pairs,
-                                const float* gain, float2* H, hipStream_t s);
+                                const float* gain, float2* H, hipStream_t s, const SupBox* box = nullptr,
+                                const uint16_t* C = nullptr);
 // Exact BF columns out[b][s] for BF pixels [bf_offset, bf_offset + count)
 // of the list rows/cols: uint8 frames -> uint8 (in_u8), uint16 frames ->
 // uint16, or uint8 when narrow.
@@ -149,7 +174,8 @@ ApIso iso_aperture(const Optics& o, df& rout2_df);
 // (c = 0..N/2) and the plane echi[c][q] = e^{i chi(Q)} for the current
 // aberrations.
 hipError_t launch_phasors(int N, const BfInfo* info, int nbf_hint, const GeomPtrs& gp, QuadChi m, float2* rowph,
-                          float2* colph, float2* echi, hipStream_t s);
+                          float2* colph, float2* echi, hipStream_t s,
+                          const QuadChi* m_dev = nullptr);
 // Trotter over Hermitian slot columns: from the row-transformed stack H
 // (fused scan-column FFT; from_h) or from the resident stack Gt[b][c][q],
 // c = 0..N/2.  Writes partial planes S_part[g][c][q] (do_trotter) and, if Gt
@@ -182,7 +208,8 @@ hipError_t launch_phase_moments(int N, const float2* Gt, int gcols, const BfInfo
                                 const float2* rowph, const float2* colph, const float2* echi, const ChiPoly* poly,
                                 const Optics& o,
                                 const float2* tw, float2* T, float2* M1, float2* M2, float* var_part,
-                                float* phase, float* loss, hipStream_t s);
+                                float* phase, float* loss, hipStream_t s,
+                                const ChiPoly* poly_dev = nullptr);
 
 // Learned-initializer features of the resident Hermitian stack at the 8
 // scan-frequency grid points (qrow[i], qcol[i]) (device arrays): out is

@@ -40,8 +40,12 @@ template <int N, int SG, int BLOCK, bool ISO, bool FULL, bool POLY>
 __global__ void __launch_bounds__(BLOCK)
 k_corr_colifft(const float2* __restrict__ Gt, const BfInfo* __restrict__ info, GeomPtrs gp,
                const float2* __restrict__ rowph_g, const float2* __restrict__ colph_g,
-               const float2* __restrict__ echi_g, ChiPoly cp, ApIso ap, df rout2, Optics o,
-               const float2* __restrict__ tw_g, int b_begin, int b_count, float2* __restrict__ T) {
+               const float2* __restrict__ echi_g, ChiPoly cp_arg, const ChiPoly* __restrict__ cp_dev, ApIso ap,
+               df rout2, Optics o, const float2* __restrict__ tw_g, int b_begin, int b_count,
+               float2* __restrict__ T) {
+  // the chi polynomial: a kernel argument, or read from device memory (a
+  // captured batch of trials replays with new coefficients)
+  const ChiPoly cp = (POLY && cp_dev) ? *cp_dev : cp_arg;
   constexpr int HALF = N / 2;
   constexpr int NSG = HALF / SG;
   constexpr int P = SG * N / BLOCK;
@@ -137,8 +141,8 @@ k_corr_colifft(const float2* __restrict__ Gt, const BfInfo* __restrict__ info, G
       const float2 colp = POLY ? make_float2(1.f, 0.f) : colph_g[(long long)b * GCOLS + j];
       float am, app;
       if constexpr (ISO) {
-        am = aperture_from_d(df_add(rowTA, colA_s[s]), ap);
-        app = aperture_from_d(df_add(rowTB, colB_s[s]), ap);
+        am = aperture_sum(rowTA, colA_s[s], ap);
+        app = aperture_sum(rowTB, colB_s[s], ap);
       } else {
         aperture_pair<ISO>(qxA, qy, kx, ky, cb, q2A, ap, o, am, app);
       }
@@ -154,10 +158,8 @@ k_corr_colifft(const float2* __restrict__ Gt, const BfInfo* __restrict__ info, G
       } else {
         echiB = POLY ? make_float2(1.f, 0.f) : ((j == 0) ? ehalfc[q] : enyqr[s]);
         if constexpr (ISO) {
-          const df dA = (j == 0) ? df_add(rowTA, colA_s[SG]) : df_add(rowTA, colB_s[s]);
-          const df dB = (j == 0) ? df_add(rowTB, colB_s[SG]) : df_add(rowTB, colA_s[s]);
-          bm = aperture_from_d(dA, ap);
-          bp = aperture_from_d(dB, ap);
+          bm = aperture_sum(rowTA, (j == 0) ? colA_s[SG] : colB_s[s], ap);
+          bp = aperture_sum(rowTB, (j == 0) ? colB_s[SG] : colA_s[s], ap);
         } else {
           const df qyB = (j == 0) ? df{qyS[SG], qySl[SG]} : df{-qy.hi, -qy.lo};
           const df qxB = (j == 0) ? qxA : qxB0;
@@ -371,7 +373,7 @@ hipError_t launch_phase_moments(int N, const float2* Gt, int gcols, const BfInfo
                                 const float2* rowph, const float2* colph, const float2* echi, const ChiPoly* poly,
                                 const Optics& o,
                                 const float2* tw, float2* T, float2* M1, float2* M2, float* var_part,
-                                float* phase, float* loss, hipStream_t s) {
+                                float* phase, float* loss, hipStream_t s, const ChiPoly* poly_dev) {
   df rout2;
   const ApIso ap = iso_aperture(o, rout2);
   const bool iso = optics_isotropic(o);
@@ -384,10 +386,10 @@ hipError_t launch_phase_moments(int N, const float2* Gt, int gcols, const BfInfo
   do {                                                                                                       \
     if (poly)                                                                                                \
       hipLaunchKernelGGL((k_corr_colifft<NN, SGV, BL, IS, FU, true>), dim3(g1), dim3(BL), 0, s, Gt, info, gp,   \
-                         rowph, colph, echi, *poly, ap, rout2, o, tw, b0, cnt, T);                            \
+                         rowph, colph, echi, *poly, poly_dev, ap, rout2, o, tw, b0, cnt, T);                  \
     else                                                                                                     \
       hipLaunchKernelGGL((k_corr_colifft<NN, SGV, BL, IS, FU, false>), dim3(g1), dim3(BL), 0, s, Gt, info, gp,  \
-                         rowph, colph, echi, ChiPoly{}, ap, rout2, o, tw, b0, cnt, T);                        \
+                         rowph, colph, echi, ChiPoly{}, nullptr, ap, rout2, o, tw, b0, cnt, T);               \
   } while (0)
 #define QSSB_P1(NN, SGV, BL)                                                    \
   do {                                                                         \

@@ -20,8 +20,9 @@ int object_groups(int N) { return N == 128 ? 64 : (N == 256 ? 16 : (N == 512 ? 8
 // c = 0..N/2 (the other columns are their conjugates); and the plane
 // echi[c][q] = e^{i chi(Q)} (column-major, like the partial sums).
 __global__ void __launch_bounds__(256)
-k_phasors(const BfInfo* __restrict__ info, GeomPtrs gp, QuadChi m, int N, float2* __restrict__ rowph,
-          float2* __restrict__ colph, float2* __restrict__ echi) {
+k_phasors(const BfInfo* __restrict__ info, GeomPtrs gp, QuadChi m_arg, int N, float2* __restrict__ rowph,
+          float2* __restrict__ colph, float2* __restrict__ echi, const QuadChi* __restrict__ m_dev) {
+  const QuadChi m = m_dev ? *m_dev : m_arg;
   const int nbf = info->nbf;
   const int ncol = N / 2 + 1;
   const long long per_b = (long long)N + ncol;
@@ -270,8 +271,8 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
         const df qy{qyS[s], qySl[s]};
         float am, app;
         if constexpr (ISO) {
-          am = aperture_from_d(df_add(rowTA, colA_s[cur][s]), ap);
-          app = aperture_from_d(df_add(rowTB, colB_s[cur][s]), ap);
+          am = aperture_sum(rowTA, colA_s[cur][s], ap);
+          app = aperture_sum(rowTB, colB_s[cur][s], ap);
         } else {
           aperture_pair<ISO>(qxA, qy, kx, ky, cb, q2A[i], ap, o, am, app);
         }
@@ -303,10 +304,8 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
           float bm, bp;
           if constexpr (ISO) {
             // j == 0: row q, column N/2; q == N/2: row N/2 (this row), column -qy
-            const df dA = (j == 0) ? df_add(rowTA, colA_s[cur][SG]) : df_add(rowTA, colB_s[cur][s]);
-            const df dB = (j == 0) ? df_add(rowTB, colB_s[cur][SG]) : df_add(rowTB, colA_s[cur][s]);
-            bm = aperture_from_d(dA, ap);
-            bp = aperture_from_d(dB, ap);
+            bm = aperture_sum(rowTA, (j == 0) ? colA_s[cur][SG] : colB_s[cur][s], ap);
+            bp = aperture_sum(rowTB, (j == 0) ? colB_s[cur][SG] : colA_s[cur][s], ap);
           } else {
             const df qyB = (j == 0) ? df{qyS[SG], qySl[SG]} : df{-qy.hi, -qy.lo};
             const df qxB = (j == 0) ? qxA : qxB0;
@@ -538,11 +537,12 @@ ApIso iso_aperture(const Optics& o, df& rout2_df) {
 bool optics_isotropic(const Optics& o) { return o.angy.hi == o.angx.hi && o.angy.lo == o.angx.lo; }
 
 hipError_t launch_phasors(int N, const BfInfo* info, int nbf_hint, const GeomPtrs& gp, QuadChi m, float2* rowph,
-                          float2* colph, float2* echi, hipStream_t s) {
+                          float2* colph, float2* echi, hipStream_t s,
+                          const QuadChi* m_dev) {
   long long total = (long long)(nbf_hint > 0 ? nbf_hint : 1) * (N + N / 2 + 1);
   if (total < (long long)N * N) total = (long long)N * N;
   hipLaunchKernelGGL(k_phasors, dim3(grid_items((total + 255) / 256, 4096)), dim3(256), 0, s, info, gp, m, N, rowph,
-                     colph, echi);
+                     colph, echi, m_dev);
   return hipGetLastError();
 }
 

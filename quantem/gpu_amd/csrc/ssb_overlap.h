@@ -32,6 +32,19 @@ __device__ __forceinline__ float aperture_from_d(df d, const ApIso& a) {
   return fminf(fmaxf(e, 0.0f), 1.0f);
 }
 
+// aperture_from_d(a + b) for d given as two double-float terms: the float32
+// sum of their high words is within a few 1e-6 of d for the magnitudes here
+// (|a|, |b| < 32), so outside the soft band by more than 1e-4 the value is
+// exactly 1 or 0 without the double-float addition; inside, the same
+// double-float evaluation as aperture_from_d.
+__device__ __forceinline__ float aperture_sum(const df& a, const df& b, const ApIso& ap) {
+  constexpr float kMargin = 1e-4f;
+  const float est = a.hi + b.hi;
+  if (est >= ap.d_in + kMargin) return 1.0f;
+  if (est <= ap.d_out - kMargin) return 0.0f;
+  return aperture_from_d(df_add(a, b), ap);
+}
+
 // Aperture pair A(Q - k), A(Q + k).  ISO: from d = (Rout^2 - |k|^2) - |Q|^2
 // +/- 2 Q.k; otherwise the general double-float soft-edge expression.
 template <bool ISO>

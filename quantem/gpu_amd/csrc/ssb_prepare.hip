@@ -75,6 +75,77 @@ k_dp_partial(const uint16_t* __restrict__ data, long long frames, int D, int fra
   }
 }
 
+// The uint16 stream pass of a reconstruction: the same partial sums, and
+// every frame's superset box copied compactly to C[frame][row - r0][col - c0]
+// (bh * bw uint16 per frame), so the BF gather reads contiguous box rows
+// instead of 128-byte lines under scattered BF pixels.  A thread's 8 pixels
+// never cross a detector row (det_cols % 8 == 0).
+__global__ void __launch_bounds__(256)
+k_dp_partial_sup(const uint16_t* __restrict__ data, long long frames, int D, int det_cols, int frames_per_range,
+                 uint32_t* __restrict__ partial, const SupBox* __restrict__ boxp, uint16_t* __restrict__ C) {
+  const int pix0 = (blockIdx.x * 256 + threadIdx.x) * 8;
+  const long long f0 = (long long)blockIdx.y * frames_per_range;
+  long long f1 = f0 + frames_per_range;
+  if (f1 > frames) f1 = frames;
+  if (pix0 >= D) return;
+  const SupBox box = *boxp;
+  const int row = pix0 / det_cols, col0 = pix0 - row * det_cols;
+  const long long SB = (long long)box.bh * box.bw;
+  int e_lo = 8, e_hi = 0;
+  long long cbase = 0;
+  if (box.bw > 0 && row >= box.r0 && row < box.r0 + box.bh) {
+    e_lo = max(0, box.c0 - col0);
+    e_hi = min(8, box.c0 + box.bw - col0);
+    cbase = (long long)(row - box.r0) * box.bw + (col0 - box.c0);
+  }
+  const bool copy = e_lo < e_hi;
+  uint32_t acc[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) acc[e] = 0u;
+  const uint16_t* p = data + f0 * (long long)D + pix0;
+  uint16_t* cp = C + f0 * SB + cbase;
+  auto put = [&](const u32x4& v, uint16_t* dst) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+      if (e >= e_lo && e < e_hi) dst[e] = (uint16_t)((v[e >> 1] >> (16 * (e & 1))) & 0xFFFFu);
+  };
+  long long f = f0;
+  for (; f + 2 <= f1; f += 2) {
+    u32x4 v[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+      v[u] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p + (long long)u * D));
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        acc[2 * w] += v[u][w] & 0xFFFFu;
+        acc[2 * w + 1] += v[u][w] >> 16;
+      }
+    }
+    if (copy) {
+      put(v[0], cp);
+      put(v[1], cp + SB);
+    }
+    p += 2LL * D;
+    cp += 2 * SB;
+  }
+  for (; f < f1; ++f) {
+    u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      acc[2 * w] += v[w] & 0xFFFFu;
+      acc[2 * w + 1] += v[w] >> 16;
+    }
+    if (copy) put(v, cp);
+    p += D;
+    cp += SB;
+  }
+  uint32_t* out = partial + (long long)blockIdx.y * D + pix0;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) out[e] = acc[e];
+}
+
 // Partial-sum arithmetic per sample type: P is the stored partial.
 template <typename T> struct DpElem;
 template <> struct DpElem<uint8_t> { typedef uint32_t P; };
@@ -336,7 +407,9 @@ __global__ void __launch_bounds__(MASK_THREADS)
 k_select_bf(const float* __restrict__ mean_g, const unsigned long long* __restrict__ sums,
             const float* __restrict__ dcpix, int exact_dc,
             int det_rows, int det_cols, float threshold, float bf_radius, int max_bf,
-            int* __restrict__ bf_rows, int* __restrict__ bf_cols, BfInfo* __restrict__ info) {
+            int* __restrict__ bf_rows, int* __restrict__ bf_cols, BfInfo* __restrict__ info,
+            const SupBox* __restrict__ boxp) {
+  const SupBox box = boxp ? *boxp : SupBox{0, 0, 0, 0};
   __shared__ float sh_f[2 * MASK_WAVES];
   __shared__ long long sh_i[3 * MASK_WAVES];
   __shared__ int sh_scan[MASK_WAVES];
@@ -440,7 +513,7 @@ k_select_bf(const float* __restrict__ mean_g, const unsigned long long* __restri
   // keep candidates inside the disk, compacted in raster order; also count
   // candidates and sum the exact integer columns of the kept pixels (dc)
   unsigned long long mask_bits = 0ull;
-  long long any_cand = 0, dcs = 0;
+  long long any_cand = 0, dcs = 0, outside = 0;
   int local = 0;
   Kahan kd;
   kd.init();
@@ -452,11 +525,11 @@ k_select_bf(const float* __restrict__ mean_g, const unsigned long long* __restri
         local += 1;
         if (exact_dc) dcs += (long long)sums[i];
         else kd.add(dcpix[i]);
+        if (r_ < box.r0 || r_ >= box.r0 + box.bh || c_ < box.c0 || c_ >= box.c0 + box.bw) outside += 1;
       }
     }
   }
-  long long dummy = 0;
-  block_sum3_i64(any_cand, dcs, dummy, sh_i);
+  block_sum3_i64(any_cand, dcs, outside, sh_i);
   const float dc_float = exact_dc ? 0.0f : block_sum_comp(kd.s, -kd.c, sh_f);
   // exclusive scan of `local` over the block: wave scan + wave offsets
   int incl = local;
@@ -509,7 +582,82 @@ k_select_bf(const float* __restrict__ mean_g, const unsigned long long* __restri
     }
     info->dc_re = dc;
     info->dc_im = 0.0f;
+    info->sup_ok = (box.bw > 0 && outside == 0) ? 1 : 0;
   }
+}
+
+// ------------------------------------------------------------------------
+// Superset box of the bright field, before the stream pass: per-pixel sums
+// of kSupSamples frames spread evenly over the scan (kSupSplits partial
+// sums per pixel), then one workgroup takes the box around every pixel above
+// a quarter of the brightest, widened by kSupMargin pixels.  The exact BF
+// selection later checks that it lies inside (BfInfo::sup_ok); if not, the
+// gather reads the frames.
+// ------------------------------------------------------------------------
+constexpr int kSupSamples = 512;
+constexpr int kSupSplits = 8;
+constexpr int kSupMargin = 2;
+
+__global__ void __launch_bounds__(256)
+k_sup_sample(const uint16_t* __restrict__ data, long long frames, int D, uint32_t* __restrict__ part) {
+  const int pix = blockIdx.x * 256 + threadIdx.x;
+  if (pix >= D) return;
+  const long long n = frames < kSupSamples ? frames : kSupSamples;
+  const long long stride = frames / n;
+  uint32_t acc = 0u;
+#pragma unroll 8
+  for (long long i = blockIdx.y; i < n; i += kSupSplits) acc += data[i * stride * D + pix];
+  part[(long long)blockIdx.y * D + pix] = acc;
+}
+
+__global__ void __launch_bounds__(MASK_THREADS)
+k_sup_box(const uint32_t* __restrict__ part, int det_rows, int det_cols, SupBox* __restrict__ box) {
+  __shared__ float sh_f[2 * MASK_WAVES];
+  const int tid = threadIdx.x;
+  const int D = det_rows * det_cols;
+  auto total = [&](int i) {
+    uint32_t t = 0u;
+#pragma unroll
+    for (int k = 0; k < kSupSplits; ++k) t += part[(long long)k * D + i];
+    return (float)t;
+  };
+  float lmax = 0.0f;
+  for (int i = tid; i < D; i += MASK_THREADS) lmax = fmaxf(lmax, total(i));
+  const float thr = 0.25f * block_max(lmax, sh_f);
+  float rmin = -INFINITY, rmax = -INFINITY, cmin = -INFINITY, cmax = -INFINITY;   // rmin/cmin negated
+  for (int i = tid; i < D; i += MASK_THREADS) {
+    if (total(i) > thr) {
+      const int r = i / det_cols, c = i - (i / det_cols) * det_cols;
+      rmin = fmaxf(rmin, (float)-r);
+      rmax = fmaxf(rmax, (float)r);
+      cmin = fmaxf(cmin, (float)-c);
+      cmax = fmaxf(cmax, (float)c);
+    }
+  }
+  rmin = block_max(rmin, sh_f);
+  rmax = block_max(rmax, sh_f);
+  cmin = block_max(cmin, sh_f);
+  cmax = block_max(cmax, sh_f);
+  if (tid == 0) {
+    SupBox b{0, 0, 0, 0};
+    if (thr > 0.0f && rmax >= 0.0f) {
+      const int r0 = max(0, (int)(-rmin) - kSupMargin), r1 = min(det_rows - 1, (int)rmax + kSupMargin);
+      const int c0 = max(0, (int)(-cmin) - kSupMargin), c1 = min(det_cols - 1, (int)cmax + kSupMargin);
+      const int bh = r1 - r0 + 1, bw = c1 - c0 + 1;
+      if (bh > 0 && bw > 0 && bh * bw <= kSupMaxPixels) b = SupBox{r0, c0, bh, bw};
+    }
+    *box = b;
+  }
+}
+
+int superset_part_count() { return kSupSplits; }
+
+hipError_t launch_superset(const uint16_t* data, long long frames, int det_rows, int det_cols, uint32_t* part,
+                           SupBox* box, hipStream_t s) {
+  const int D = det_rows * det_cols;
+  hipLaunchKernelGGL(k_sup_sample, dim3((D + 255) / 256, kSupSplits), dim3(256), 0, s, data, frames, D, part);
+  hipLaunchKernelGGL(k_sup_box, dim3(1), dim3(MASK_THREADS), 0, s, part, det_rows, det_cols, box);
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------------------
@@ -518,9 +666,11 @@ k_select_bf(const float* __restrict__ mean_g, const unsigned long long* __restri
 // ------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 k_bf_geometry(const int* __restrict__ bf_rows, const int* __restrict__ bf_cols,
-              const BfInfo* __restrict__ info, int max_bf, df recip_y, df recip_x, df cos_a, df sin_a, Optics o,
+              const BfInfo* __restrict__ info, int max_bf, df recip_y, df recip_x, df cos_arg, df sin_arg, Optics o,
               float* __restrict__ kx_out, float* __restrict__ kxl_out, float* __restrict__ ky_out,
-              float* __restrict__ kyl_out, float* __restrict__ ak_out) {
+              float* __restrict__ kyl_out, float* __restrict__ ak_out, const df* __restrict__ rot_dev) {
+  const df cos_a = rot_dev ? rot_dev[0] : cos_arg;
+  const df sin_a = rot_dev ? rot_dev[1] : sin_arg;
   const int b = blockIdx.x * 256 + threadIdx.x;
   const int nbf = info->nbf;
   if (b >= nbf || b >= max_bf) return;
@@ -551,7 +701,8 @@ __global__ void __launch_bounds__(256)
 k_gather_rowfft(const T* __restrict__ data, int D, int det_cols,
                 const int* __restrict__ bf_rows, const int* __restrict__ bf_cols,
                 const BfInfo* __restrict__ info, const float2* __restrict__ tw_g,
-                const float* __restrict__ gain, float2* __restrict__ H) {
+                const float* __restrict__ gain, float2* __restrict__ H, const SupBox* __restrict__ boxp,
+                const uint16_t* __restrict__ C) {
   constexpr int STRIDE = N + 1;
   constexpr int CH = 2 * PAIRS;
   __shared__ float2 z[PAIRS * STRIDE];
@@ -563,6 +714,12 @@ k_gather_rowfft(const T* __restrict__ data, int D, int det_cols,
   const int nbf = info->nbf;
   const int nchunks = (nbf + CH - 1) / CH;
   const long long items = (long long)nchunks * N;
+  // compact superset copy: frame stride SB, pixel (r, c) at (r - r0) * bw + (c - c0)
+  const bool sup = std::is_same<T, uint16_t>::value && C != nullptr && info->sup_ok != 0;
+  SupBox box{0, 0, 0, 0};
+  if (sup) box = *boxp;
+  const long long fstride = sup ? (long long)box.bh * box.bw : (long long)D;
+  const T* src = sup ? reinterpret_cast<const T*>(C) : data;
   for (long long item = blockIdx.x; item < items; item += gridDim.x) {
     // XCD-aware order: workgroups go round-robin over the 8 XCDs, so give
     // XCD x the items (row, chunk) of rows x, x+8, ... with the chunks of one
@@ -576,7 +733,7 @@ k_gather_rowfft(const T* __restrict__ data, int D, int det_cols,
     if (tid < CH) {
       const int b = b0 + tid;
       const int o = (b < nbf) ? bf_rows[b] * det_cols + bf_cols[b] : -1;
-      offs[tid] = o;
+      offs[tid] = (o < 0) ? -1 : (sup ? (bf_rows[b] - box.r0) * box.bw + (bf_cols[b] - box.c0) : o);
       gsc[tid] = (gain && o >= 0) ? gain[o] : 1.0f;
     }
     __syncthreads();
@@ -592,10 +749,10 @@ k_gather_rowfft(const T* __restrict__ data, int D, int det_cols,
       const float gj = gsc[j];
       const int c0 = tid / CH;
       float* zf = reinterpret_cast<float*>(z);
-      const T* base = data + (long long)rr * N * D + (off >= 0 ? off : 0);
+      const T* base = src + (long long)rr * N * fstride + (off >= 0 ? off : 0);
       T raw[NL];
 #pragma unroll
-      for (int m = 0; m < NL; ++m) raw[m] = base[(long long)(c0 + m * FR) * D];
+      for (int m = 0; m < NL; ++m) raw[m] = base[(long long)(c0 + m * FR) * fstride];
 #pragma unroll
       for (int m = 0; m < NL; ++m) {
         const int c = c0 + m * FR;
@@ -714,6 +871,16 @@ hipError_t launch_mean_dp_stream(int dtype, const void* data, long long frames, 
     case QSSB_DTYPE_F32: launch_partial_t<float>(data, frames, D, vec, ranges, fpr, partial, s); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_mean_dp_stream_sup(const uint16_t* data, long long frames, int D, int det_cols, uint32_t* partial,
+                                     int max_ranges, int target_wg, const SupBox* box, uint16_t* C, hipStream_t s) {
+  if (det_cols % 8 != 0 || stream_vec(QSSB_DTYPE_U16, D, data) != 8) return hipErrorInvalidValue;
+  int ranges, fpr;
+  range_split(frames, D, 8, max_ranges, target_wg, ranges, fpr);
+  dim3 grid((D / 8 + 255) / 256, ranges);
+  hipLaunchKernelGGL(k_dp_partial_sup, grid, dim3(256), 0, s, data, frames, D, det_cols, fpr, partial, box, C);
   return hipGetLastError();
 }
 
@@ -890,21 +1057,21 @@ hipError_t launch_detect_bf_radius(const float* mean, int R, int C, df ratio, in
 
 hipError_t launch_select_bf(const float* mean, const unsigned long long* sums, const float* dcpix, bool exact_dc,
                             int det_rows, int det_cols, float threshold, float bf_radius, int max_bf, int* rows,
-                            int* cols, BfInfo* info, hipStream_t s) {
+                            int* cols, BfInfo* info, hipStream_t s, const SupBox* box) {
   if (det_rows * det_cols <= SELECT_CACHE_PIX)
     hipLaunchKernelGGL(k_select_bf<true>, dim3(1), dim3(MASK_THREADS), 0, s, mean, sums, dcpix, (int)exact_dc,
-                       det_rows, det_cols, threshold, bf_radius, max_bf, rows, cols, info);
+                       det_rows, det_cols, threshold, bf_radius, max_bf, rows, cols, info, box);
   else
     hipLaunchKernelGGL(k_select_bf<false>, dim3(1), dim3(MASK_THREADS), 0, s, mean, sums, dcpix, (int)exact_dc,
-                       det_rows, det_cols, threshold, bf_radius, max_bf, rows, cols, info);
+                       det_rows, det_cols, threshold, bf_radius, max_bf, rows, cols, info, box);
   return hipGetLastError();
 }
 
 hipError_t launch_bf_geometry(const int* rows, const int* cols, const BfInfo* info, int max_bf, df recip_y,
                               df recip_x, df cos_a, df sin_a, const Optics& o, float* kx, float* kxl, float* ky,
-                              float* kyl, float* ak, hipStream_t s) {
+                              float* kyl, float* ak, hipStream_t s, const df* rot_dev) {
   hipLaunchKernelGGL(k_bf_geometry, dim3((max_bf + 255) / 256), dim3(256), 0, s, rows, cols, info, max_bf,
-                     recip_y, recip_x, cos_a, sin_a, o, kx, kxl, ky, kyl, ak);
+                     recip_y, recip_x, cos_a, sin_a, o, kx, kxl, ky, kyl, ak, rot_dev);
   return hipGetLastError();
 }
 
@@ -922,7 +1089,8 @@ int gather_pairs_candidates(int N, int* out) {
 
 hipError_t launch_gather_rowfft(int N, int dtype, const void* data, int D, int det_cols, const int* rows,
                                 const int* cols, const BfInfo* info, int nbf_hint, const float2* tw, int pairs,
-                                const float* gain, float2* H, hipStream_t s) {
+                                const float* gain, float2* H, hipStream_t s, const SupBox* box, const uint16_t* C) {
+  if (!box) C = nullptr;
   if (pairs <= 0) pairs = gather_pairs_default(N);
   auto grid_for = [&](int ch) {
     long long items = (long long)((nbf_hint + ch - 1) / ch) * N;
@@ -933,7 +1101,7 @@ hipError_t launch_gather_rowfft(int N, int dtype, const void* data, int D, int d
 #define QSSB_GATHER(NN, PP, T)                                                                              \
   if (N == NN && pairs == PP) {                                                                             \
     hipLaunchKernelGGL((k_gather_rowfft<NN, PP, T>), grid_for(2 * PP), dim3(256), 0, s,                     \
-                       reinterpret_cast<const T*>(data), D, det_cols, rows, cols, info, tw, gain, H);        \
+                       reinterpret_cast<const T*>(data), D, det_cols, rows, cols, info, tw, gain, H, box, C); \
     return hipGetLastError();                                                                               \
   }
   if (dtype == QSSB_DTYPE_U16) {

@@ -743,7 +743,7 @@ class HipSSBBackend:
 
     def fit_learned(self, net, feature_config, *, trials: int = 16, refine: str | None = "newton",
                     xatol: float = 1e-4, rounds: int | None = None, half_widths=None, bounds=None, seed: int = 42,
-                    newton_step=None, verbose: bool = False):
+                    newton_step=None, newton_ftol: float | None = None, verbose: bool = False):
         """Learned start, a ``trials``-candidate batched Sobol search in a box
         around it (clipped to ``bounds``, the global search box), then a local
         refinement of (C10, C30, rotation) on the exact objective:
@@ -775,14 +775,16 @@ class HipSSBBackend:
                                            bounds.get(k, (-math.inf, math.inf))[1]) for k, v in start.items()})
         keys = tuple(start)
         self._local_refine(refine, keys, [half[k] / 2 for k in keys], rounds, 16, bounds, seed, xatol, verbose,
-                           newton_step or self._LEARNED_NEWTON_STEP, self._LEARNED_NEWTON_FTOL)
+                           newton_step or self._LEARNED_NEWTON_STEP,
+                           self._LEARNED_NEWTON_FTOL if newton_ftol is None else newton_ftol)
         self._learned_start = start
         self._n_trials = trials
         self._elapsed_optimize = time.perf_counter() - t0
         return self
 
     def fit_search(self, *, trials: int = 512, refine: str | None = "newton", xatol: float = 1e-4,
-                   rounds: int | None = None, bounds=None, seed: int = 42, newton_step=None, verbose: bool = False):
+                   rounds: int | None = None, bounds=None, seed: int = 42, newton_step=None,
+                   newton_ftol: float | None = None, verbose: bool = False):
         """Plain batched fit without the network: a ``trials``-candidate global
         Sobol search over ``bounds`` (C10, C30, rotation), then the same local
         refinement as ``fit_learned`` from its best point (a wider first Newton
@@ -793,7 +795,7 @@ class HipSSBBackend:
                       verbose=False)
         keys = tuple(bounds)
         self._local_refine(refine, keys, [0.05 * (hi - lo) for lo, hi in bounds.values()], rounds, 16, bounds,
-                           seed, xatol, verbose, newton_step, self._NEWTON_FTOL)
+                           seed, xatol, verbose, newton_step, self._NEWTON_FTOL if newton_ftol is None else newton_ftol)
         self._elapsed_optimize = time.perf_counter() - t0
         return self
 

@@ -455,9 +455,11 @@ class HipSSBEngine:
                   a.ctypes.data_as(_abi._dp), _abi.ptr(phase), _abi.ptr(loss), self._stream())
         return phase, loss
 
-    def batch_phase_loss(self, mags, angles, rotations=None, out=None):
+    def batch_phase_loss(self, mags, angles, rotations=None, out=None, graph: bool = False):
         """Exact losses of T trials (``mags``/``angles``: [T, 14]; ``rotations``:
-        [T] rad or None) as a float32 device tensor, enqueued without a sync."""
+        [T] rad or None) as a float32 device tensor, enqueued without a sync.
+        ``graph``: replay the batch from a captured hipGraph (one upload
+        synchronisation; the same bits)."""
         torch = self.torch
         if not self._has_g:
             raise ValueError("no resident Fourier stack")
@@ -476,9 +478,9 @@ class HipSSBEngine:
         if out is None:
             out = torch.empty(t, dtype=torch.float32, device=self.plan.device)
         check_out(out, "out", t, torch.float32, self.plan.device)
-        _abi.call("ssb_batch_phase_loss", self.plan.handle, int(t), m.ctypes.data_as(_abi._dp),
-                  a.ctypes.data_as(_abi._dp), r.ctypes.data_as(_abi._dp) if r is not None else None,
-                  _abi.ptr(out), self._stream())
+        _abi.call("ssb_batch_phase_loss_graph" if graph else "ssb_batch_phase_loss", self.plan.handle, int(t),
+                  m.ctypes.data_as(_abi._dp), a.ctypes.data_as(_abi._dp),
+                  r.ctypes.data_as(_abi._dp) if r is not None else None, _abi.ptr(out), self._stream())
         return out
 
     def reconstruct_full(self, mags, angles):

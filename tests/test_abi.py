@@ -61,7 +61,7 @@ def test_version_and_stage_names(lib):
     assert lib.ssb_version() >= 1
     n = lib.ssb_stage_count()
     names = [lib.ssb_stage_name(i).decode() for i in range(n)]
-    assert names[0] == "mean_dp_stream" and "colfft_trotter" in names
+    assert names[:2] == ["superset", "mean_dp_stream"] and "colfft_trotter" in names
 
 
 def test_argument_errors_map_to_reference_exceptions(lib):

@@ -417,3 +417,34 @@ def test_detect_bf_radius_hard_disks():
     assert O.detect_bf_radius(edge) == ((0, 0), 16)            # max_r < 2: min(shape) // 4
     with pytest.raises(ValueError, match="no positive finite"):
         O.detect_bf_radius(np.zeros((8, 8), np.float32))
+
+
+def test_phase_variance_is_the_object_phase_variance():
+    """The batched search's objective: with a quadratic chi it is np.var of
+    the Fourier-sum object's phase (the pinned object path), and a C30 term
+    enters through the full chi (a different value)."""
+    data = C.poisson_data(124)
+    p = O.prepare(data, bf_radius=3, **C.POISSON_KW)
+    mags, angs = np.zeros(14), np.zeros(14)
+    mags[0] = -120.0
+    ph = np.angle(O.object_wave(p.G, p.dc, p.geom, -120.0, 0.0, 0.0))
+    assert O.phase_variance(p.G, p.dc, p.geom, mags, angs) == pytest.approx(float(np.var(ph)), rel=1e-12)
+    mags[4] = 4.0e5
+    assert abs(O.phase_variance(p.G, p.dc, p.geom, mags, angs) - float(np.var(ph))) > 1e-3 * float(np.var(ph))
+
+
+def test_phase_variance_peaks_at_the_true_defocus():
+    """Contrast of a weak-phase object's SSB phase is largest at the
+    generating defocus (scan 128, detector 64, C10 = 200 A)."""
+    from quantem.gpu_amd.ssb.synthetic import SyntheticConfig, generate
+    cfg = SyntheticConfig(scan=128, det=64, dose=1e6, seed=5, sigma_px=3.0, aberrations={"C10": 200.0})
+    data = generate(cfg, "cpu")
+    p = O.prepare(data, voltage_kV=300, semiangle_mrad=20.0, scan_sampling=cfg.step_A,
+                  det_sampling=cfg.det_sampling_mrad)
+    grid = np.arange(0.0, 401.0, 50.0)
+    var = []
+    for c10 in grid:
+        mags = np.zeros(14)
+        mags[0] = c10
+        var.append(O.phase_variance(p.G, p.dc, p.geom, mags, np.zeros(14)))
+    assert grid[int(np.argmax(var))] == 200.0
