@@ -123,6 +123,22 @@ int ssb_normalize_scan_u16(const uint16_t* src, int H, int W, int det_rows, int 
 int ssb_normalize_scan_typed(const void* src, int dtype, int H, int W, int det_rows, int det_cols, void* dst,
                              int T, void* stream);
 
+/*
+ * ssb_normalize_scan_u16 without any device allocation or synchronisation.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Its partial sums and pad frame live in a library-owned
+ * device array of QSSB_NOALLOC_SCRATCH_BYTES per GPU (held until the
+ * process ends); inputs whose workspace exceeds it (more than 65,536
+ * detector pixels at the largest scans) are rejected with QSSB_ERR_ARG
+ * before any launch.  Results are bit-identical to ssb_normalize_scan_u16.
+ * All calls on one GPU share the array: calls that overlap in time on one
+ * GPU (other streams or threads) race.
+ */
+#define QSSB_NOALLOC_SCRATCH_BYTES (4456448) /* 4 MiB + 256 KiB */
+int ssb_normalize_scan_u16_noalloc(const uint16_t* src, int H, int W, int det_rows, int det_cols, uint16_t* dst,
+                                   int T, void* stream);
+
 /* Library version (major*10000 + minor*100 + patch). */
 int ssb_version(void);
 

@@ -19,7 +19,7 @@ REPO = PKG.parent.parent
 LIB_DIR = PKG / "lib"
 LIB_NAME = "libquantem_ssb_hip.so"
 SOURCES = ["ssb_prepare.hip", "ssb_object.hip", "ssb_loss.hip", "ssb_learn.hip", "ssb_bslz4.hip", "ssb_bin.hip", "ssb_api.hip",
-           "ssb_trials.hip"]
+           "ssb_trials.hip", "ssb_noalloc.hip"]
 
 ARCH = os.environ.get("QSSB_OFFLOAD_ARCH", "gfx950")
 

@@ -84,6 +84,7 @@ SIGNATURES: dict[str, list] = {
                         _vp, _vp],
     "ssb_normalize_scan_typed": [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _vp],
     "ssb_bf_columns_u8": [_vp, _vp, _c_int, _c_int, _vp, _vp],
+    "ssb_normalize_scan_u16_noalloc": [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _vp],
     "ssb_batch_phase_loss_graph": [_vp, _c_int, _dp, _dp, _dp, _vp, _vp],
     "ssb_trials_create": [_pp, _c_int, _c_int, _pp],
     "ssb_trials_destroy": [_vp],
