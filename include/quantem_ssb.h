@@ -441,9 +441,10 @@ int ssb_bin_u16(const uint16_t* src, long long frames, int rows, int cols, int f
  * QSSB_TUNE_STREAM_WG: workgroups of the raw-data stream pass;
  * QSSB_TUNE_GATHER_PAIRS: complex FFTs (two BF pixels each) per gather
  * workgroup; QSSB_TUNE_TROTTER: tile-shape variant of the fused column FFT +
- * trotter; QSSB_TUNE_SUPERSET: 1 (default) the stream pass of uint16 frames
- * also copies a sampled bright-field box of every frame compactly for the
- * gather, 2 the gather reads the frames.  ssb_tune_candidates writes the
+ * trotter; QSSB_TUNE_SUPERSET: 2 (default) the gather reads the frames, 1
+ * the stream pass of uint16 frames also copies a sampled bright-field box of
+ * every frame compactly for the gather (bit-identical; its extra writes cost
+ * the stream pass more than the gather saves at 256x256x128x128).  ssb_tune_candidates writes the
  * allowed values for a scan size
  * (the first is the default) and returns their count; ssb_tune_set accepts
  * one of them or 0 (default); ssb_tune_get reports the value in use.  The

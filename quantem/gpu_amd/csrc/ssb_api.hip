@@ -150,7 +150,7 @@ int ssb_plan_create(int scan_n, int det_rows, int det_cols, int max_bf, double w
   p->D = det_rows * det_cols;
   p->max_bf = max_bf;
   p->nbf_hint = max_bf;
-  p->max_groups = object_groups(scan_n);
+  p->max_groups = trotter_groups_max(scan_n);
   p->wavelength = wavelength_A;
   p->semiangle = semiangle_rad;
   p->ang_y = ang_row_rad;
@@ -577,15 +577,16 @@ int ssb_object(ssb_plan* p, double c10, double c12, double phi12, void* obj_out,
   }
   const QuadChi m = quad_chi(p, c10, c12, phi12);
   HIP_TRY(launch_phasors(p->n, p->info, p->nbf_hint, geom_ptrs(p), m, p->rowph, p->colph, p->echi, s), "phasors");
+  int groups = object_groups(p->n);
   if (p->gcols == p->n / 2 + 1) {
     HIP_TRY(launch_trotter_slots(p->n, false, p->Gt, p->info, geom_ptrs(p), p->rowph, p->colph, p->echi, p->optics, p->tw,
-                                 p->S_part, nullptr, true, 0, s),
+                                 p->S_part, nullptr, true, 0, s, &groups),
             "trotter");
   } else {
     HIP_TRY(launch_trotter_full(p->n, p->Gt, p->info, geom_ptrs(p), p->rowph, m, p->optics, p->S_part, s),
             "trotter_full");
   }
-  HIP_TRY(launch_finish_object(p->n, p->S_part, p->info, p->tw, p->S,
+  HIP_TRY(launch_finish_object(p->n, p->S_part, groups, p->info, p->tw, p->S,
                                reinterpret_cast<float2*>(obj_out), phase_out, s),
           "finish_object");
   return QSSB_OK;
@@ -999,7 +1000,7 @@ static int recon_typed(ssb_plan* p, const ssb_plan* after, const void* data, int
   const long long frames = (long long)p->n * p->n;
   // bright-field superset (uint16 frames of whole 16-byte detector-row groups):
   // sampled box, copied compactly by the stream pass for the gather
-  const bool sup = dtype == QSSB_DTYPE_U16 && p->tune_superset != 2 && p->det_cols % 8 == 0 &&
+  const bool sup = dtype == QSSB_DTYPE_U16 && p->tune_superset == 1 && p->det_cols % 8 == 0 &&
                    (reinterpret_cast<uintptr_t>(data) & 15u) == 0;
   if (sup) {
     rc = alloc_superset(p);
@@ -1047,11 +1048,12 @@ static int recon_typed(ssb_plan* p, const ssb_plan* after, const void* data, int
   mark(6);
   const QuadChi m = quad_chi(p, c10, c12, phi12);
   HIP_TRY(launch_phasors(p->n, p->info, p->nbf_hint, geom_ptrs(p), m, p->rowph, p->colph, p->echi, s), "phasors");
+  int groups = object_groups(p->n);
   HIP_TRY(launch_trotter_slots(p->n, true, p->H, p->info, geom_ptrs(p), p->rowph, p->colph, p->echi, p->optics, p->tw, p->S_part,
-                               nullptr, true, p->tune_trotter, s),
+                               nullptr, true, p->tune_trotter, s, &groups),
           "colfft_trotter");
   mark(7);
-  HIP_TRY(launch_finish_object(p->n, p->S_part, p->info, p->tw, p->S,
+  HIP_TRY(launch_finish_object(p->n, p->S_part, groups, p->info, p->tw, p->S,
                                reinterpret_cast<float2*>(obj_out), phase_out, s),
           "finish_object");
   mark(8);
@@ -1160,8 +1162,8 @@ int ssb_tune_candidates(int scan_n, int knob, int* values_host, int capacity) {
       for (int i = 0; i < n; ++i) vals[i] = i;
       break;
     case QSSB_TUNE_SUPERSET:
-      vals[0] = 1;
-      vals[1] = 2;
+      vals[0] = 2;
+      vals[1] = 1;
       n = 2;
       break;
     default:
@@ -1201,7 +1203,7 @@ int ssb_tune_get(ssb_plan* p, int knob, int* value_host) {
       *value_host = p->tune_gather_pairs ? p->tune_gather_pairs : gather_pairs_default(p->n);
       break;
     case QSSB_TUNE_TROTTER: *value_host = p->tune_trotter; break;
-    case QSSB_TUNE_SUPERSET: *value_host = p->tune_superset ? p->tune_superset : 1; break;
+    case QSSB_TUNE_SUPERSET: *value_host = p->tune_superset ? p->tune_superset : 2; break;
     default: return fail(QSSB_ERR_ARG, "unknown tuning knob " + std::to_string(knob));
   }
   return QSSB_OK;

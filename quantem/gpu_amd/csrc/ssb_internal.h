@@ -165,8 +165,10 @@ struct ApIso {
   float scale;      // lambda / ang
 };
 
-// Number of BF groups (partial planes) of the object-path trotter.
+// Number of BF groups (partial planes) of the object-path trotter's tile
+// kernels, and the most any of its kernels uses (the partial-plane buffer).
 int object_groups(int N);
+int trotter_groups_max(int N);
 bool optics_isotropic(const Optics& o);
 ApIso iso_aperture(const Optics& o, df& rout2_df);
 
@@ -180,20 +182,22 @@ hipError_t launch_phasors(int N, const BfInfo* info, int nbf_hint, const GeomPtr
 // (fused scan-column FFT; from_h) or from the resident stack Gt[b][c][q],
 // c = 0..N/2.  Writes partial planes S_part[g][c][q] (do_trotter) and, if Gt
 // is given with from_h, the resident stack itself.
-// variant: tile shape of the reconstruction path's trotter, 0 (default) to
-// trotter_variants(N) - 1; other paths always use variant 0.
+// variant: kernel of the reconstruction path's trotter, 0 (default) to
+// trotter_variants(N) - 1 (at N = 256, 0 is the wave-local kernel and 1..3
+// the tile shapes); other paths always use variant 0.  groups_used receives
+// the number of partial planes written (launch_finish_object's `groups`).
 int trotter_variants(int N);
 hipError_t launch_trotter_slots(int N, bool from_h, const float2* src, const BfInfo* info, const GeomPtrs& gp,
                                 const float2* rowph, const float2* colph, const float2* echi, const Optics& o,
                                 const float2* tw, float2* S_part, float2* Gt, bool do_trotter, int variant,
-                                hipStream_t s);
+                                hipStream_t s, int* groups_used = nullptr);
 // Trotter from an imported full-plane (non-Hermitian) stack Gt[b][c][q], c < N.
 hipError_t launch_trotter_full(int N, const float2* Gt, const BfInfo* info, const GeomPtrs& gp, const float2* rowph,
                                QuadChi m, const Optics& o, float2* S_part, hipStream_t s);
 // Sum the partial planes (Kahan, fixed order), divide by N_bf, set the DC
 // bin, inverse 2D FFT (1/N^2), write object (row-major c64) and phase.
-hipError_t launch_finish_object(int N, const float2* S_part, const BfInfo* info, const float2* tw, float2* S,
-                                float2* obj, float* phase, hipStream_t s);
+hipError_t launch_finish_object(int N, const float2* S_part, int groups, const BfInfo* info, const float2* tw,
+                                float2* S, float2* obj, float* phase, hipStream_t s);
 
 // Exact per-BF phase path: BF pixels per chunk of column-transformed planes,
 // BF splits of the moment planes.

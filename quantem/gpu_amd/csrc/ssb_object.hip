@@ -356,6 +356,392 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
 }
 
 // ------------------------------------------------------------------------
+// Wave-local fused column FFT + trotter of the reconstruction (row-transformed
+// stack H, isotropic detector sampling).  A workgroup of 8 waves takes one
+// BF group and 16 adjacent slot columns -- the 128 bytes of every row of H
+// that one cache line holds -- and each wave owns 2 of those columns for
+// every BF pixel of the group.  Lane l holds rows l + 64 m (m < R = N / 64):
+// one 16-byte load per row brings both columns.  The column FFT needs no
+// workgroup barrier: an R-point DFT in registers (stride-64 elements) with
+// twiddles w_N^(l k1), then 64-point Stockham radix-4 stages over the 2R
+// sequences through the wave's own LDS (every lane reads its operands before
+// any lane writes), so X[k1 + R k2] = Z_k1[k2] and lane l reads back its
+// frequencies q = l + 64 m.
+// Per point the trotter term is that of k_trotter_slots (same double-float
+// apertures, phasors and guard), summed over the group's BF pixels in
+// float64.  The aperture of a point far from the soft edge is decided from
+// a float32 estimate of d (error < 2e-5 for |terms| < 32, inside the 1e-4
+// margin of aperture_sum); only points near the edge evaluate d in
+// double-float.  Outputs B of the mirror points are written as -conj(A);
+// column N/2 (slot 0) is accumulated with A; the Nyquist-row outputs
+// S(N/2, N - j) are summed after the BF loop, one BF pixel per lane, from
+// the G_b(N/2, j) the loop leaves in LDS.  The column terms of d for each
+// (BF pixel, column) are computed once per workgroup into LDS, 64 BF pixels
+// at a time; H and the phasors of the next BF pixel are loaded while this
+// one is transformed.
+// ------------------------------------------------------------------------
+constexpr int kWaveTrotterWaves = 8;
+constexpr int kWaveTrotterGroups = 64;
+constexpr int kWaveChunk = 64;           // BF pixels per column-term table / Nyquist pass
+
+__device__ __forceinline__ void wave_lds_fence() { asm volatile("" ::: "memory"); }
+
+// Column terms of d(Q -/+ k) for one (BF pixel, column): (Rout^2 - |k|^2) -
+// qy^2 +/- 2 qy ky (isotropic sampling).
+struct ColTerm {
+  df A, B;
+};
+
+__device__ __forceinline__ ColTerm col_term(df rout2, df kx, df ky, df qy) {
+  const df cbv = df_sub(rout2, df_add(df_mul(kx, kx), df_mul(ky, ky)));
+  const df base = df_sub(cbv, df_mul(qy, qy));
+  const df qk = df_mul(qy, df{2.0f * ky.hi, 2.0f * ky.lo});
+  return ColTerm{df_add(base, qk), df_sub(base, qk)};
+}
+
+// Per-BF operands of one wave other than H (loaded one BF pixel ahead).
+template <int R>
+struct WaveBf {
+  float2 rowA[R];       // e^{i qx a_b} at the lane's rows
+  float2 colp[2];       // e^{i qy b_b} at the wave's columns
+  float2 colHalf;       // ... at column N/2
+  float kx, kxl, ak;
+};
+
+template <int N, int R>
+__device__ __forceinline__ void wave_bf_load(WaveBf<R>& o, const float2* __restrict__ rowph_g,
+                                             const float2* __restrict__ colph_g, const GeomPtrs& gp, int b,
+                                             int lane, int j0) {
+  constexpr int HALF = N / 2, GCOLS = HALF + 1;
+#pragma unroll
+  for (int m = 0; m < R; ++m) o.rowA[m] = rowph_g[(long long)b * N + lane + 64 * m];
+  o.colp[0] = colph_g[(long long)b * GCOLS + j0];
+  o.colp[1] = colph_g[(long long)b * GCOLS + j0 + 1];
+  o.colHalf = colph_g[(long long)b * GCOLS + HALF];
+  o.kx = gp.kx[b];
+  o.kxl = gp.kxl[b];
+  o.ak = gp.ak[b];
+}
+
+// Normalised overlap weight and the term G * conj(g) of one point.
+__device__ __forceinline__ float2 trotter_point(float2 G, float am, float app, float2 echi, float2 eth, float ak) {
+  const float wr = (am - app) * echi.x;
+  const float wi = -(am + app) * echi.y;
+  float gr = ak * (eth.x * wr - eth.y * wi);
+  float gi = ak * (eth.x * wi + eth.y * wr);
+  const float mag2 = gr * gr + gi * gi;
+  const float sc = (mag2 > 1e-16f) ? rsqrtf(mag2) : 1e8f;
+  gr *= sc;
+  gi *= sc;
+  return make_float2(G.x * gr + G.y * gi, G.y * gr - G.x * gi);
+}
+
+// The aperture from a float32 estimate of d when it lies beyond the 1e-4
+// margin of the soft band (the value aperture_sum gives there); false when
+// the estimate is too close to the band to decide.
+__device__ __forceinline__ bool aperture_decided(float est, const ApIso& ap, float& a) {
+  constexpr float kMargin = 1e-4f;
+  a = (est >= ap.d_in + kMargin) ? 1.0f : 0.0f;
+  return est >= ap.d_in + kMargin || est <= ap.d_out - kMargin;
+}
+
+// One wave's share of a work item: BF pixels [b0, b1) for slot columns j0,
+// j0 + 1; SLOT0 for the wave holding slot 0, whose first column packs the
+// real columns 0 and N/2.  Every wave of the workgroup calls it with the
+// same BF range (the barriers of the column-term tables match).
+template <int N, bool SLOT0>
+__device__ __forceinline__ void wave_trotter_item(const float2* __restrict__ H, const GeomPtrs& gp,
+                                                  const float2* __restrict__ rowph_g,
+                                                  const float2* __restrict__ colph_g,
+                                                  const float2* __restrict__ echi_g, const ApIso& ap, df rout2,
+                                                  const float2* __restrict__ tw, float2* __restrict__ xb,
+                                                  float2* __restrict__ nyqz, ColTerm (*colT)[2 * kWaveTrotterWaves + 1],
+                                                  const float2 (*echiS)[N], const df* __restrict__ qxS,
+                                                  const df* __restrict__ q2xS, int cg, int w, int tid, int b0, int b1,
+                                                  float2* __restrict__ sp) {
+  constexpr int HALF = N / 2;
+  constexpr int R = N / 64;
+  constexpr int SP = 65;                 // padded sequence stride in the wave's LDS
+  constexpr int CG = 2 * kWaveTrotterWaves;
+  constexpr int NBH = SLOT0 ? R : 1;     // column N/2 accumulators (slot 0 only)
+  const int lane = tid & 63;
+  const int j0 = cg * CG + 2 * w;
+  double aR[2][R], aI[2][R], hR[NBH], hI[NBH];
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      aR[c][m] = 0.0;
+      aI[c][m] = 0.0;
+    }
+#pragma unroll
+  for (int i = 0; i < NBH; ++i) {
+    hR[i] = 0.0;
+    hI[i] = 0.0;
+  }
+  double nyR[2] = {0.0, 0.0}, nyI[2] = {0.0, 0.0};   // Nyquist-row sums (uniform)
+  const float4* hrow = reinterpret_cast<const float4*>(H + (long long)lane * HALF + j0);
+  constexpr long long ROWS64 = 64LL * HALF / 2;     // float4 between rows l and l + 64
+  constexpr long long PER_B = (long long)N * HALF / 2;
+  float4 h[R];
+  WaveBf<R> nx;
+  if (b0 < b1) {
+#pragma unroll
+    for (int m = 0; m < R; ++m) h[m] = hrow[(long long)b0 * PER_B + m * ROWS64];
+    wave_bf_load<N, R>(nx, rowph_g, colph_g, gp, b0, lane, j0);
+  }
+  for (int c0 = b0; c0 < b1; c0 += kWaveChunk) {
+    const int n = min(kWaveChunk, b1 - c0);
+    // column terms of the chunk's BF pixels at the workgroup's 16 columns
+    // and column N/2, computed once for all waves
+    __syncthreads();
+    for (int e = tid; e < n * (CG + 1); e += 64 * kWaveTrotterWaves) {
+      const int bi = e / (CG + 1), ci = e - (e / (CG + 1)) * (CG + 1);
+      const int b = c0 + bi;
+      const int col = (ci < CG) ? cg * CG + ci : HALF;
+      colT[bi][ci] = col_term(rout2, df{gp.kx[b], gp.kxl[b]}, df{gp.ky[b], gp.kyl[b]},
+                              df{gp.qy[col], gp.qyl[col]});
+    }
+    __syncthreads();
+    for (int b = c0; b < c0 + n; ++b) {
+      const WaveBf<R> cu = nx;
+      if (b + 1 < b1) wave_bf_load<N, R>(nx, rowph_g, colph_g, gp, b + 1, lane, j0);
+      // column FFT: in-register radix-4 over m, twiddle, into the wave's LDS.
+      // The exchange runs across the lanes of this wave only: a compiler
+      // memory barrier at each phase boundary keeps every LDS access in
+      // program order (the hardware keeps a wave's LDS operations in order).
+      wave_lds_fence();
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        float2 x[R];
+#pragma unroll
+        for (int m = 0; m < R; ++m) x[m] = c ? make_float2(h[m].z, h[m].w) : make_float2(h[m].x, h[m].y);
+        const float2 apc = cadd(x[0], x[2]), amc = csub(x[0], x[2]);
+        const float2 bpd = cadd(x[1], x[3]), bmd = cmul_mi<false>(csub(x[1], x[3]));
+        xb[(c * R + 0) * SP + lane] = cadd(apc, bpd);
+        xb[(c * R + 1) * SP + lane] = cmul(cadd(amc, bmd), tw[lane & (N - 1)]);
+        xb[(c * R + 2) * SP + lane] = cmul(csub(apc, bpd), tw[(2 * lane) & (N - 1)]);
+        xb[(c * R + 3) * SP + lane] = cmul(csub(amc, bmd), tw[(3 * lane) & (N - 1)]);
+      }
+      // H of the next BF pixel: issued once this one's rows are consumed
+      if (b + 1 < b1) {
+#pragma unroll
+        for (int m = 0; m < R; ++m) h[m] = hrow[(long long)(b + 1) * PER_B + m * ROWS64];
+      }
+      // 64-point Stockham radix-4 stages over the 2R sequences (two
+      // butterflies per lane): all operands are read before any is written
+#pragma unroll
+      for (int ns = 1; ns < 64; ns *= 4) {
+        wave_lds_fence();
+        float2 v[2][4];
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+          const int it = lane + 64 * p;
+          const int sq = it >> 4, jj = it & 15, k = jj & (ns - 1);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            float2 xv = xb[sq * SP + jj + 16 * r];
+            if (ns > 1 && r > 0) xv = cmul(xv, tw[(4 * r * k * (16 / ns)) & (N - 1)]);
+            v[p][r] = xv;
+          }
+        }
+        wave_lds_fence();
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+          const int it = lane + 64 * p;
+          const int sq = it >> 4, jj = it & 15, k = jj & (ns - 1);
+          const int base = sq * SP + (jj / ns) * ns * 4 + k;
+          const float2 apc = cadd(v[p][0], v[p][2]), amc = csub(v[p][0], v[p][2]);
+          const float2 bpd = cadd(v[p][1], v[p][3]), bmd = cmul_mi<false>(csub(v[p][1], v[p][3]));
+          xb[base] = cadd(apc, bpd);
+          xb[base + ns] = cadd(amc, bmd);
+          xb[base + 2 * ns] = csub(apc, bpd);
+          xb[base + 3 * ns] = csub(amc, bmd);
+        }
+      }
+      wave_lds_fence();
+      // G_b(N/2, j0 + c) for the deferred Nyquist pass (lane 0 holds q = N/2)
+      if (lane == 0) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c) nyqz[2 * (b - c0) + c] = xb[(c * R + (HALF & 3)) * SP + (HALF >> 2)];
+      }
+      const ColTerm ct[2] = {colT[b - c0][2 * w], colT[b - c0][2 * w + 1]};
+      const float ak = cu.ak;
+      const df kx2{2.0f * cu.kx, 2.0f * cu.kxl};
+#pragma unroll
+      for (int m = 0; m < R; ++m) {
+        const int q = lane + 64 * m;
+        const df qx = qxS[q];
+        const df q2x = q2xS[q];
+        // float32 estimates of the row terms +/- qx 2kx - qx^2
+        const float uq = fmaf(qx.hi, kx2.hi, -q2x.hi);
+        const float vq = -fmaf(qx.hi, kx2.hi, q2x.hi);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          const float2 z = xb[(c * R + (lane & 3)) * SP + (lane >> 2) + 16 * m];
+          float2 GA;
+          if (SLOT0 && c == 0) {
+            const int qq = (N - q) & (N - 1);
+            const float2 zm = xb[(qq & 3) * SP + (qq >> 2)];
+            GA = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
+          } else {
+            GA = z;
+          }
+          float am, app;
+          const bool da = aperture_decided(uq + ct[c].A.hi, ap, am);
+          const bool dp = aperture_decided(vq + ct[c].B.hi, ap, app);
+          if (!(da && dp)) {
+            const df u = df_mul(qx, kx2);
+            if (!da) am = aperture_from_d(df_add(df_sub(u, q2x), ct[c].A), ap);
+            if (!dp) app = aperture_from_d(df_add(df_neg(df_add(q2x, u)), ct[c].B), ap);
+          }
+          if (am != 0.0f || app != 0.0f) {
+            const float2 t = trotter_point(GA, am, app, echiS[2 * w + c][q], cmul(cu.rowA[m], cu.colp[c]), ak);
+            aR[c][m] += (double)t.x;
+            aI[c][m] += (double)t.y;
+          }
+          if constexpr (SLOT0) {
+            if (c == 0) {
+              // output B of slot 0: column N/2 at row q
+              const int qq = (N - q) & (N - 1);
+              const float2 zm = xb[(qq & 3) * SP + (qq >> 2)];
+              const float2 GB = make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x));
+              const ColTerm th = colT[b - c0][CG];
+              float bm, bp;
+              const bool dm = aperture_decided(uq + th.A.hi, ap, bm);
+              const bool dq = aperture_decided(vq + th.B.hi, ap, bp);
+              if (!(dm && dq)) {
+                const df u = df_mul(qx, kx2);
+                if (!dm) bm = aperture_from_d(df_add(df_sub(u, q2x), th.A), ap);
+                if (!dq) bp = aperture_from_d(df_add(df_neg(df_add(q2x, u)), th.B), ap);
+              }
+              if (bm != 0.0f || bp != 0.0f) {
+                const float2 t = trotter_point(GB, bm, bp, echiS[CG][q], cmul(cu.rowA[m], cu.colHalf), ak);
+                hR[m] += (double)t.x;
+                hI[m] += (double)t.y;
+              }
+            }
+          }
+          // one point at a time: keeps the scheduler from overlapping the
+          // points' temporaries
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+    }
+    // Nyquist-row terms of the chunk: lane i takes BF pixel c0 + i
+    wave_lds_fence();
+    {
+      float2 t[2] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+      if (lane < n) {
+        const int b = c0 + lane;
+        const float ak = gp.ak[b];
+        const df u = df_mul(qxS[HALF], df{2.0f * gp.kx[b], 2.0f * gp.kxl[b]});
+        const df rowTA = df_sub(u, q2xS[HALF]);
+        const df rowTB = df_neg(df_add(q2xS[HALF], u));
+        const float2 rA = rowph_g[(long long)b * N + HALF];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          const int j = j0 + c;
+          if (j == 0) continue;          // slot 0: column N/2 holds this point
+          const ColTerm ctn = colT[lane][2 * w + c];
+          const float bm = aperture_sum(rowTA, ctn.B, ap);
+          const float bp = aperture_sum(rowTB, ctn.A, ap);
+          if (bm != 0.0f || bp != 0.0f) {
+            const float2 echiB = echi_g[(long long)((N - j) & (N - 1)) * N + HALF];
+            const float2 eB = cmul(rA, cconj(colph_g[(long long)b * (HALF + 1) + j]));
+            t[c] = trotter_point(cconj(nyqz[2 * lane + c]), bm, bp, echiB, eB, ak);
+          }
+        }
+      }
+      // fixed-order tree over the lanes, in float64
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        double re = (double)t[c].x, im = (double)t[c].y;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+          re += __shfl_xor(re, off, 64);
+          im += __shfl_xor(im, off, 64);
+        }
+        nyR[c] += re;
+        nyI[c] += im;
+      }
+    }
+    wave_lds_fence();
+  }
+#pragma unroll
+  for (int m = 0; m < R; ++m) {
+    const int q = lane + 64 * m;
+    const int qm = (N - q) & (N - 1);
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int j = j0 + c;
+      const float2 outA = make_float2((float)aR[c][m], (float)aI[c][m]);
+      sp[(long long)j * N + q] = outA;
+      if (j == 0) {
+        if constexpr (SLOT0) sp[(long long)HALF * N + q] = make_float2((float)hR[m], (float)hI[m]);
+      } else if (q != HALF) {
+        sp[(long long)(N - j) * N + qm] = make_float2(-outA.x, outA.y);
+      } else {
+        sp[(long long)(N - j) * N + HALF] = make_float2((float)nyR[c], (float)nyI[c]);
+      }
+    }
+  }
+}
+
+template <int N>
+__global__ void __launch_bounds__(64 * kWaveTrotterWaves)
+k_trotter_wave(const float2* __restrict__ H, const BfInfo* __restrict__ info, GeomPtrs gp,
+               const float2* __restrict__ rowph_g, const float2* __restrict__ colph_g,
+               const float2* __restrict__ echi_g, ApIso ap, df rout2, const float2* __restrict__ tw_g, int groups,
+               float2* __restrict__ S_part) {
+  constexpr int HALF = N / 2;
+  constexpr int R = N / 64;
+  static_assert(R == 4, "the in-register pass is radix 4");
+  constexpr int W = kWaveTrotterWaves;
+  constexpr int CG = 2 * W;              // slot columns per workgroup
+  constexpr int NCG = HALF / CG;
+  constexpr int SP = 65;
+  __shared__ float2 tw[N];
+  __shared__ float2 xbuf[W][2 * R * SP];
+  __shared__ float2 nyqz[W][2 * kWaveChunk];
+  __shared__ ColTerm colT[kWaveChunk][CG + 1];
+  __shared__ float2 echiS[CG + 1][N];    // e^{i chi} of the 16 columns and of column N/2
+  __shared__ df qxS[N], q2xS[N];
+  const int tid = threadIdx.x;
+  const int w = tid >> 6;
+  for (int i = tid; i < N; i += 64 * W) {
+    tw[i] = tw_g[i];
+    const df qx{gp.qx[i], gp.qxl[i]};
+    qxS[i] = qx;
+    q2xS[i] = df_mul(qx, qx);
+  }
+  const int nbf = info->nbf;
+  const long long items = (long long)groups * NCG;
+  for (long long item = blockIdx.x; item < items; item += gridDim.x) {
+    // workgroups go round-robin over the 8 XCDs; rotating the column group
+    // by the BF group spreads the heavier slot-0 items over all of them
+    const int g = (int)(item / NCG);
+    const int cg = (int)((item + g) % NCG);
+    int b0, b1;
+    group_range(g, groups, nbf, b0, b1);
+    __syncthreads();
+    for (int i = tid; i < (CG + 1) * N; i += 64 * W) {
+      const int c = i / N, q = i - (i / N) * N;
+      const int col = (c < CG) ? cg * CG + c : HALF;
+      echiS[c][q] = echi_g[(long long)col * N + q];
+    }
+    // (the first column-term table's barrier orders these writes)
+    float2* sp = S_part + (long long)g * N * N;
+    if (cg == 0 && w == 0)
+      wave_trotter_item<N, true>(H, gp, rowph_g, colph_g, echi_g, ap, rout2, tw, xbuf[w], nyqz[w], colT, echiS, qxS,
+                                 q2xS, cg, w, tid, b0, b1, sp);
+    else
+      wave_trotter_item<N, false>(H, gp, rowph_g, colph_g, echi_g, ap, rout2, tw, xbuf[w], nyqz[w], colT, echiS, qxS,
+                                  q2xS, cg, w, tid, b0, b1, sp);
+  }
+}
+
+// ------------------------------------------------------------------------
 // Trotter from a full (non-Hermitian) stack Gt[b][c][q], c < N: each full
 // column is its own output (test seam for imported full-plane stacks).
 // ------------------------------------------------------------------------
@@ -550,17 +936,33 @@ hipError_t launch_phasors(int N, const BfInfo* info, int nbf_hint, const GeomPtr
 // trotter on the row-transformed stack (the reconstruction path); variant 0
 // is the default, the others are the tuner's candidates.  SG is even there:
 // each lane's 16-byte staging copy moves two adjacent slots of a row of H.
-int trotter_variants(int N) { return N == 1024 ? 1 : 3; }
+// At N = 256 variant 0 is the wave-local kernel (the object path of a
+// reconstruction: row-transformed stack, isotropic sampling) and 1..3 are
+// the workgroup-tile shapes below; elsewhere the tile shapes only.
+int trotter_variants(int N) { return N == 1024 ? 1 : (N == 256 ? 4 : 3); }
+
+int trotter_groups_max(int N) { return N == 256 ? kWaveTrotterGroups : object_groups(N); }
 
 hipError_t launch_trotter_slots(int N, bool from_h, const float2* src, const BfInfo* info, const GeomPtrs& gp,
                                 const float2* rowph, const float2* colph, const float2* echi, const Optics& o,
                                 const float2* tw, float2* S_part, float2* Gt, bool do_trotter, int variant,
-                                hipStream_t s) {
+                                hipStream_t s, int* groups_used) {
   df rout2;
   const ApIso ap = iso_aperture(o, rout2);
-  const int groups = object_groups(N);
   const int dt = do_trotter ? 1 : 0;
   const bool iso = optics_isotropic(o);
+  if (N == 256 && from_h && iso && do_trotter && !Gt && variant == 0) {
+    constexpr int groups = kWaveTrotterGroups;
+    const unsigned grid = grid_items((long long)groups * ((256 / 2) / (2 * kWaveTrotterWaves)), 65536);
+    hipLaunchKernelGGL((k_trotter_wave<256>), dim3(grid), dim3(64 * kWaveTrotterWaves), 0, s, src, info, gp, rowph,
+                       colph, echi, ap, rout2, tw, groups, S_part);
+    if (groups_used) *groups_used = groups;
+    return hipGetLastError();
+  }
+  // workgroup-tile shapes: at N = 256 their variants follow the wave-local one
+  if (N == 256 && variant > 0) variant -= 1;
+  const int groups = object_groups(N);
+  if (groups_used) *groups_used = groups;
 #define QSSB_SLOTS_ONE(NN, SGV, BL, FH, IS)                                                                 \
   hipLaunchKernelGGL((k_trotter_slots<NN, SGV, BL, FH, IS>), dim3(grid), dim3(BL), 0, s, src, info, gp, rowph, \
                      colph, echi, ap, rout2, o, tw, groups, S_part, Gt, dt)
@@ -641,9 +1043,9 @@ hipError_t launch_trotter_full(int N, const float2* Gt, const BfInfo* info, cons
   return hipGetLastError();
 }
 
-hipError_t launch_finish_object(int N, const float2* S_part, const BfInfo* info, const float2* tw, float2* S,
-                                float2* obj, float* phase, hipStream_t s) {
-  const int G = object_groups(N);
+hipError_t launch_finish_object(int N, const float2* S_part, int groups, const BfInfo* info, const float2* tw,
+                                float2* S, float2* obj, float* phase, hipStream_t s) {
+  const int G = groups;
   switch (N) {
     case 128:
       hipLaunchKernelGGL((k_reduce_ifft_cols<128, 4>), dim3(128 / 4), dim3(256), 0, s, S_part, info, G, tw, S);

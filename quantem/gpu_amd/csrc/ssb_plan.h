@@ -50,7 +50,7 @@ struct ssb_plan {
   int tune_stream_wg = 0;
   int tune_gather_pairs = 0;
   int tune_trotter = 0;
-  int tune_superset = 0;     // 0/1: compact BF superset copy on; 2: off
+  int tune_superset = 0;     // 1: compact BF superset copy; 0/2: the gather reads the frames
   // bright-field superset of the reconstruction path (allocated on first use)
   uint32_t* sup_part = nullptr;
   qssb::SupBox* sup_box = nullptr;

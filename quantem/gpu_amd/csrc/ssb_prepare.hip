@@ -78,8 +78,11 @@ k_dp_partial(const uint16_t* __restrict__ data, long long frames, int D, int fra
 // The uint16 stream pass of a reconstruction: the same partial sums, and
 // every frame's superset box copied compactly to C[frame][row - r0][col - c0]
 // (bh * bw uint16 per frame), so the BF gather reads contiguous box rows
-// instead of 128-byte lines under scattered BF pixels.  A thread's 8 pixels
-// never cross a detector row (det_cols % 8 == 0).
+// instead of 128-byte lines under scattered BF pixels.  The box spans whole
+// 8-column groups (c0 % 8 == 0, bw % 8 == 0) and det_cols % 8 == 0, so a
+// thread's 8 pixels are all inside or all outside it and each inside thread
+// copies a frame with one 16-byte store (C rows and frames stay 16-byte
+// aligned).  Plain stores: the gather reads C soon after.
 __global__ void __launch_bounds__(256)
 k_dp_partial_sup(const uint16_t* __restrict__ data, long long frames, int D, int det_cols, int frames_per_range,
                  uint32_t* __restrict__ partial, const SupBox* __restrict__ boxp, uint16_t* __restrict__ C) {
@@ -91,55 +94,49 @@ k_dp_partial_sup(const uint16_t* __restrict__ data, long long frames, int D, int
   const SupBox box = *boxp;
   const int row = pix0 / det_cols, col0 = pix0 - row * det_cols;
   const long long SB = (long long)box.bh * box.bw;
-  int e_lo = 8, e_hi = 0;
-  long long cbase = 0;
-  if (box.bw > 0 && row >= box.r0 && row < box.r0 + box.bh) {
-    e_lo = max(0, box.c0 - col0);
-    e_hi = min(8, box.c0 + box.bw - col0);
-    cbase = (long long)(row - box.r0) * box.bw + (col0 - box.c0);
-  }
-  const bool copy = e_lo < e_hi;
+  const bool copy = box.bw > 0 && row >= box.r0 && row < box.r0 + box.bh && col0 >= box.c0 &&
+                    col0 < box.c0 + box.bw;
   uint32_t acc[8];
 #pragma unroll
   for (int e = 0; e < 8; ++e) acc[e] = 0u;
-  const uint16_t* p = data + f0 * (long long)D + pix0;
-  uint16_t* cp = C + f0 * SB + cbase;
-  auto put = [&](const u32x4& v, uint16_t* dst) {
-#pragma unroll
-    for (int e = 0; e < 8; ++e)
-      if (e >= e_lo && e < e_hi) dst[e] = (uint16_t)((v[e >> 1] >> (16 * (e & 1))) & 0xFFFFu);
+  u32x4* cp = reinterpret_cast<u32x4*>(
+      C + f0 * SB + (copy ? (long long)(row - box.r0) * box.bw + (col0 - box.c0) : 0));
+  const long long cstep = SB / 8;   // u32x4 per frame of C
+  // Loads and stores share one in-order completion counter: the loads of
+  // the next frame pair are issued before the stores of the current one, so
+  // waiting for those loads never waits for the stores.  The last pair's
+  // prefetch re-reads the range's last frame (in bounds, unused).
+  const long long last = f1 - 1;
+  auto frame = [&](long long f) {
+    return __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(data + (f < last ? f : last) * D + pix0));
   };
-  long long f = f0;
-  for (; f + 2 <= f1; f += 2) {
-    u32x4 v[2];
-#pragma unroll
-    for (int u = 0; u < 2; ++u)
-      v[u] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p + (long long)u * D));
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-#pragma unroll
-      for (int w = 0; w < 4; ++w) {
-        acc[2 * w] += v[u][w] & 0xFFFFu;
-        acc[2 * w + 1] += v[u][w] >> 16;
-      }
-    }
-    if (copy) {
-      put(v[0], cp);
-      put(v[1], cp + SB);
-    }
-    p += 2LL * D;
-    cp += 2 * SB;
-  }
-  for (; f < f1; ++f) {
-    u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
+  auto add = [&](const u32x4& v) {
 #pragma unroll
     for (int w = 0; w < 4; ++w) {
       acc[2 * w] += v[w] & 0xFFFFu;
       acc[2 * w + 1] += v[w] >> 16;
     }
-    if (copy) put(v, cp);
-    p += D;
-    cp += SB;
+  };
+  long long f = f0;
+  if (f + 2 <= f1) {
+    u32x4 cur0 = frame(f), cur1 = frame(f + 1);
+    for (; f + 2 <= f1; f += 2) {
+      const u32x4 nxt0 = frame(f + 2), nxt1 = frame(f + 3);
+      if (copy) {
+        cp[0] = cur0;
+        cp[cstep] = cur1;
+      }
+      add(cur0);
+      add(cur1);
+      cp += 2 * cstep;
+      cur0 = nxt0;
+      cur1 = nxt1;
+    }
+  }
+  if (f < f1) {
+    const u32x4 v = frame(f);
+    add(v);
+    if (copy) cp[0] = v;
   }
   uint32_t* out = partial + (long long)blockIdx.y * D + pix0;
 #pragma unroll
@@ -589,14 +586,17 @@ k_select_bf(const float* __restrict__ mean_g, const unsigned long long* __restri
 // ------------------------------------------------------------------------
 // Superset box of the bright field, before the stream pass: per-pixel sums
 // of kSupSamples frames spread evenly over the scan (kSupSplits partial
-// sums per pixel), then one workgroup takes the box around every pixel above
-// a quarter of the brightest, widened by kSupMargin pixels.  The exact BF
-// selection later checks that it lies inside (BfInfo::sup_ok); if not, the
-// gather reads the frames.
+// sums per pixel, kSupSamples / kSupSplits loads per thread, all in flight
+// together), then one workgroup takes the box around every pixel above a
+// quarter of the brightest, widened by kSupMargin pixels and out to whole
+// 8-column groups.  The exact BF selection later checks that it lies inside
+// (BfInfo::sup_ok); if not, the gather reads the frames.
 // ------------------------------------------------------------------------
-constexpr int kSupSamples = 512;
+constexpr int kSupSamples = 128;
 constexpr int kSupSplits = 8;
 constexpr int kSupMargin = 2;
+constexpr int kSupPer = kSupSamples / kSupSplits;
+constexpr int kBoxPix = 8;   // pixels per thread and round of the box pass (64 loads in flight)
 
 __global__ void __launch_bounds__(256)
 k_sup_sample(const uint16_t* __restrict__ data, long long frames, int D, uint32_t* __restrict__ part) {
@@ -604,9 +604,15 @@ k_sup_sample(const uint16_t* __restrict__ data, long long frames, int D, uint32_
   if (pix >= D) return;
   const long long n = frames < kSupSamples ? frames : kSupSamples;
   const long long stride = frames / n;
+  uint32_t v[kSupPer];
+#pragma unroll
+  for (int k = 0; k < kSupPer; ++k) {
+    const long long i = blockIdx.y + (long long)k * kSupSplits;
+    v[k] = i < n ? data[i * stride * D + pix] : 0u;
+  }
   uint32_t acc = 0u;
-#pragma unroll 8
-  for (long long i = blockIdx.y; i < n; i += kSupSplits) acc += data[i * stride * D + pix];
+#pragma unroll
+  for (int k = 0; k < kSupPer; ++k) acc += v[k];
   part[(long long)blockIdx.y * D + pix] = acc;
 }
 
@@ -615,23 +621,46 @@ k_sup_box(const uint32_t* __restrict__ part, int det_rows, int det_cols, SupBox*
   __shared__ float sh_f[2 * MASK_WAVES];
   const int tid = threadIdx.x;
   const int D = det_rows * det_cols;
-  auto total = [&](int i) {
-    uint32_t t = 0u;
+  // totals of the pixels base + tid + k * MASK_THREADS, k < kBoxPix (-1 past D)
+  auto totals = [&](int base, float* t) {
+    uint32_t v[kBoxPix][kSupSplits];
 #pragma unroll
-    for (int k = 0; k < kSupSplits; ++k) t += part[(long long)k * D + i];
-    return (float)t;
+    for (int k = 0; k < kBoxPix; ++k) {
+      const int i = base + tid + k * MASK_THREADS;
+#pragma unroll
+      for (int j = 0; j < kSupSplits; ++j) v[k][j] = i < D ? part[(long long)j * D + i] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < kBoxPix; ++k) {
+      uint32_t s = 0u;
+#pragma unroll
+      for (int j = 0; j < kSupSplits; ++j) s += v[k][j];
+      t[k] = base + tid + k * MASK_THREADS < D ? (float)s : -1.0f;
+    }
   };
+  constexpr int kChunk = kBoxPix * MASK_THREADS;
   float lmax = 0.0f;
-  for (int i = tid; i < D; i += MASK_THREADS) lmax = fmaxf(lmax, total(i));
+  for (int base = 0; base < D; base += kChunk) {
+    float t[kBoxPix];
+    totals(base, t);
+#pragma unroll
+    for (int k = 0; k < kBoxPix; ++k) lmax = fmaxf(lmax, t[k]);
+  }
   const float thr = 0.25f * block_max(lmax, sh_f);
   float rmin = -INFINITY, rmax = -INFINITY, cmin = -INFINITY, cmax = -INFINITY;   // rmin/cmin negated
-  for (int i = tid; i < D; i += MASK_THREADS) {
-    if (total(i) > thr) {
-      const int r = i / det_cols, c = i - (i / det_cols) * det_cols;
-      rmin = fmaxf(rmin, (float)-r);
-      rmax = fmaxf(rmax, (float)r);
-      cmin = fmaxf(cmin, (float)-c);
-      cmax = fmaxf(cmax, (float)c);
+  for (int base = 0; base < D; base += kChunk) {
+    float t[kBoxPix];
+    totals(base, t);
+#pragma unroll
+    for (int k = 0; k < kBoxPix; ++k) {
+      if (t[k] > thr) {
+        const int i = base + tid + k * MASK_THREADS;
+        const int r = i / det_cols, c = i - r * det_cols;
+        rmin = fmaxf(rmin, (float)-r);
+        rmax = fmaxf(rmax, (float)r);
+        cmin = fmaxf(cmin, (float)-c);
+        cmax = fmaxf(cmax, (float)c);
+      }
     }
   }
   rmin = block_max(rmin, sh_f);
@@ -642,9 +671,11 @@ k_sup_box(const uint32_t* __restrict__ part, int det_rows, int det_cols, SupBox*
     SupBox b{0, 0, 0, 0};
     if (thr > 0.0f && rmax >= 0.0f) {
       const int r0 = max(0, (int)(-rmin) - kSupMargin), r1 = min(det_rows - 1, (int)rmax + kSupMargin);
-      const int c0 = max(0, (int)(-cmin) - kSupMargin), c1 = min(det_cols - 1, (int)cmax + kSupMargin);
+      // whole 8-column groups (det_cols % 8 == 0 on this path)
+      const int c0 = (max(0, (int)(-cmin) - kSupMargin) / 8) * 8;
+      const int c1 = min(det_cols, (((int)cmax + kSupMargin) / 8 + 1) * 8) - 1;
       const int bh = r1 - r0 + 1, bw = c1 - c0 + 1;
-      if (bh > 0 && bw > 0 && bh * bw <= kSupMaxPixels) b = SupBox{r0, c0, bh, bw};
+      if (bh > 0 && bw > 0 && bw % 8 == 0 && bh * bw <= kSupMaxPixels) b = SupBox{r0, c0, bh, bw};
     }
     *box = b;
   }

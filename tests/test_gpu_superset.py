@@ -13,7 +13,7 @@ from tests._gpu import engine_for_data, synthetic
 
 pytestmark = pytest.mark.gpu
 
-SUPERSET_KNOB = 3          # QSSB_TUNE_SUPERSET: 1 on (default), 2 off
+SUPERSET_KNOB = 3          # QSSB_TUNE_SUPERSET: 1 compact copy, 2 frames (default)
 
 
 @pytest.mark.parametrize("case", ["synthetic-128x64", "poisson-radius3", "wide-radius24", "synthetic-256x128"])
