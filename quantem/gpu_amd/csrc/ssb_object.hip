@@ -368,8 +368,8 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
 // any lane writes), so X[k1 + R k2] = Z_k1[k2] and lane l reads back its
 // frequencies q = l + 64 m.
 // Per point the trotter term is that of k_trotter_slots (same double-float
-// apertures, phasors and guard), summed over the group's BF pixels in
-// float64.  The aperture of a point far from the soft edge is decided from
+// apertures, phasors and guard), with Kahan FP32 sums over the group's BF
+// pixels.  The aperture of a point far from the soft edge is decided from
 // a float32 estimate of d (error < 2e-5 for |terms| < 32, inside the 1e-4
 // margin of aperture_sum); only points near the edge evaluate d in
 // double-float.  Outputs B of the mirror points are written as -conj(A);
@@ -466,20 +466,21 @@ __device__ __forceinline__ void wave_trotter_item(const float2* __restrict__ H, 
   constexpr int NBH = SLOT0 ? R : 1;     // column N/2 accumulators (slot 0 only)
   const int lane = tid & 63;
   const int j0 = cg * CG + 2 * w;
-  double aR[2][R], aI[2][R], hR[NBH], hI[NBH];
+  Kahan aR[2][R], aI[2][R], hR[NBH], hI[NBH];
 #pragma unroll
   for (int c = 0; c < 2; ++c)
 #pragma unroll
     for (int m = 0; m < R; ++m) {
-      aR[c][m] = 0.0;
-      aI[c][m] = 0.0;
+      aR[c][m].init();
+      aI[c][m].init();
     }
 #pragma unroll
   for (int i = 0; i < NBH; ++i) {
-    hR[i] = 0.0;
-    hI[i] = 0.0;
+    hR[i].init();
+    hI[i].init();
   }
-  double nyR[2] = {0.0, 0.0}, nyI[2] = {0.0, 0.0};   // Nyquist-row sums (uniform)
+  // Nyquist-row sums (wave-uniform), value + running error term
+  df nyR[2] = {df{0.f, 0.f}, df{0.f, 0.f}}, nyI[2] = {df{0.f, 0.f}, df{0.f, 0.f}};
   const float4* hrow = reinterpret_cast<const float4*>(H + (long long)lane * HALF + j0);
   constexpr long long ROWS64 = 64LL * HALF / 2;     // float4 between rows l and l + 64
   constexpr long long PER_B = (long long)N * HALF / 2;
@@ -597,8 +598,8 @@ __device__ __forceinline__ void wave_trotter_item(const float2* __restrict__ H, 
           }
           if (am != 0.0f || app != 0.0f) {
             const float2 t = trotter_point(GA, am, app, echiS[2 * w + c][q], cmul(cu.rowA[m], cu.colp[c]), ak);
-            aR[c][m] += (double)t.x;
-            aI[c][m] += (double)t.y;
+            aR[c][m].add(t.x);
+            aI[c][m].add(t.y);
           }
           if constexpr (SLOT0) {
             if (c == 0) {
@@ -617,8 +618,8 @@ __device__ __forceinline__ void wave_trotter_item(const float2* __restrict__ H, 
               }
               if (bm != 0.0f || bp != 0.0f) {
                 const float2 t = trotter_point(GB, bm, bp, echiS[CG][q], cmul(cu.rowA[m], cu.colHalf), ak);
-                hR[m] += (double)t.x;
-                hI[m] += (double)t.y;
+                hR[m].add(t.x);
+                hI[m].add(t.y);
               }
             }
           }
@@ -653,17 +654,22 @@ __device__ __forceinline__ void wave_trotter_item(const float2* __restrict__ H, 
           }
         }
       }
-      // fixed-order tree over the lanes, in float64
+      // compensated xor tree over the lanes: two_sum's error term is exact,
+      // so every lane holds the same (sum, error) pair
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
-        double re = (double)t[c].x, im = (double)t[c].y;
+        df re{t[c].x, 0.f}, im{t[c].y, 0.f};
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) {
-          re += __shfl_xor(re, off, 64);
-          im += __shfl_xor(im, off, 64);
+          const df ro{__shfl_xor(re.hi, off, 64), __shfl_xor(re.lo, off, 64)};
+          const df io{__shfl_xor(im.hi, off, 64), __shfl_xor(im.lo, off, 64)};
+          const df rs = two_sum(re.hi, ro.hi), is = two_sum(im.hi, io.hi);
+          re = df{rs.hi, (re.lo + ro.lo) + rs.lo};
+          im = df{is.hi, (im.lo + io.lo) + is.lo};
         }
-        nyR[c] += re;
-        nyI[c] += im;
+        const df rs = two_sum(nyR[c].hi, re.hi), is = two_sum(nyI[c].hi, im.hi);
+        nyR[c] = df{rs.hi, (nyR[c].lo + re.lo) + rs.lo};
+        nyI[c] = df{is.hi, (nyI[c].lo + im.lo) + is.lo};
       }
     }
     wave_lds_fence();
@@ -675,14 +681,14 @@ __device__ __forceinline__ void wave_trotter_item(const float2* __restrict__ H, 
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
       const int j = j0 + c;
-      const float2 outA = make_float2((float)aR[c][m], (float)aI[c][m]);
+      const float2 outA = make_float2(aR[c][m].s - aR[c][m].c, aI[c][m].s - aI[c][m].c);
       sp[(long long)j * N + q] = outA;
       if (j == 0) {
-        if constexpr (SLOT0) sp[(long long)HALF * N + q] = make_float2((float)hR[m], (float)hI[m]);
+        if constexpr (SLOT0) sp[(long long)HALF * N + q] = make_float2(hR[m].s - hR[m].c, hI[m].s - hI[m].c);
       } else if (q != HALF) {
         sp[(long long)(N - j) * N + qm] = make_float2(-outA.x, outA.y);
       } else {
-        sp[(long long)(N - j) * N + HALF] = make_float2((float)nyR[c], (float)nyI[c]);
+        sp[(long long)(N - j) * N + HALF] = make_float2(nyR[c].hi + nyR[c].lo, nyI[c].hi + nyI[c].lo);
       }
     }
   }
