@@ -144,10 +144,15 @@ def fit_leg(dev, n_sets: int, train_steps: int, search_trials: int, learned_tria
     Sobol) + the same refine.
 
     Per held-out dataset (C10 U(+-500 A), C30 U(+-2e5 A), rotation U(+-5 deg),
-    5e4 e-/probe): both fits run from zero aberrations; the converged optimum
-    of the exact objective (the better fit polished by Nelder-Mead, untimed)
-    defines the reference phase, and each fit's phase error is the RMS of the
-    wrapped phase difference to it.  Training time is reported on its own."""
+    5e4 e-/probe): both fits run from zero aberrations.  The reference optimum
+    of the exact objective is the minimum of a least-squares quadratic model
+    through 2 x 64 points around a fit's result (untimed), taken from the fit
+    with the lower loss; the same model from the other fit's result gives the
+    reference's own spread.  (The exact loss is rough at the ~3e-8 level near
+    its optimum: Nelder-Mead polishes from two nearby starts land up to
+    ~1.2e-4 rad apart in phase, the model minima within ~4e-5.)  Each fit's
+    phase error is the RMS of the wrapped phase difference to the reference.
+    Training time is reported on its own."""
     import numpy as np
     import torch
     from quantem.gpu_amd.ssb.learned import LearnedInit, FeatureConfig
@@ -186,13 +191,18 @@ def fit_leg(dev, n_sets: int, train_steps: int, search_trials: int, learned_tria
                 ssb.fit_learned(net, fcfg, trials=learned_trials, refine=refine, rounds=rounds)
             torch.cuda.synchronize()
             out[name] = (time.perf_counter() - t0, ssb._search_state(), float(ssb._best_loss), ssb._refine_nfev)
-        best = min(out.values(), key=lambda r: r[2])
-        ssb._set_search_state(best[1])
-        ssb.refine_keys(xatol=1e-5, fatol=1e-12, max_iter=600)
-        opt_state = ssb._search_state()
+        refs = {}
+        for name, (_, state, _, _) in out.items():
+            ssb._set_search_state(state)
+            ssb.refine_polish(rounds=2, batch=64, seed=0)
+            refs[name] = (ssb._search_state(), float(ssb._best_loss))
+        best_name = min(out, key=lambda n: out[n][2])
+        opt_state, opt_loss = refs[best_name]
         ph_opt = _phase_of(ssb, opt_state)
+        spread = _rms_wrapped(_phase_of(ssb, refs["search"][0]), _phase_of(ssb, refs["learned"][0]))
         row = {"truth": {"C10": truth["C10"], "C30": truth["C30"], "rotation_deg": rot},
-               "optimum": {k: opt_state[k] for k in keys}, "optimum_loss": float(ssb._best_loss)}
+               "optimum": {k: opt_state[k] for k in keys}, "optimum_loss": opt_loss,
+               "reference_spread_rad": spread}
         for name, (secs, state, loss, nfev) in out.items():
             row[name] = {"s": secs, "loss": loss, "phase_err_rad": _rms_wrapped(_phase_of(ssb, state), ph_opt),
                          "params": {k: state[k] for k in keys}, "refine_evals": nfev}
@@ -213,6 +223,8 @@ def fit_leg(dev, n_sets: int, train_steps: int, search_trials: int, learned_tria
         "max_phase_err_rad": {"search": max(r["search"]["phase_err_rad"] for r in rows),
                               "learned": max(r["learned"]["phase_err_rad"] for r in rows)},
         "phase_err_bar_rad": 1e-4,
+        "reference": "least-squares quadratic model minimum (2 x 64 points) around the lower-loss fit's result",
+        "reference_spread_max_rad": max(r["reference_spread_rad"] for r in rows),
         "sets_within_bar": {name: sum(r[name]["phase_err_rad"] <= 1e-4 for r in rows) for name in ("search", "learned")},
         "n_sets": len(rows),
         "training_s": train_s, "training_steps": train_steps, "training_loss": [hist[0][1], hist[-1][1]],

@@ -146,10 +146,10 @@ hipError_t launch_max_u16(const uint16_t* data, long long n, unsigned int* parti
 int max_u16_parts();
 
 // Scan normalisation: crop/pad the H x W scan to T x T (partial holds
-// normalize_partial_groups(H, T) * D uint32, mean D uint16).
+// normalize_partial_groups(QSSB_DTYPE_U16, H, T, D) * D uint32, mean D uint16).
 hipError_t launch_normalize_scan(const uint16_t* src, int H, int W, int D, uint16_t* dst, int T,
                                  uint32_t* partial, uint16_t* mean, hipStream_t s);
-int normalize_partial_groups(int H, int T);
+int normalize_partial_groups(int dtype, int H, int T, int D);
 // The same for any QSSB_DTYPE_* sample type; partial and mean as laid out by
 // normalize_workspace_bytes (mean at *mean_offset of the workspace).
 hipError_t launch_normalize_scan_typed(int dtype, const void* src, int H, int W, int D, void* dst, int T,

@@ -1176,16 +1176,29 @@ hipError_t launch_gather_rowfft(int N, int dtype, const void* data, int D, int d
 // ------------------------------------------------------------------------
 constexpr int kWindowRows = 64;   // window rows per partial sum: 64*1024*65535 < 2^32
 
+// Window rows per partial sum: at most kWindowRows; integer samples split
+// further so that a small detector still spreads over ~256 workgroups
+// (integer sums are exact in any split); float32 keeps kWindowRows, since
+// its double-float sums depend on the split.
+static int window_rows(int dtype, int h, int D) {
+  if (dtype == QSSB_DTYPE_F32) return kWindowRows;
+  const int tiles = (D + 255) / 256;
+  const int want = (256 + tiles - 1) / tiles;
+  const int rows = (h + want - 1) / want;
+  return rows < 1 ? 1 : (rows > kWindowRows ? kWindowRows : rows);
+}
+
 __global__ void __launch_bounds__(256)
-k_window_partial(const uint16_t* __restrict__ src, int W, int D, int r0, int c0, int h, int w,
+k_window_partial(const uint16_t* __restrict__ src, int W, int D, int r0, int c0, int h, int w, int rows,
                  uint32_t* __restrict__ partial) {
   const int p = blockIdx.x * 256 + threadIdx.x;
   if (p >= D) return;
-  const int ra = blockIdx.y * kWindowRows;
-  const int rb = min(h, ra + kWindowRows);
+  const int ra = blockIdx.y * rows;
+  const int rb = min(h, ra + rows);
   uint32_t acc = 0u;
   for (int r = ra; r < rb; ++r) {
     const uint16_t* row = src + ((long long)(r0 + r) * W + c0) * D + p;
+#pragma unroll 8
     for (int c = 0; c < w; ++c) acc += row[(long long)c * D];
   }
   partial[(long long)blockIdx.y * D + p] = acc;
@@ -1229,9 +1242,10 @@ hipError_t launch_normalize_scan(const uint16_t* src, int H, int W, int D, uint1
   const int w = W < T ? W : T;
   const int pad_top = (T - h) / 2, pad_left = (T - w) / 2;
   if (h < T || w < T) {
-    const int groups = (h + kWindowRows - 1) / kWindowRows;
+    const int rows = window_rows(QSSB_DTYPE_U16, h, D);
+    const int groups = (h + rows - 1) / rows;
     hipLaunchKernelGGL(k_window_partial, dim3((D + 255) / 256, groups), dim3(256), 0, s, src, W, D, r0, c0, h, w,
-                       partial);
+                       rows, partial);
     hipLaunchKernelGGL(k_window_mean, dim3((D + 255) / 256), dim3(256), 0, s, partial, groups, D,
                        (unsigned long long)h * (unsigned long long)w, mean);
   }
@@ -1251,9 +1265,10 @@ hipError_t launch_normalize_scan(const uint16_t* src, int H, int W, int D, uint1
   return hipGetLastError();
 }
 
-int normalize_partial_groups(int H, int T) {
+int normalize_partial_groups(int dtype, int H, int T, int D) {
   const int h = H < T ? H : T;
-  return (h + kWindowRows - 1) / kWindowRows;
+  const int rows = window_rows(dtype, h, D);
+  return (h + rows - 1) / rows;
 }
 
 // Scan normalisation of the other sample types: per 64-row group int64
@@ -1265,12 +1280,12 @@ template <> struct WinSum<float> { typedef float2 S; };
 
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_window_partial_t(const T* __restrict__ src, int W, int D, int r0, int c0, int h, int w,
+k_window_partial_t(const T* __restrict__ src, int W, int D, int r0, int c0, int h, int w, int rows,
                    typename WinSum<T>::S* __restrict__ partial) {
   const int p = blockIdx.x * 256 + threadIdx.x;
   if (p >= D) return;
-  const int ra = blockIdx.y * kWindowRows;
-  const int rb = min(h, ra + kWindowRows);
+  const int ra = blockIdx.y * rows;
+  const int rb = min(h, ra + rows);
   if constexpr (std::is_same<T, float>::value) {
     float hi = 0.0f, lo = 0.0f;
     for (int r = ra; r < rb; ++r) {
@@ -1286,6 +1301,7 @@ k_window_partial_t(const T* __restrict__ src, int W, int D, int r0, int c0, int 
     long long acc = 0;
     for (int r = ra; r < rb; ++r) {
       const T* row = src + ((long long)(r0 + r) * W + c0) * D + p;
+#pragma unroll 8
       for (int c = 0; c < w; ++c) acc += (long long)row[(long long)c * D];
     }
     partial[(long long)blockIdx.y * D + p] = acc;
@@ -1351,10 +1367,11 @@ static hipError_t normalize_typed(const void* src_v, int H, int W, int D, void* 
   const T* src = reinterpret_cast<const T*>(src_v);
   T* mean = reinterpret_cast<T*>(mean_v);
   if (h < T_ || w < T_) {
-    const int groups = (h + kWindowRows - 1) / kWindowRows;
+    const int rows = window_rows(std::is_same<T, float>::value ? QSSB_DTYPE_F32 : QSSB_DTYPE_I32, h, D);
+    const int groups = (h + rows - 1) / rows;
     S* partial = reinterpret_cast<S*>(partial_v);
     hipLaunchKernelGGL(k_window_partial_t<T>, dim3((D + 255) / 256, groups), dim3(256), 0, s, src, W, D, r0, c0, h,
-                       w, partial);
+                       w, rows, partial);
     hipLaunchKernelGGL(k_window_mean_t<T>, dim3((D + 255) / 256), dim3(256), 0, s, partial, groups, D,
                        (long long)h * w, mean);
   }
@@ -1396,7 +1413,7 @@ hipError_t launch_normalize_scan_typed(int dtype, const void* src, int H, int W,
 
 // bytes of the partial sums and of the pad frame of launch_normalize_scan_typed
 size_t normalize_workspace_bytes(int dtype, int H, int T, int D, size_t* mean_offset) {
-  const int groups = normalize_partial_groups(H, T);
+  const int groups = normalize_partial_groups(dtype, H, T, D);
   const size_t per = dtype == QSSB_DTYPE_U16 ? 4 : 8;
   size_t part = (size_t)groups * D * per;
   part = (part + 255) & ~(size_t)255;

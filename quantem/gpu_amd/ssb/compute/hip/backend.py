@@ -707,6 +707,48 @@ class HipSSBBackend:
             self._print_summary(f"Refine (newton, {n_evals} evals)", self._elapsed_refine)
         return self
 
+    _POLISH_BOX = {"C10_nm": 1.5, "C30": 6.0e3, "rotation_angle_deg": 0.1}
+
+    def refine_polish(self, keys=("C10_nm", "C30", "rotation_angle_deg"), box=None, rounds: int = 2,
+                      batch: int = 16, bounds=None, seed: int = 20, verbose: bool = False):
+        """Least-squares quadratic polish on the exact objective
+        (``optimizer.model_polish``): ``rounds`` batches of ``batch``
+        candidates in a box of half-widths ``box`` (default ~3x the parameter
+        change that moves the phase by 1e-4 rad RMS) around the current
+        state, each followed by a move to the fitted model's minimiser; the
+        state becomes the last minimiser (evaluated once more for
+        ``_best_loss``).  Near the optimum the exact loss is rough at the
+        ~3e-8 level, which makes its lowest sample a noisy estimate; the
+        model's minimiser averages that roughness."""
+        t0 = time.perf_counter()
+        base = self._search_state()
+        keys = tuple(keys)
+        box = box or self._POLISH_BOX
+        bounds = bounds or {}
+        lo = np.array([bounds.get(k, (-math.inf, math.inf))[0] for k in keys])
+        hi = np.array([bounds.get(k, (-math.inf, math.inf))[1] for k in keys])
+
+        def evaluate(points):
+            params = []
+            for x in np.asarray(points, dtype=np.float64).reshape(-1, len(keys)):
+                p = dict(base)
+                p.update({k: float(v) for k, v in zip(keys, x)})
+                params.append(p)
+            return self._evaluate_params(params)
+
+        x0 = np.array([base[k] for k in keys])
+        x, n_evals = _opt.model_polish(evaluate, x0, np.array([box[k] for k in keys], dtype=np.float64),
+                                       rounds=rounds, batch=batch, seed=seed, lo=lo, hi=hi)
+        self._best_loss = float(np.asarray(evaluate(x[None, :])).reshape(-1)[0])
+        self._set_search_state({k: float(v) for k, v in zip(keys, x)})
+        # the polish completes a fit's refinement: its evaluations and time add to it
+        self._refine_nfev = int(getattr(self, "_refine_nfev", 0) or 0) + int(n_evals) + 1
+        self._refine_method = f"{self._refine_method}+polish" if self._refine_method else "polish"
+        self._elapsed_refine = time.perf_counter() - t0 + float(getattr(self, "_elapsed_refine", 0.0) or 0.0)
+        if verbose:
+            self._print_summary(f"Polish (least-squares model, {n_evals + 1} evals)", time.perf_counter() - t0)
+        return self
+
     def learned_start(self, net, feature_config) -> dict:
         """(C10, C30, rotation) predicted by the learned initializer from the
         resident Fourier stack, in the search's parameter names."""
@@ -743,16 +785,19 @@ class HipSSBBackend:
 
     def fit_learned(self, net, feature_config, *, trials: int = 16, refine: str | None = "newton",
                     xatol: float = 1e-4, rounds: int | None = None, half_widths=None, bounds=None, seed: int = 42,
-                    newton_step=None, newton_ftol: float | None = None, verbose: bool = False):
+                    newton_step=None, newton_ftol: float | None = None, polish: bool = True,
+                    verbose: bool = False):
         """Learned start, a ``trials``-candidate batched Sobol search in a box
         around it (clipped to ``bounds``, the global search box), then a local
         refinement of (C10, C30, rotation) on the exact objective:
         ``"newton"`` (default: batched finite-difference Newton,
         ``refine_newton``, at most ``rounds`` batches, default 8),
         ``"nelder-mead"`` (the reference's coefficients, ``refine_keys``) or
-        ``"quadratic"`` (``refine_quadratic``, ``rounds`` default 4).  The
-        global batched search (``fit_search``) is the fallback when no trained
-        network is available."""
+        ``"quadratic"`` (``refine_quadratic``, ``rounds`` default 4); after
+        the Newton refine, with ``polish``, the least-squares model polish
+        (``refine_polish``), whose minimiser is the result.
+        The global batched search (``fit_search``) is the fallback when no
+        trained network is available."""
         t0 = time.perf_counter()
         start = self.learned_start(net, feature_config)
         half = dict(self._LEARNED_HALF_WIDTHS)
@@ -777,6 +822,8 @@ class HipSSBBackend:
         self._local_refine(refine, keys, [half[k] / 2 for k in keys], rounds, 16, bounds, seed, xatol, verbose,
                            newton_step or self._LEARNED_NEWTON_STEP,
                            self._LEARNED_NEWTON_FTOL if newton_ftol is None else newton_ftol)
+        if polish and refine == "newton":
+            self.refine_polish(keys, bounds=bounds, verbose=verbose)
         self._learned_start = start
         self._n_trials = trials
         self._elapsed_optimize = time.perf_counter() - t0
@@ -784,11 +831,12 @@ class HipSSBBackend:
 
     def fit_search(self, *, trials: int = 512, refine: str | None = "newton", xatol: float = 1e-4,
                    rounds: int | None = None, bounds=None, seed: int = 42, newton_step=None,
-                   newton_ftol: float | None = None, verbose: bool = False):
+                   newton_ftol: float | None = None, polish: bool = True, verbose: bool = False):
         """Plain batched fit without the network: a ``trials``-candidate global
         Sobol search over ``bounds`` (C10, C30, rotation), then the same local
-        refinement as ``fit_learned`` from its best point (a wider first Newton
-        stencil: the global search's best point is farther out)."""
+        refinement and polish as ``fit_learned`` from its best point (a wider
+        first Newton stencil: the global search's best point is farther
+        out)."""
         t0 = time.perf_counter()
         bounds = bounds or dict(self._FIT_BOUNDS)
         self.optimize(aberrations=dict(bounds, C12_nm=0.0, phi12_deg=0.0), n_trials=trials, seed=seed,
@@ -796,6 +844,8 @@ class HipSSBBackend:
         keys = tuple(bounds)
         self._local_refine(refine, keys, [0.05 * (hi - lo) for lo, hi in bounds.values()], rounds, 16, bounds,
                            seed, xatol, verbose, newton_step, self._NEWTON_FTOL if newton_ftol is None else newton_ftol)
+        if polish and refine == "newton":
+            self.refine_polish(keys, bounds=bounds, verbose=verbose)
         self._elapsed_optimize = time.perf_counter() - t0
         return self
 

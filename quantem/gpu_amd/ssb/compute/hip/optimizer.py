@@ -270,6 +270,47 @@ def quadratic_refine(evaluate: Evaluate, x0, radius, rounds: int = 3, batch: int
     return np.asarray(xs[best]), float(fs[best]), len(fs)
 
 
+def model_polish(evaluate: Evaluate, x0, box, rounds: int = 2, batch: int = 16, seed: int = 20, clip: float = 1.5,
+                 lo=None, hi=None) -> tuple[np.ndarray, int]:
+    """Least-squares quadratic polish near an optimum; returns (x, n_evals).
+
+    Round r evaluates ``batch`` scrambled-Sobol points of the box
+    ``x +/- box`` in one ``evaluate`` call, fits a full quadratic to them in
+    box units and moves ``x`` to the model's minimiser (the Newton step with
+    the Hessian's eigenvalues taken by magnitude, at most ``clip`` boxes per
+    axis).  The result is the last model minimiser, not the lowest loss seen:
+    where the objective is rough at the scale of the answer (the exact loss
+    moves in small steps as overlap points cross the aperture edge), a fit
+    through many points averages the roughness that picking the lowest
+    sample would select.  The minimiser itself is not evaluated.  ``lo``/``hi``
+    clip every point.  It has no reference counterpart.
+    """
+    from scipy.stats import qmc
+
+    x = np.asarray(x0, dtype=np.float64).reshape(-1).copy()
+    d = x.size
+    box = np.broadcast_to(np.asarray(box, dtype=np.float64), x.shape).copy()
+    need = 1 + d + d * (d + 1) // 2
+    if batch < need:
+        raise ValueError(f"batch must hold at least {need} points for {d} parameters")
+    if np.any(box <= 0):
+        raise ValueError("box must be positive")
+    lo = np.full(d, -np.inf) if lo is None else np.asarray(lo, dtype=np.float64)
+    hi = np.full(d, np.inf) if hi is None else np.asarray(hi, dtype=np.float64)
+    x = np.clip(x, lo, hi)
+    n = 0
+    for r in range(rounds):
+        u = 2.0 * qmc.Sobol(d=d, scramble=True, seed=seed + r).random_base2(int(math.ceil(math.log2(batch))))[:batch] - 1.0
+        pts = np.clip(x + u * box, lo, hi)
+        f = np.asarray(evaluate(pts), dtype=np.float64).reshape(-1)
+        n += len(f)
+        step = _quadratic_minimiser((pts - x) / box, f)
+        if step is None or not np.all(np.isfinite(step)):
+            continue
+        x = np.clip(x + np.clip(step, -clip, clip) * box, lo, hi)
+    return x, n
+
+
 def _stencil(d: int) -> np.ndarray:
     """Unit offsets of the finite-difference stencil: the centre, +/- each
     axis, and +i+j for every pair (1 + 2d + d(d-1)/2 points)."""

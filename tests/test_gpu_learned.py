@@ -184,3 +184,62 @@ def test_newton_refine_reaches_the_converged_optimum(cuda):
     assert loss <= opt_loss + 1e-7
     ssb.close()
     del data
+
+
+@pytest.mark.parametrize("c10, c30, rot, seed", [(-202.21, 48505.9, 0.609, 9002), (396.96, -188428.7, 4.615, 9004)])
+def test_learned_fit_reaches_the_phase_bar_with_a_fraction_of_the_search(cuda, c10, c30, rot, seed):
+    """End-to-end gain at the 1e-4 rad bar on the benchmark's workload (scan
+    256, detector 128, 5e4 e-/probe): with a network trained on the GPU
+    (4000 steps), both fits -- learned start + 16-candidate box + batched
+    Newton + least-squares polish, and the global 512-candidate search + the
+    same refine and polish -- end within 1e-4 rad RMS of the reference
+    optimum (the least-squares model minimum through 2 x 64 points around the
+    lower-loss result), and the learned fit spends less than a quarter of the
+    search's exact-objective evaluations."""
+    from quantem.gpu_amd.ssb.learned import LearnedInit, FeatureConfig
+    from quantem.gpu_amd.ssb.synthetic import SyntheticConfig, generate
+    from quantem.gpu_amd.ssb.compute.hip.backend import HipSSBBackend
+    sc = SyntheticConfig(scan=256, det=128, aberrations={"C10": c10, "C30": c30}, rotation_deg=rot, dose=5e4,
+                         seed=seed)
+    fcfg = FeatureConfig.for_scan(256, sc.step_A, sc.semiangle_mrad * 1e-3, sc.det_sampling_mrad * 1e-3,
+                                  sc.wavelength)
+    net = LearnedInit(seed=1)
+    net.train(fcfg, steps=4000, batch=256, lr=2e-3, seed=2, log_every=4000)
+    data = generate(sc, device="cuda")
+    ssb = HipSSBBackend(data, semiangle=sc.semiangle_mrad, scan_sampling=sc.step_A, det_sampling=sc.det_sampling_mrad,
+                        voltage_kV=300)
+    counted = []
+    evaluate = ssb._evaluate_params
+
+    def counting(params):
+        counted.append(len(params))
+        return evaluate(params)
+    ssb._evaluate_params = counting
+    zero = {"C10_nm": 0.0, "C12_nm": 0.0, "phi12_deg": 0.0, "C30": 0.0, "rotation_angle_deg": 0.0}
+    fits = {}
+    for name in ("search", "learned"):
+        counted.clear()
+        ssb._set_search_state(zero)
+        if name == "search":
+            ssb.fit_search(trials=512)
+        else:
+            ssb.fit_learned(net, fcfg)
+        fits[name] = (ssb._search_state(), ssb._best_loss, sum(counted))
+    ssb._evaluate_params = evaluate
+    best = min(fits, key=lambda n: fits[n][1])
+    ssb._set_search_state(fits[best][0])
+    ssb.refine_polish(rounds=2, batch=64, seed=0)
+    ref_state = ssb._search_state()
+
+    def phase(state):
+        ssb._set_search_state(state)
+        m = np.zeros(14)
+        m[0], m[4] = state["C10_nm"], state["C30"]
+        return ssb.reconstruct_full(m, np.zeros(14)).cpu().numpy().astype(np.float64)
+
+    ref = phase(ref_state)
+    errs = {n: float(np.sqrt(np.mean(np.angle(np.exp(1j * (phase(f[0]) - ref))) ** 2))) for n, f in fits.items()}
+    assert errs["search"] < 1e-4 and errs["learned"] < 1e-4, (errs, {n: f[2] for n, f in fits.items()})
+    assert fits["learned"][2] * 4 < fits["search"][2], {n: f[2] for n, f in fits.items()}
+    ssb.close()
+    del data

@@ -207,3 +207,51 @@ def test_newton_refine_reaches_a_minimum_outside_the_box_at_the_bound():
     x, fx, _ = opt.newton_refine(_quadratic(np.eye(1), [6.0]), [4.0], [0.5], max_iter=10, lo=np.array([-5.0]),
                                  hi=np.array([5.0]))
     assert x[0] == pytest.approx(5.0, abs=1e-9)
+
+
+def _rough(A, c, step):
+    """Quadratic bowl 0.5 (x - c)^T A (x - c) plus a staircase of height
+    ``step`` that flips sign on a fine pseudo-random grid: a loss whose lowest
+    sample near the minimum is decided by the staircase."""
+    A = np.asarray(A, float)
+    c = np.asarray(c, float)
+
+    def f(points):
+        x = np.atleast_2d(np.asarray(points, float))
+        d = x - c
+        smooth = 0.5 * np.einsum("ni,ij,nj->n", d, A, d)
+        cells = np.floor(x / 0.013).astype(np.int64)
+        h = (cells[:, 0] * 73856093) ^ (cells[:, -1] * 19349663)
+        return smooth + step * np.where(h % 2 == 0, 1.0, -1.0)
+    return f
+
+
+def test_model_polish_averages_a_rough_loss_that_the_lowest_sample_does_not():
+    A = [[2.0, 0.4, 0.0], [0.4, 1.0, 0.2], [0.0, 0.2, 3.0]]
+    c = np.array([0.31, -0.17, 0.08])
+    f = _rough(A, c, 2e-3)
+    x0 = c + np.array([0.6, -0.5, 0.4])
+    seen = []
+
+    def evaluate(points):
+        v = f(points)
+        seen.extend(zip(np.atleast_2d(points), v))
+        return v
+    x, n = opt.model_polish(evaluate, x0, [0.5, 0.5, 0.5], rounds=2, batch=32, seed=3)
+    assert n == 64
+    err_model = float(np.linalg.norm(x - c))
+    lowest = min(seen, key=lambda t: t[1])[0]
+    assert err_model < 0.03
+    assert err_model < float(np.linalg.norm(lowest - c))
+
+
+def test_model_polish_recovers_an_exact_quadratic_and_respects_bounds():
+    A = np.diag([1.0, 4.0])
+    x, _ = opt.model_polish(_quadratic(A, [0.2, -0.1]), [1.0, 1.0], [1.0, 1.0], rounds=2, batch=8)
+    np.testing.assert_allclose(x, [0.2, -0.1], atol=1e-9)
+    lo, hi = np.array([-1.0, 0.0]), np.array([1.0, 1.0])
+    x, _ = opt.model_polish(_quadratic(A, [0.2, -0.5]), [0.5, 0.5], [0.5, 0.5], rounds=3, batch=8, lo=lo, hi=hi)
+    assert np.all(x >= lo) and np.all(x <= hi)
+    assert x[1] == pytest.approx(0.0, abs=1e-9)
+    with pytest.raises(ValueError, match="batch"):
+        opt.model_polish(_quadratic(A, [0.0, 0.0]), [0.0, 0.0], [1.0, 1.0], batch=5)
