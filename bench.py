@@ -470,7 +470,40 @@ def trials_leg(dev, data, cfg, n_trials: int = 512, reps: int = 5, seed: int = 3
                   "trials_per_s": n_trials / (ms_exact_graph * 1e-3)},
         "uneven_mix": {"scans": [128, 256, 256], "source_scans": [128, 192, 256], "ms_per_batch_graph": ms_mix,
                        "units_per_s": mix_units / (ms_mix * 1e-3)},
+        "roofline": search_unit_roofline(units / (ms_graph * 1e-3)),
     }
+
+
+SEARCH_PROFILE = REPO / "profiles" / "trials_search_512x256x128.md"
+VALU_LANES_PER_S = 1024 * 16 * 2.4e9   # 256 CUs x 4 SIMDs, one wave64 VALU instruction per 4 cycles, 2.4 GHz
+
+
+def read_profile_figure(path: Path, key: str) -> float | None:
+    """A ``key: value`` line of a committed rocprofv3 summary, or None."""
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.strip().startswith(key + ":"):
+            try:
+                return float(line.split(":", 1)[1].strip())
+            except ValueError:
+                return None
+    return None
+
+
+def search_unit_roofline(units_per_s: float) -> dict:
+    """The batched search against its VALU issue bound: VALU (and
+    transcendental) lane-instructions per search unit from the committed
+    counter summary of the same launch, the bound = FP32 VALU lane rate
+    (packed FP32 unused) / instructions per unit, frac = achieved / bound."""
+    per = read_profile_figure(SEARCH_PROFILE, "valu_lane_instructions_per_unit")
+    trans = read_profile_figure(SEARCH_PROFILE, "trans_f32_lane_instructions_per_unit")
+    if not per:
+        return {"bound": "valu", "achieved": units_per_s, "unit": "units/s", "peak": None, "frac": None}
+    peak = VALU_LANES_PER_S / per
+    return {"bound": "valu", "achieved": units_per_s, "peak": peak, "unit": "units/s", "frac": units_per_s / peak,
+            "valu_lane_instructions_per_unit": per, "trans_f32_lane_instructions_per_unit": trans,
+            "counters": str(SEARCH_PROFILE.relative_to(REPO))}
 
 
 def copy_probe() -> int:
