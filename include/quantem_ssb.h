@@ -444,7 +444,11 @@ int ssb_bin_u16(const uint16_t* src, long long frames, int rows, int cols, int f
  * trotter; QSSB_TUNE_SUPERSET: 2 (default) the gather reads the frames, 1
  * the stream pass of uint16 frames also copies a sampled bright-field box of
  * every frame compactly for the gather (bit-identical; its extra writes cost
- * the stream pass more than the gather saves at 256x256x128x128).  ssb_tune_candidates writes the
+ * the stream pass more than the gather saves at 256x256x128x128);
+ * QSSB_TUNE_SPLIT: 2 (default) one pass, 1 the gather and the wave-local
+ * trotter of a 256x256 scan in two halves of the BF list, the second half's
+ * gather on a second stream beside the first half's trotter (the two group
+ * the BF sums differently, so they agree to float rounding).  ssb_tune_candidates writes the
  * allowed values for a scan size
  * (the first is the default) and returns their count; ssb_tune_set accepts
  * one of them or 0 (default); ssb_tune_get reports the value in use.  The
@@ -456,6 +460,7 @@ int ssb_bin_u16(const uint16_t* src, long long frames, int rows, int cols, int f
 #define QSSB_TUNE_GATHER_PAIRS 1
 #define QSSB_TUNE_TROTTER 2
 #define QSSB_TUNE_SUPERSET 3
+#define QSSB_TUNE_SPLIT 4
 int ssb_tune_candidates(int scan_n, int knob, int* values_host, int capacity);
 int ssb_tune_set(ssb_plan* plan, int knob, int value);
 int ssb_tune_get(ssb_plan* plan, int knob, int* value_host);

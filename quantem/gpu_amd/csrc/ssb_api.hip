@@ -299,6 +299,9 @@ int ssb_plan_destroy(ssb_plan* p) {
   if (p->gev_in) (void)hipEventDestroy(p->gev_in);
   if (p->gev_out) (void)hipEventDestroy(p->gev_out);
   if (p->gstream) (void)hipStreamDestroy(p->gstream);
+  if (p->pev_a) (void)hipEventDestroy(p->pev_a);
+  if (p->pev_b) (void)hipEventDestroy(p->pev_b);
+  if (p->pstream) (void)hipStreamDestroy(p->pstream);
   release_events(p);
   delete p;
   return QSSB_OK;
@@ -1041,17 +1044,55 @@ static int recon_typed(ssb_plan* p, const ssb_plan* after, const void* data, int
   rc = alloc_h(p, p->max_bf);
   if (rc) return rc;
   mark(5);
-  HIP_TRY(launch_gather_rowfft(p->n, dtype, data, p->D, p->det_cols, p->bf_rows, p->bf_cols, p->info, p->nbf_hint,
-                               p->tw, p->tune_gather_pairs, p->gain, p->H, s, sup ? p->sup_box : nullptr,
-                               sup ? p->sup_C : nullptr),
-          "gather_rowfft");
-  mark(6);
   const QuadChi m = quad_chi(p, c10, c12, phi12);
-  HIP_TRY(launch_phasors(p->n, p->info, p->nbf_hint, geom_ptrs(p), m, p->rowph, p->colph, p->echi, s), "phasors");
   int groups = object_groups(p->n);
-  HIP_TRY(launch_trotter_slots(p->n, true, p->H, p->info, geom_ptrs(p), p->rowph, p->colph, p->echi, p->optics, p->tw, p->S_part,
-                               nullptr, true, p->tune_trotter, s, &groups),
-          "colfft_trotter");
+  // With QSSB_TUNE_SPLIT 1: two halves of the BF list (raster order, so
+  // disjoint detector rows), the second half's gather (HBM-bound) on the
+  // plan's second stream beside the first half's trotter (VALU-bound).
+  // Measured at 256x256x128x128 the two compete for the CUs (the trotter
+  // holds one 8-wave workgroup per CU at 225 VGPRs) and the pair is slower
+  // than one pass, so one pass is the default.  Stage marks with the split:
+  // 5 -> 6 the first half's gather and the phasors, 6 -> 7 both trotters
+  // and the second gather.
+  const bool two = p->tune_split == 1 &&
+                   trotter_wave_path(p->n, true, p->optics, true, false, p->tune_trotter);
+  if (two) {
+    if (!p->pstream) HIP_TRY(hipStreamCreateWithFlags(&p->pstream, hipStreamNonBlocking), "second stream");
+    if (!p->pev_a) HIP_TRY(hipEventCreateWithFlags(&p->pev_a, hipEventDisableTiming), "split event");
+    if (!p->pev_b) HIP_TRY(hipEventCreateWithFlags(&p->pev_b, hipEventDisableTiming), "split event");
+    HIP_TRY(launch_gather_rowfft(p->n, dtype, data, p->D, p->det_cols, p->bf_rows, p->bf_cols, p->info,
+                                 p->nbf_hint, p->tw, p->tune_gather_pairs, p->gain, p->H, s,
+                                 sup ? p->sup_box : nullptr, sup ? p->sup_C : nullptr, 0, 2),
+            "gather_rowfft");
+    HIP_TRY(hipEventRecord(p->pev_a, s), "split event");
+    HIP_TRY(hipStreamWaitEvent(p->pstream, p->pev_a, 0), "split wait");
+    HIP_TRY(launch_gather_rowfft(p->n, dtype, data, p->D, p->det_cols, p->bf_rows, p->bf_cols, p->info,
+                                 p->nbf_hint, p->tw, p->tune_gather_pairs, p->gain, p->H, p->pstream,
+                                 sup ? p->sup_box : nullptr, sup ? p->sup_C : nullptr, 1, 2),
+            "gather_rowfft");
+    HIP_TRY(hipEventRecord(p->pev_b, p->pstream), "split event");
+    HIP_TRY(launch_phasors(p->n, p->info, p->nbf_hint, geom_ptrs(p), m, p->rowph, p->colph, p->echi, s),
+            "phasors");
+    mark(6);
+    HIP_TRY(launch_trotter_slots(p->n, true, p->H, p->info, geom_ptrs(p), p->rowph, p->colph, p->echi, p->optics,
+                                 p->tw, p->S_part, nullptr, true, p->tune_trotter, s, &groups, 0, 2),
+            "colfft_trotter");
+    HIP_TRY(hipStreamWaitEvent(s, p->pev_b, 0), "split wait");
+    HIP_TRY(launch_trotter_slots(p->n, true, p->H, p->info, geom_ptrs(p), p->rowph, p->colph, p->echi, p->optics,
+                                 p->tw, p->S_part, nullptr, true, p->tune_trotter, s, &groups, 1, 2),
+            "colfft_trotter");
+  } else {
+    HIP_TRY(launch_gather_rowfft(p->n, dtype, data, p->D, p->det_cols, p->bf_rows, p->bf_cols, p->info,
+                                 p->nbf_hint, p->tw, p->tune_gather_pairs, p->gain, p->H, s,
+                                 sup ? p->sup_box : nullptr, sup ? p->sup_C : nullptr),
+            "gather_rowfft");
+    mark(6);
+    HIP_TRY(launch_phasors(p->n, p->info, p->nbf_hint, geom_ptrs(p), m, p->rowph, p->colph, p->echi, s),
+            "phasors");
+    HIP_TRY(launch_trotter_slots(p->n, true, p->H, p->info, geom_ptrs(p), p->rowph, p->colph, p->echi, p->optics,
+                                 p->tw, p->S_part, nullptr, true, p->tune_trotter, s, &groups),
+            "colfft_trotter");
+  }
   mark(7);
   HIP_TRY(launch_finish_object(p->n, p->S_part, groups, p->info, p->tw, p->S,
                                reinterpret_cast<float2*>(obj_out), phase_out, s),
@@ -1166,6 +1207,11 @@ int ssb_tune_candidates(int scan_n, int knob, int* values_host, int capacity) {
       vals[1] = 1;
       n = 2;
       break;
+    case QSSB_TUNE_SPLIT:
+      vals[0] = 2;
+      vals[1] = 1;
+      n = 2;
+      break;
     default:
       return fail(QSSB_ERR_ARG, "unknown tuning knob " + std::to_string(knob));
   }
@@ -1188,6 +1234,7 @@ int ssb_tune_set(ssb_plan* p, int knob, int value) {
     case QSSB_TUNE_STREAM_WG: p->tune_stream_wg = value; break;
     case QSSB_TUNE_GATHER_PAIRS: p->tune_gather_pairs = value; break;
     case QSSB_TUNE_SUPERSET: p->tune_superset = value; break;
+    case QSSB_TUNE_SPLIT: p->tune_split = value; break;
     default: p->tune_trotter = value; break;
   }
   return QSSB_OK;
@@ -1204,6 +1251,7 @@ int ssb_tune_get(ssb_plan* p, int knob, int* value_host) {
       break;
     case QSSB_TUNE_TROTTER: *value_host = p->tune_trotter; break;
     case QSSB_TUNE_SUPERSET: *value_host = p->tune_superset ? p->tune_superset : 2; break;
+    case QSSB_TUNE_SPLIT: *value_host = p->tune_split ? p->tune_split : 2; break;
     default: return fail(QSSB_ERR_ARG, "unknown tuning knob " + std::to_string(knob));
   }
   return QSSB_OK;

@@ -134,7 +134,7 @@ int gather_pairs_candidates(int N, int* out);   // out[2]; returns the count
 hipError_t launch_gather_rowfft(int N, int dtype, const void* data, int D, int det_cols, const int* rows,
                                 const int* cols, const BfInfo* info, int nbf_hint, const float2* tw, int pairs,
                                 const float* gain, float2* H, hipStream_t s, const SupBox* box = nullptr,
-                                const uint16_t* C = nullptr);
+                                const uint16_t* C = nullptr, int part = 0, int parts = 1);
 // Exact BF columns out[b][s] for BF pixels [bf_offset, bf_offset + count)
 // of the list rows/cols: uint8 frames -> uint8 (in_u8), uint16 frames ->
 // uint16, or uint8 when narrow.
@@ -190,7 +190,11 @@ int trotter_variants(int N);
 hipError_t launch_trotter_slots(int N, bool from_h, const float2* src, const BfInfo* info, const GeomPtrs& gp,
                                 const float2* rowph, const float2* colph, const float2* echi, const Optics& o,
                                 const float2* tw, float2* S_part, float2* Gt, bool do_trotter, int variant,
-                                hipStream_t s, int* groups_used = nullptr);
+                                hipStream_t s, int* groups_used = nullptr, int part = 0, int parts = 1);
+// Whether launch_trotter_slots takes the wave-local kernel (which can run
+// in BF parts: part p of `parts` sums its share of the BF pixels into its
+// share of the partial planes).
+bool trotter_wave_path(int N, bool from_h, const Optics& o, bool do_trotter, bool has_gt, int variant);
 // Trotter from an imported full-plane (non-Hermitian) stack Gt[b][c][q], c < N.
 hipError_t launch_trotter_full(int N, const float2* Gt, const BfInfo* info, const GeomPtrs& gp, const float2* rowph,
                                QuadChi m, const Optics& o, float2* S_part, hipStream_t s);

@@ -699,7 +699,7 @@ __global__ void __launch_bounds__(64 * kWaveTrotterWaves)
 k_trotter_wave(const float2* __restrict__ H, const BfInfo* __restrict__ info, GeomPtrs gp,
                const float2* __restrict__ rowph_g, const float2* __restrict__ colph_g,
                const float2* __restrict__ echi_g, ApIso ap, df rout2, const float2* __restrict__ tw_g, int groups,
-               float2* __restrict__ S_part) {
+               float2* __restrict__ S_part, int part, int parts) {
   constexpr int HALF = N / 2;
   constexpr int R = N / 64;
   static_assert(R == 4, "the in-register pass is radix 4");
@@ -721,15 +721,23 @@ k_trotter_wave(const float2* __restrict__ H, const BfInfo* __restrict__ info, Ge
     qxS[i] = qx;
     q2xS[i] = df_mul(qx, qx);
   }
-  const int nbf = info->nbf;
-  const long long items = (long long)groups * NCG;
+  // part `part` of `parts`: BF pixels [lo, hi) into partial planes
+  // [part * gpp, (part + 1) * gpp), gpp = groups / parts
+  const int nbf_all = info->nbf;
+  const int lo = (int)((long long)nbf_all * part / parts);
+  const int hi = (int)((long long)nbf_all * (part + 1) / parts);
+  const int gpp = groups / parts;
+  const long long items = (long long)gpp * NCG;
   for (long long item = blockIdx.x; item < items; item += gridDim.x) {
     // workgroups go round-robin over the 8 XCDs; rotating the column group
     // by the BF group spreads the heavier slot-0 items over all of them
-    const int g = (int)(item / NCG);
-    const int cg = (int)((item + g) % NCG);
+    const int gl = (int)(item / NCG);
+    const int cg = (int)((item + gl) % NCG);
+    const int g = part * gpp + gl;
     int b0, b1;
-    group_range(g, groups, nbf, b0, b1);
+    group_range(gl, gpp, hi - lo, b0, b1);
+    b0 += lo;
+    b1 += lo;
     __syncthreads();
     for (int i = tid; i < (CG + 1) * N; i += 64 * W) {
       const int c = i / N, q = i - (i / N) * N;
@@ -844,12 +852,12 @@ k_reduce_ifft_cols(const float2* __restrict__ S_part, const BfInfo* __restrict__
     kr.init();
     ki.init();
     int g = 0;
-    for (; g + 8 <= groups; g += 8) {      // eight loads in flight, then the adds in group order
-      float2 v[8];
+    for (; g + 16 <= groups; g += 16) {    // sixteen loads in flight, then the adds in group order
+      float2 v[16];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) v[u] = S_part[(long long)(g + u) * NN + e];
+      for (int u = 0; u < 16; ++u) v[u] = S_part[(long long)(g + u) * NN + e];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (int u = 0; u < 16; ++u) {
         kr.add(v[u].x);
         ki.add(v[u].y);
       }
@@ -949,22 +957,28 @@ int trotter_variants(int N) { return N == 1024 ? 1 : (N == 256 ? 4 : 3); }
 
 int trotter_groups_max(int N) { return N == 256 ? kWaveTrotterGroups : object_groups(N); }
 
+bool trotter_wave_path(int N, bool from_h, const Optics& o, bool do_trotter, bool has_gt, int variant) {
+  return N == 256 && from_h && optics_isotropic(o) && do_trotter && !has_gt && variant == 0;
+}
+
 hipError_t launch_trotter_slots(int N, bool from_h, const float2* src, const BfInfo* info, const GeomPtrs& gp,
                                 const float2* rowph, const float2* colph, const float2* echi, const Optics& o,
                                 const float2* tw, float2* S_part, float2* Gt, bool do_trotter, int variant,
-                                hipStream_t s, int* groups_used) {
+                                hipStream_t s, int* groups_used, int part, int parts) {
   df rout2;
   const ApIso ap = iso_aperture(o, rout2);
   const int dt = do_trotter ? 1 : 0;
   const bool iso = optics_isotropic(o);
-  if (N == 256 && from_h && iso && do_trotter && !Gt && variant == 0) {
+  if (trotter_wave_path(N, from_h, o, do_trotter, Gt != nullptr, variant)) {
     constexpr int groups = kWaveTrotterGroups;
-    const unsigned grid = grid_items((long long)groups * ((256 / 2) / (2 * kWaveTrotterWaves)), 65536);
+    if (parts < 1 || part < 0 || part >= parts || groups % parts) return hipErrorInvalidValue;
+    const unsigned grid = grid_items((long long)(groups / parts) * ((256 / 2) / (2 * kWaveTrotterWaves)), 65536);
     hipLaunchKernelGGL((k_trotter_wave<256>), dim3(grid), dim3(64 * kWaveTrotterWaves), 0, s, src, info, gp, rowph,
-                       colph, echi, ap, rout2, tw, groups, S_part);
+                       colph, echi, ap, rout2, tw, groups, S_part, part, parts);
     if (groups_used) *groups_used = groups;
     return hipGetLastError();
   }
+  if (parts != 1) return hipErrorInvalidValue;
   // workgroup-tile shapes: at N = 256 their variants follow the wave-local one
   if (N == 256 && variant > 0) variant -= 1;
   const int groups = object_groups(N);
@@ -1058,7 +1072,7 @@ hipError_t launch_finish_object(int N, const float2* S_part, int groups, const B
       hipLaunchKernelGGL((k_ifft_rows_phase<128, 8>), dim3(128 / 8), dim3(256), 0, s, S, tw, obj, phase);
       break;
     case 256:
-      hipLaunchKernelGGL((k_reduce_ifft_cols<256, 2>), dim3(256 / 2), dim3(256), 0, s, S_part, info, G, tw, S);
+      hipLaunchKernelGGL((k_reduce_ifft_cols<256, 1>), dim3(256), dim3(256), 0, s, S_part, info, G, tw, S);
       hipLaunchKernelGGL((k_ifft_rows_phase<256, 4>), dim3(256 / 4), dim3(256), 0, s, S, tw, obj, phase);
       break;
     case 512:

@@ -51,6 +51,7 @@ struct ssb_plan {
   int tune_gather_pairs = 0;
   int tune_trotter = 0;
   int tune_superset = 0;     // 1: compact BF superset copy; 0/2: the gather reads the frames
+  int tune_split = 0;        // 0/1: gather and wave-local trotter in two BF halves on two streams; 2: one pass
   // bright-field superset of the reconstruction path (allocated on first use)
   uint32_t* sup_part = nullptr;
   qssb::SupBox* sup_box = nullptr;
@@ -90,6 +91,11 @@ struct ssb_plan {
   std::vector<ExactTrial> xt_host;
   std::vector<int> xt_pattern;         // per trial: higher-order chi (topology of the graph)
   hipStream_t gstream = nullptr;       // the graph's own stream
+  // second stream of the reconstruction: the gather of the second half of
+  // the BF pixels runs on it while the first half's trotter runs on the
+  // caller's stream
+  hipStream_t pstream = nullptr;
+  hipEvent_t pev_a = nullptr, pev_b = nullptr;
   hipEvent_t gev_in = nullptr, gev_out = nullptr;
   hipGraph_t xgraph = nullptr;
   hipGraphExec_t xexec = nullptr;

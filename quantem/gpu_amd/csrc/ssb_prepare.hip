@@ -420,6 +420,7 @@ k_select_bf(const float* __restrict__ mean_g, const unsigned long long* __restri
   // each thread owns the contiguous pixels [p0, p0 + per)
   const int per = (D + MASK_THREADS - 1) / MASK_THREADS;   // <= MASK_PER
   if (CACHE) {
+#pragma unroll 16
     for (int i = tid; i < D; i += MASK_THREADS) smean[i + i / per] = mean_g[i];
     __syncthreads();
   }
@@ -520,9 +521,33 @@ k_select_bf(const float* __restrict__ mean_g, const unsigned long long* __restri
       if (dist2_f32((float)r_, (float)c_, center_r, center_c) <= r2) {
         mask_bits |= 1ull << (i - p0);
         local += 1;
-        if (exact_dc) dcs += (long long)sums[i];
-        else kd.add(dcpix[i]);
         if (r_ < box.r0 || r_ >= box.r0 + box.bh || c_ < box.c0 || c_ >= box.c0 + box.bw) outside += 1;
+      }
+    }
+  }
+  // the kept pixels' dc terms, loads issued 8 at a time and added in pixel
+  // order (the integer sums are exact in any order anyway)
+  for (int i0 = p0; i0 < p1; i0 += 8) {
+    if (exact_dc) {
+      unsigned long long v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = i0 + u;
+        v[u] = (i < p1 && ((mask_bits >> (i - p0)) & 1ull)) ? sums[i] : 0ull;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) dcs += (long long)v[u];
+    } else {
+      float v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = i0 + u;
+        v[u] = (i < p1 && ((mask_bits >> (i - p0)) & 1ull)) ? dcpix[i] : 0.0f;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = i0 + u;
+        if (i < p1 && ((mask_bits >> (i - p0)) & 1ull)) kd.add(v[u]);
       }
     }
   }
@@ -733,7 +758,7 @@ k_gather_rowfft(const T* __restrict__ data, int D, int det_cols,
                 const int* __restrict__ bf_rows, const int* __restrict__ bf_cols,
                 const BfInfo* __restrict__ info, const float2* __restrict__ tw_g,
                 const float* __restrict__ gain, float2* __restrict__ H, const SupBox* __restrict__ boxp,
-                const uint16_t* __restrict__ C) {
+                const uint16_t* __restrict__ C, int part, int parts) {
   constexpr int STRIDE = N + 1;
   constexpr int CH = 2 * PAIRS;
   __shared__ float2 z[PAIRS * STRIDE];
@@ -742,8 +767,12 @@ k_gather_rowfft(const T* __restrict__ data, int D, int det_cols,
   __shared__ float gsc[CH];
   const int tid = threadIdx.x;
   for (int i = tid; i < N; i += 256) tw[i] = tw_g[i];
-  const int nbf = info->nbf;
-  const int nchunks = (nbf + CH - 1) / CH;
+  // BF pixels [lo, hi) of this part (raster order: the parts read disjoint
+  // detector rows, so splitting adds no frame traffic)
+  const int nbf_all = info->nbf;
+  const int lo = (int)((long long)nbf_all * part / parts);
+  const int hi = (int)((long long)nbf_all * (part + 1) / parts);
+  const int nchunks = (hi - lo + CH - 1) / CH;
   const long long items = (long long)nchunks * N;
   // compact superset copy: frame stride SB, pixel (r, c) at (r - r0) * bw + (c - c0)
   const bool sup = std::is_same<T, uint16_t>::value && C != nullptr && info->sup_ok != 0;
@@ -759,11 +788,11 @@ k_gather_rowfft(const T* __restrict__ data, int D, int det_cols,
     const long long xcd = item & 7, t = item >> 3;
     const int rr = (int)((t / nchunks) * 8 + xcd);
     const int chunk = (int)(t % nchunks);
-    const int b0 = chunk * CH;
+    const int b0 = lo + chunk * CH;
     __syncthreads();
     if (tid < CH) {
       const int b = b0 + tid;
-      const int o = (b < nbf) ? bf_rows[b] * det_cols + bf_cols[b] : -1;
+      const int o = (b < hi) ? bf_rows[b] * det_cols + bf_cols[b] : -1;
       offs[tid] = (o < 0) ? -1 : (sup ? (bf_rows[b] - box.r0) * box.bw + (bf_cols[b] - box.c0) : o);
       gsc[tid] = (gain && o >= 0) ? gain[o] : 1.0f;
     }
@@ -798,7 +827,7 @@ k_gather_rowfft(const T* __restrict__ data, int D, int det_cols,
       const int jb = it / HALF;           // BF within chunk
       const int k = it - jb * HALF;       // slot
       const int b = b0 + jb;
-      if (b >= nbf) continue;
+      if (b >= hi) continue;
       const int p = jb >> 1;
       const float2 zk = z[p * STRIDE + k];
       float2 out;
@@ -1120,11 +1149,14 @@ int gather_pairs_candidates(int N, int* out) {
 
 hipError_t launch_gather_rowfft(int N, int dtype, const void* data, int D, int det_cols, const int* rows,
                                 const int* cols, const BfInfo* info, int nbf_hint, const float2* tw, int pairs,
-                                const float* gain, float2* H, hipStream_t s, const SupBox* box, const uint16_t* C) {
+                                const float* gain, float2* H, hipStream_t s, const SupBox* box, const uint16_t* C,
+                                int part, int parts) {
   if (!box) C = nullptr;
   if (pairs <= 0) pairs = gather_pairs_default(N);
+  if (parts < 1 || part < 0 || part >= parts) return hipErrorInvalidValue;
+  const int part_hint = (nbf_hint + parts - 1) / parts + 1;   // BF pixels of the largest part
   auto grid_for = [&](int ch) {
-    long long items = (long long)((nbf_hint + ch - 1) / ch) * N;
+    long long items = (long long)((part_hint + ch - 1) / ch) * N;
     if (items < 1) items = 1;
     if (items > 16384) items = 16384;
     return dim3((unsigned)items);
@@ -1132,7 +1164,8 @@ hipError_t launch_gather_rowfft(int N, int dtype, const void* data, int D, int d
 #define QSSB_GATHER(NN, PP, T)                                                                              \
   if (N == NN && pairs == PP) {                                                                             \
     hipLaunchKernelGGL((k_gather_rowfft<NN, PP, T>), grid_for(2 * PP), dim3(256), 0, s,                     \
-                       reinterpret_cast<const T*>(data), D, det_cols, rows, cols, info, tw, gain, H, box, C); \
+                       reinterpret_cast<const T*>(data), D, det_cols, rows, cols, info, tw, gain, H, box, C,  \
+                       part, parts);                                                                        \
     return hipGetLastError();                                                                               \
   }
   if (dtype == QSSB_DTYPE_U16) {

@@ -24,9 +24,10 @@ import numpy as np
 
 from . import _abi
 
-KNOBS = {"stream_wg": 0, "gather_pairs": 1, "trotter": 2}
-# the recon stage whose time each knob changes
-STAGE = {"stream_wg": "mean_dp_stream", "gather_pairs": "gather_rowfft", "trotter": "colfft_trotter"}
+KNOBS = {"stream_wg": 0, "gather_pairs": 1, "trotter": 2, "split": 4}
+# the recon stages whose time each knob changes
+STAGE = {"stream_wg": ("mean_dp_stream",), "gather_pairs": ("gather_rowfft",), "trotter": ("colfft_trotter",),
+         "split": ("gather_rowfft", "colfft_trotter")}
 
 
 def candidates(scan_n: int, knob: str) -> list[int]:
@@ -72,14 +73,16 @@ def load_cached(engine, nbf: int, path: Path | None = None) -> dict | None:
     return entry
 
 
-def _time_stage(engine, run, stage: str, repeats: int) -> float:
-    """Median per-call milliseconds of one recon stage over ``repeats`` calls."""
+def _time_stage(engine, run, stages, repeats: int) -> float:
+    """Median per-call milliseconds of the recon stages ``stages`` (summed)
+    over ``repeats`` calls."""
     run()                                   # warm the new launch shape
     times = []
     for _ in range(repeats):
         engine.timing_begin(1)
         run()
-        times.append(engine.timing_end()[stage])
+        t = engine.timing_end()
+        times.append(sum(t[st] for st in stages))
     return float(np.median(times))
 
 
