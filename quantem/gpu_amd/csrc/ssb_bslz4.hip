@@ -21,8 +21,9 @@
 //                   same sequence headers; literal and match bytes are
 //                   copied by all lanes, a match of offset d < length
 //                   copied in one pass as out[o + t] = out[o - d + t mod d]),
-//                   then inverts the bit-shuffle, 8 elements per lane step,
-//                   with 16-byte stores.
+//                   then inverts the bit-shuffle, 8 elements per lane step
+//                   (two 8x8 bit-matrix transposes by delta swaps and byte
+//                   permutes), with 16-byte stores.
 // Malformed input never reads or writes out of bounds: it sets a bit in the
 // status word and the block is skipped.
 #include <cstring>
@@ -205,26 +206,44 @@ k_bslz4_decode(const uint8_t* __restrict__ comp, const BlockDesc* __restrict__ t
       continue;                                        // uniform: every lane saw the same stream
     }
     __syncthreads();
-    // inverse bit-shuffle: 8 elements per step from 16 plane bytes
+    // inverse bit-shuffle, 8 elements per step: the 16 plane bytes of the
+    // group form two 8x8 bit matrices (planes 0-7 and 8-15, byte = plane,
+    // bit = element); each is transposed in a 64-bit word by three
+    // delta swaps, after which byte i holds element i's low or high byte,
+    // and v_perm interleaves them into the output words.
     const int planeb = bd.elems / 8;
     for (int g = lane; g < planeb; g += 64) {
-      unsigned v[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+      unsigned bt[16];
 #pragma unroll
-      for (int b = 0; b < 16; ++b) {
-        const unsigned byte = cout[b * planeb + g];
+      for (int b = 0; b < 16; ++b) bt[b] = cout[b * planeb + g];
+      unsigned long long m[2];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) v[i] |= ((byte >> i) & 1u) << b;
+      for (int h = 0; h < 2; ++h) {
+        const unsigned lo = bt[8 * h] | (bt[8 * h + 1] << 8) | (bt[8 * h + 2] << 16) | (bt[8 * h + 3] << 24);
+        const unsigned hi = bt[8 * h + 4] | (bt[8 * h + 5] << 8) | (bt[8 * h + 6] << 16) | (bt[8 * h + 7] << 24);
+        unsigned long long x = ((unsigned long long)hi << 32) | lo;
+        unsigned long long t = (x ^ (x >> 7)) & 0x00AA00AA00AA00AAull;
+        x = x ^ t ^ (t << 7);
+        t = (x ^ (x >> 14)) & 0x0000CCCC0000CCCCull;
+        x = x ^ t ^ (t << 14);
+        t = (x ^ (x >> 28)) & 0x00000000F0F0F0F0ull;
+        x = x ^ t ^ (t << 28);
+        m[h] = x;
       }
+      const unsigned xl = (unsigned)m[0], xh = (unsigned)(m[0] >> 32);
+      const unsigned yl = (unsigned)m[1], yh = (unsigned)(m[1] >> 32);
+      // bytes {x.b0, y.b0, x.b1, y.b1} = elements 0 and 1, and so on
+      uint4 w;
+      w.x = __builtin_amdgcn_perm(yl, xl, 0x05010400u);
+      w.y = __builtin_amdgcn_perm(yl, xl, 0x07030602u);
+      w.z = __builtin_amdgcn_perm(yh, xh, 0x05010400u);
+      w.w = __builtin_amdgcn_perm(yh, xh, 0x07030602u);
       if ((bd.dst & 7) == 0) {                       // uniform: 16-byte aligned groups
-        uint4 w;
-        w.x = v[0] | (v[1] << 16);
-        w.y = v[2] | (v[3] << 16);
-        w.z = v[4] | (v[5] << 16);
-        w.w = v[6] | (v[7] << 16);
         *reinterpret_cast<uint4*>(out + bd.dst + 8LL * g) = w;
       } else {
+        const unsigned ww[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
-        for (int i = 0; i < 8; ++i) out[bd.dst + 8LL * g + i] = (uint16_t)v[i];
+        for (int i = 0; i < 8; ++i) out[bd.dst + 8LL * g + i] = (uint16_t)(ww[i >> 1] >> (16 * (i & 1)));
       }
     }
   }
