@@ -118,11 +118,14 @@ k_corr_colifft(const float2* __restrict__ Gt, const BfInfo* __restrict__ info, G
       chi_parts(cp, kx.hi, ky.hi, e, od);
       chik = e + od;
     }
-    df rowTA{0.f, 0.f}, rowTB{0.f, 0.f}, cb{0.f, 0.f};
+    // ISO row terms of d(Q -/+ k) as float32 estimates (double-float only
+    // for points near the soft edge)
+    const df kx2{2.0f * kx.hi, 2.0f * kx.lo};
+    float uq = 0.0f, vq = 0.0f;
+    df cb{0.f, 0.f};
     if constexpr (ISO) {
-      const df u = df_mul(qxA, df{2.0f * kx.hi, 2.0f * kx.lo});
-      rowTA = df_sub(u, q2x);
-      rowTB = df_neg(df_add(q2x, u));
+      uq = row_estimate_minus(qxA, kx2, q2x);
+      vq = row_estimate_plus(qxA, kx2, q2x);
     } else {
       cb = df_sub(rout2, df_add(df_mul(kx, kx), df_mul(ky, ky)));
     }
@@ -141,8 +144,7 @@ k_corr_colifft(const float2* __restrict__ Gt, const BfInfo* __restrict__ info, G
       const float2 colp = POLY ? make_float2(1.f, 0.f) : colph_g[(long long)b * GCOLS + j];
       float am, app;
       if constexpr (ISO) {
-        am = aperture_sum(rowTA, colA_s[s], ap);
-        app = aperture_sum(rowTB, colB_s[s], ap);
+        iso_aperture_pair(uq, vq, qxA, kx2, q2x, colA_s[s], colB_s[s], ap, am, app);
       } else {
         aperture_pair<ISO>(qxA, qy, kx, ky, cb, q2A, ap, o, am, app);
       }
@@ -158,8 +160,8 @@ k_corr_colifft(const float2* __restrict__ Gt, const BfInfo* __restrict__ info, G
       } else {
         echiB = POLY ? make_float2(1.f, 0.f) : ((j == 0) ? ehalfc[q] : enyqr[s]);
         if constexpr (ISO) {
-          bm = aperture_sum(rowTA, (j == 0) ? colA_s[SG] : colB_s[s], ap);
-          bp = aperture_sum(rowTB, (j == 0) ? colB_s[SG] : colA_s[s], ap);
+          iso_aperture_pair(uq, vq, qxA, kx2, q2x, (j == 0) ? colA_s[SG] : colB_s[s],
+                            (j == 0) ? colB_s[SG] : colA_s[s], ap, bm, bp);
         } else {
           const df qyB = (j == 0) ? df{qyS[SG], qySl[SG]} : df{-qy.hi, -qy.lo};
           const df qxB = (j == 0) ? qxA : qxB0;

@@ -239,13 +239,14 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
         }
         continue;
       }
-      // ISO row terms of d(Q -/+ k): -qx^2 +/- 2 qx kx
-      df rowTA{0.f, 0.f}, rowTB{0.f, 0.f};
+      // ISO row terms of d(Q -/+ k), -qx^2 +/- 2 qx kx, as float32 estimates
+      // (the double-float terms are formed only for points near the soft edge)
+      const df kx2{2.0f * kx.hi, 2.0f * kx.lo};
+      float uq = 0.0f, vq = 0.0f;
       df cb{0.f, 0.f};
       if constexpr (ISO) {
-        const df u = df_mul(qxA, df{2.0f * kx.hi, 2.0f * kx.lo});
-        rowTA = df_sub(u, q2x);
-        rowTB = df_neg(df_add(q2x, u));
+        uq = row_estimate_minus(qxA, kx2, q2x);
+        vq = row_estimate_plus(qxA, kx2, q2x);
       } else {
         cb = df_sub(rout2, df_add(df_mul(kx, kx), df_mul(ky, ky)));
       }
@@ -271,8 +272,7 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
         const df qy{qyS[s], qySl[s]};
         float am, app;
         if constexpr (ISO) {
-          am = aperture_sum(rowTA, colA_s[cur][s], ap);
-          app = aperture_sum(rowTB, colB_s[cur][s], ap);
+          iso_aperture_pair(uq, vq, qxA, kx2, q2x, colA_s[cur][s], colB_s[cur][s], ap, am, app);
         } else {
           aperture_pair<ISO>(qxA, qy, kx, ky, cb, q2A[i], ap, o, am, app);
         }
@@ -304,8 +304,8 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
           float bm, bp;
           if constexpr (ISO) {
             // j == 0: row q, column N/2; q == N/2: row N/2 (this row), column -qy
-            bm = aperture_sum(rowTA, (j == 0) ? colA_s[cur][SG] : colB_s[cur][s], ap);
-            bp = aperture_sum(rowTB, (j == 0) ? colB_s[cur][SG] : colA_s[cur][s], ap);
+            iso_aperture_pair(uq, vq, qxA, kx2, q2x, (j == 0) ? colA_s[cur][SG] : colB_s[cur][s],
+                              (j == 0) ? colB_s[cur][SG] : colA_s[cur][s], ap, bm, bp);
           } else {
             const df qyB = (j == 0) ? df{qyS[SG], qySl[SG]} : df{-qy.hi, -qy.lo};
             const df qxB = (j == 0) ? qxA : qxB0;
@@ -380,11 +380,12 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
 // at a time; H and the phasors of the next BF pixel are loaded while this
 // one is transformed.
 // ------------------------------------------------------------------------
-constexpr int kWaveTrotterWaves = 8;
+constexpr int kWaveColumnsDefault = 2;   // slot columns per wave of the default variant
 constexpr int kWaveTrotterGroups = 64;
 constexpr int kWaveChunk = 64;           // BF pixels per column-term table / Nyquist pass
 
 __device__ __forceinline__ void wave_lds_fence() { asm volatile("" ::: "memory"); }
+
 
 // Column terms of d(Q -/+ k) for one (BF pixel, column): (Rout^2 - |k|^2) -
 // qy^2 +/- 2 qy ky (isotropic sampling).
@@ -399,24 +400,36 @@ __device__ __forceinline__ ColTerm col_term(df rout2, df kx, df ky, df qy) {
   return ColTerm{df_add(base, qk), df_sub(base, qk)};
 }
 
+// After the wave-local column FFT, lane l holds frequencies
+// q = wave_qlane(l) + 64 r (r = 0..3) of each of its columns: the outputs of the
+// last Stockham stage stay in the registers of the lane that computed them.
+// Per-row tables in LDS are stored in that order, entry 64 r + l for lane l.
+__device__ __forceinline__ int wave_qlane(int lane) { return (lane >> 4) + 4 * (lane & 15); }
+
+// Column c of one row's load: a float4 holds two adjacent slot columns.
+__device__ __forceinline__ float2 column_of(float4 v, int c) {
+  return c ? make_float2(v.z, v.w) : make_float2(v.x, v.y);
+}
+__device__ __forceinline__ float2 column_of(float2 v, int) { return v; }
+
 // Per-BF operands of one wave other than H (loaded one BF pixel ahead).
-template <int R>
+template <int R, int CPW>
 struct WaveBf {
-  float2 rowA[R];       // e^{i qx a_b} at the lane's rows
-  float2 colp[2];       // e^{i qy b_b} at the wave's columns
+  float2 rowA[R];       // e^{i qx a_b} at the lane's frequencies
+  float2 colp[CPW];     // e^{i qy b_b} at the wave's columns
   float2 colHalf;       // ... at column N/2
   float kx, kxl, ak;
 };
 
-template <int N, int R>
-__device__ __forceinline__ void wave_bf_load(WaveBf<R>& o, const float2* __restrict__ rowph_g,
+template <int N, int R, int CPW>
+__device__ __forceinline__ void wave_bf_load(WaveBf<R, CPW>& o, const float2* __restrict__ rowph_g,
                                              const float2* __restrict__ colph_g, const GeomPtrs& gp, int b,
                                              int lane, int j0) {
   constexpr int HALF = N / 2, GCOLS = HALF + 1;
 #pragma unroll
-  for (int m = 0; m < R; ++m) o.rowA[m] = rowph_g[(long long)b * N + lane + 64 * m];
-  o.colp[0] = colph_g[(long long)b * GCOLS + j0];
-  o.colp[1] = colph_g[(long long)b * GCOLS + j0 + 1];
+  for (int m = 0; m < R; ++m) o.rowA[m] = rowph_g[(long long)b * N + wave_qlane(lane) + 64 * m];
+#pragma unroll
+  for (int c = 0; c < CPW; ++c) o.colp[c] = colph_g[(long long)b * GCOLS + j0 + c];
   o.colHalf = colph_g[(long long)b * GCOLS + HALF];
   o.kx = gp.kx[b];
   o.kxl = gp.kxl[b];
@@ -436,39 +449,32 @@ __device__ __forceinline__ float2 trotter_point(float2 G, float am, float app, f
   return make_float2(G.x * gr + G.y * gi, G.y * gr - G.x * gi);
 }
 
-// The aperture from a float32 estimate of d when it lies beyond the 1e-4
-// margin of the soft band (the value aperture_sum gives there); false when
-// the estimate is too close to the band to decide.
-__device__ __forceinline__ bool aperture_decided(float est, const ApIso& ap, float& a) {
-  constexpr float kMargin = 1e-4f;
-  a = (est >= ap.d_in + kMargin) ? 1.0f : 0.0f;
-  return est >= ap.d_in + kMargin || est <= ap.d_out - kMargin;
-}
-
-// One wave's share of a work item: BF pixels [b0, b1) for slot columns j0,
-// j0 + 1; SLOT0 for the wave holding slot 0, whose first column packs the
-// real columns 0 and N/2.  Every wave of the workgroup calls it with the
-// same BF range (the barriers of the column-term tables match).
-template <int N, bool SLOT0>
+// One wave's share of a work item: BF pixels [b0, b1) for the CPW slot
+// columns j0, j0 + 1, ...; SLOT0 for the wave holding slot 0, whose first
+// column packs the real columns 0 and N/2.  Every wave of the workgroup calls
+// it with the same BF range (the barriers of the column-term tables match).
+template <int N, int CPW, int NW, bool SLOT0>
 __device__ __forceinline__ void wave_trotter_item(const float2* __restrict__ H, const GeomPtrs& gp,
                                                   const float2* __restrict__ rowph_g,
                                                   const float2* __restrict__ colph_g,
                                                   const float2* __restrict__ echi_g, const ApIso& ap, df rout2,
                                                   const float2* __restrict__ tw, float2* __restrict__ xb,
-                                                  float2* __restrict__ nyqz, ColTerm (*colT)[2 * kWaveTrotterWaves + 1],
+                                                  float2* __restrict__ nyqz,
+                                                  ColTerm (*colT)[CPW * NW + 1],
                                                   const float2 (*echiS)[N], const df* __restrict__ qxS,
                                                   const df* __restrict__ q2xS, int cg, int w, int tid, int b0, int b1,
                                                   float2* __restrict__ sp) {
   constexpr int HALF = N / 2;
   constexpr int R = N / 64;
   constexpr int SP = 65;                 // padded sequence stride in the wave's LDS
-  constexpr int CG = 2 * kWaveTrotterWaves;
+  constexpr int CG = CPW * NW;
   constexpr int NBH = SLOT0 ? R : 1;     // column N/2 accumulators (slot 0 only)
+  static_assert(CPW == 1 || CPW == 2, "one or two columns per wave");
   const int lane = tid & 63;
-  const int j0 = cg * CG + 2 * w;
-  Kahan aR[2][R], aI[2][R], hR[NBH], hI[NBH];
+  const int j0 = cg * CG + CPW * w;
+  Kahan aR[CPW][R], aI[CPW][R], hR[NBH], hI[NBH];
 #pragma unroll
-  for (int c = 0; c < 2; ++c)
+  for (int c = 0; c < CPW; ++c)
 #pragma unroll
     for (int m = 0; m < R; ++m) {
       aR[c][m].init();
@@ -480,23 +486,34 @@ __device__ __forceinline__ void wave_trotter_item(const float2* __restrict__ H, 
     hI[i].init();
   }
   // Nyquist-row sums (wave-uniform), value + running error term
-  df nyR[2] = {df{0.f, 0.f}, df{0.f, 0.f}}, nyI[2] = {df{0.f, 0.f}, df{0.f, 0.f}};
-  const float4* hrow = reinterpret_cast<const float4*>(H + (long long)lane * HALF + j0);
-  constexpr long long ROWS64 = 64LL * HALF / 2;     // float4 between rows l and l + 64
-  constexpr long long PER_B = (long long)N * HALF / 2;
-  float4 h[R];
-  WaveBf<R> nx;
+  df nyR[CPW], nyI[CPW];
+#pragma unroll
+  for (int c = 0; c < CPW; ++c) {
+    nyR[c] = df{0.f, 0.f};
+    nyI[c] = df{0.f, 0.f};
+  }
+  // one load per row brings the wave's CPW columns (16 or 8 bytes)
+  using HV = typename std::conditional<CPW == 2, float4, float2>::type;
+  const HV* hrow = reinterpret_cast<const HV*>(H + (long long)lane * HALF + j0);
+  constexpr long long ROWS64 = 64LL * HALF / CPW;   // HV between rows l and l + 64
+  constexpr long long PER_B = (long long)N * HALF / CPW;
+  HV h[R];
+  // the per-BF operands other than H are loaded one BF pixel ahead with two
+  // columns per wave; with one (a register budget of 168) at the top of the
+  // BF pixel's iteration, where the column FFT covers their latency
+  constexpr bool BF_AHEAD = CPW == 2;
+  WaveBf<R, CPW> nx;
   if (b0 < b1) {
 #pragma unroll
     for (int m = 0; m < R; ++m) h[m] = hrow[(long long)b0 * PER_B + m * ROWS64];
-    wave_bf_load<N, R>(nx, rowph_g, colph_g, gp, b0, lane, j0);
+    if (BF_AHEAD) wave_bf_load<N, R, CPW>(nx, rowph_g, colph_g, gp, b0, lane, j0);
   }
   for (int c0 = b0; c0 < b1; c0 += kWaveChunk) {
     const int n = min(kWaveChunk, b1 - c0);
-    // column terms of the chunk's BF pixels at the workgroup's 16 columns
+    // column terms of the chunk's BF pixels at the workgroup's CG columns
     // and column N/2, computed once for all waves
     __syncthreads();
-    for (int e = tid; e < n * (CG + 1); e += 64 * kWaveTrotterWaves) {
+    for (int e = tid; e < n * (CG + 1); e += 64 * NW) {
       const int bi = e / (CG + 1), ci = e - (e / (CG + 1)) * (CG + 1);
       const int b = c0 + bi;
       const int col = (ci < CG) ? cg * CG + ci : HALF;
@@ -505,18 +522,23 @@ __device__ __forceinline__ void wave_trotter_item(const float2* __restrict__ H, 
     }
     __syncthreads();
     for (int b = c0; b < c0 + n; ++b) {
-      const WaveBf<R> cu = nx;
-      if (b + 1 < b1) wave_bf_load<N, R>(nx, rowph_g, colph_g, gp, b + 1, lane, j0);
+      WaveBf<R, CPW> cu;
+      if constexpr (BF_AHEAD) {
+        cu = nx;
+        if (b + 1 < b1) wave_bf_load<N, R, CPW>(nx, rowph_g, colph_g, gp, b + 1, lane, j0);
+      } else {
+        wave_bf_load<N, R, CPW>(cu, rowph_g, colph_g, gp, b, lane, j0);
+      }
       // column FFT: in-register radix-4 over m, twiddle, into the wave's LDS.
       // The exchange runs across the lanes of this wave only: a compiler
       // memory barrier at each phase boundary keeps every LDS access in
       // program order (the hardware keeps a wave's LDS operations in order).
       wave_lds_fence();
 #pragma unroll
-      for (int c = 0; c < 2; ++c) {
+      for (int c = 0; c < CPW; ++c) {
         float2 x[R];
 #pragma unroll
-        for (int m = 0; m < R; ++m) x[m] = c ? make_float2(h[m].z, h[m].w) : make_float2(h[m].x, h[m].y);
+        for (int m = 0; m < R; ++m) x[m] = column_of(h[m], c);
         const float2 apc = cadd(x[0], x[2]), amc = csub(x[0], x[2]);
         const float2 bpd = cadd(x[1], x[3]), bmd = cmul_mi<false>(csub(x[1], x[3]));
         xb[(c * R + 0) * SP + lane] = cadd(apc, bpd);
@@ -530,13 +552,17 @@ __device__ __forceinline__ void wave_trotter_item(const float2* __restrict__ H, 
         for (int m = 0; m < R; ++m) h[m] = hrow[(long long)(b + 1) * PER_B + m * ROWS64];
       }
       // 64-point Stockham radix-4 stages over the 2R sequences (two
-      // butterflies per lane): all operands are read before any is written
+      // butterflies per lane): all operands are read before any is written.
+      // The last stage's butterfly p of lane l is column p at frequencies
+      // q = wave_qlane(l) + 64 r; it stays in registers (slot 0's wave also
+      // stores it, for the mirror rows N - q its column 0 needs).
+      float2 zq[CPW][4];
 #pragma unroll
       for (int ns = 1; ns < 64; ns *= 4) {
         wave_lds_fence();
-        float2 v[2][4];
+        float2 v[CPW][4];
 #pragma unroll
-        for (int p = 0; p < 2; ++p) {
+        for (int p = 0; p < CPW; ++p) {
           const int it = lane + 64 * p;
           const int sq = it >> 4, jj = it & 15, k = jj & (ns - 1);
 #pragma unroll
@@ -548,38 +574,63 @@ __device__ __forceinline__ void wave_trotter_item(const float2* __restrict__ H, 
         }
         wave_lds_fence();
 #pragma unroll
-        for (int p = 0; p < 2; ++p) {
+        for (int p = 0; p < CPW; ++p) {
           const int it = lane + 64 * p;
           const int sq = it >> 4, jj = it & 15, k = jj & (ns - 1);
           const int base = sq * SP + (jj / ns) * ns * 4 + k;
           const float2 apc = cadd(v[p][0], v[p][2]), amc = csub(v[p][0], v[p][2]);
           const float2 bpd = cadd(v[p][1], v[p][3]), bmd = cmul_mi<false>(csub(v[p][1], v[p][3]));
-          xb[base] = cadd(apc, bpd);
-          xb[base + ns] = cadd(amc, bmd);
-          xb[base + 2 * ns] = csub(apc, bpd);
-          xb[base + 3 * ns] = csub(amc, bmd);
+          const float2 o[4] = {cadd(apc, bpd), cadd(amc, bmd), csub(apc, bpd), csub(amc, bmd)};
+          if (ns < 16 || SLOT0) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) xb[base + r * ns] = o[r];
+          }
+          if (ns == 16) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) zq[p][r] = o[r];
+          }
         }
       }
       wave_lds_fence();
-      // G_b(N/2, j0 + c) for the deferred Nyquist pass (lane 0 holds q = N/2)
+      // G_b(N/2, j0 + c) for the deferred Nyquist pass (lane 0 holds q = N/2 at r = 2)
+      static_assert(HALF == 128, "q = N/2 is lane 0's r = 2");
       if (lane == 0) {
 #pragma unroll
-        for (int c = 0; c < 2; ++c) nyqz[2 * (b - c0) + c] = xb[(c * R + (HALF & 3)) * SP + (HALF >> 2)];
+        for (int c = 0; c < CPW; ++c) nyqz[CPW * (b - c0) + c] = zq[c][2];
       }
-      const ColTerm ct[2] = {colT[b - c0][2 * w], colT[b - c0][2 * w + 1]};
+      ColTerm ct[CPW];
+#pragma unroll
+      for (int c = 0; c < CPW; ++c) ct[c] = colT[b - c0][CPW * w + c];
       const float ak = cu.ak;
       const df kx2{2.0f * cu.kx, 2.0f * cu.kxl};
+      // LDS operands of the lane's points at q = wave_qlane(lane) + 64 m
+      // (table entry 64 m + lane), read one row ahead so that no point waits
+      // on its own reads
+      struct RowOps {
+        df qx, q2x;
+        float2 e[CPW];
+      };
+      auto row_ops = [&](int m, RowOps& o) {
+        const int i = lane + 64 * m;
+        o.qx = qxS[i];
+        o.q2x = q2xS[i];
+#pragma unroll
+        for (int c = 0; c < CPW; ++c) o.e[c] = echiS[CPW * w + c][i];
+      };
+      RowOps next_ops;
+      row_ops(0, next_ops);
 #pragma unroll
       for (int m = 0; m < R; ++m) {
-        const int q = lane + 64 * m;
-        const df qx = qxS[q];
-        const df q2x = q2xS[q];
-        // float32 estimates of the row terms +/- qx 2kx - qx^2
-        const float uq = fmaf(qx.hi, kx2.hi, -q2x.hi);
-        const float vq = -fmaf(qx.hi, kx2.hi, q2x.hi);
+        const int q = wave_qlane(lane) + 64 * m;
+        const RowOps ops = next_ops;
+        if (m + 1 < R) row_ops(m + 1, next_ops);
+        const df qx = ops.qx;
+        const df q2x = ops.q2x;
+        const float uq = row_estimate_minus(qx, kx2, q2x);
+        const float vq = row_estimate_plus(qx, kx2, q2x);
 #pragma unroll
-        for (int c = 0; c < 2; ++c) {
-          const float2 z = xb[(c * R + (lane & 3)) * SP + (lane >> 2) + 16 * m];
+        for (int c = 0; c < CPW; ++c) {
+          const float2 z = zq[c][m];
           float2 GA;
           if (SLOT0 && c == 0) {
             const int qq = (N - q) & (N - 1);
@@ -589,15 +640,9 @@ __device__ __forceinline__ void wave_trotter_item(const float2* __restrict__ H, 
             GA = z;
           }
           float am, app;
-          const bool da = aperture_decided(uq + ct[c].A.hi, ap, am);
-          const bool dp = aperture_decided(vq + ct[c].B.hi, ap, app);
-          if (!(da && dp)) {
-            const df u = df_mul(qx, kx2);
-            if (!da) am = aperture_from_d(df_add(df_sub(u, q2x), ct[c].A), ap);
-            if (!dp) app = aperture_from_d(df_add(df_neg(df_add(q2x, u)), ct[c].B), ap);
-          }
+          iso_aperture_pair(uq, vq, qx, kx2, q2x, ct[c].A, ct[c].B, ap, am, app);
           if (am != 0.0f || app != 0.0f) {
-            const float2 t = trotter_point(GA, am, app, echiS[2 * w + c][q], cmul(cu.rowA[m], cu.colp[c]), ak);
+            const float2 t = trotter_point(GA, am, app, ops.e[c], cmul(cu.rowA[m], cu.colp[c]), ak);
             aR[c][m].add(t.x);
             aI[c][m].add(t.y);
           }
@@ -609,15 +654,9 @@ __device__ __forceinline__ void wave_trotter_item(const float2* __restrict__ H, 
               const float2 GB = make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x));
               const ColTerm th = colT[b - c0][CG];
               float bm, bp;
-              const bool dm = aperture_decided(uq + th.A.hi, ap, bm);
-              const bool dq = aperture_decided(vq + th.B.hi, ap, bp);
-              if (!(dm && dq)) {
-                const df u = df_mul(qx, kx2);
-                if (!dm) bm = aperture_from_d(df_add(df_sub(u, q2x), th.A), ap);
-                if (!dq) bp = aperture_from_d(df_add(df_neg(df_add(q2x, u)), th.B), ap);
-              }
+              iso_aperture_pair(uq, vq, qx, kx2, q2x, th.A, th.B, ap, bm, bp);
               if (bm != 0.0f || bp != 0.0f) {
-                const float2 t = trotter_point(GB, bm, bp, echiS[CG][q], cmul(cu.rowA[m], cu.colHalf), ak);
+                const float2 t = trotter_point(GB, bm, bp, echiS[CG][lane + 64 * m], cmul(cu.rowA[m], cu.colHalf), ak);
                 hR[m].add(t.x);
                 hI[m].add(t.y);
               }
@@ -632,7 +671,9 @@ __device__ __forceinline__ void wave_trotter_item(const float2* __restrict__ H, 
     // Nyquist-row terms of the chunk: lane i takes BF pixel c0 + i
     wave_lds_fence();
     {
-      float2 t[2] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+      float2 t[CPW];
+#pragma unroll
+      for (int c = 0; c < CPW; ++c) t[c] = make_float2(0.f, 0.f);
       if (lane < n) {
         const int b = c0 + lane;
         const float ak = gp.ak[b];
@@ -641,23 +682,23 @@ __device__ __forceinline__ void wave_trotter_item(const float2* __restrict__ H, 
         const df rowTB = df_neg(df_add(q2xS[HALF], u));
         const float2 rA = rowph_g[(long long)b * N + HALF];
 #pragma unroll
-        for (int c = 0; c < 2; ++c) {
+        for (int c = 0; c < CPW; ++c) {
           const int j = j0 + c;
           if (j == 0) continue;          // slot 0: column N/2 holds this point
-          const ColTerm ctn = colT[lane][2 * w + c];
+          const ColTerm ctn = colT[lane][CPW * w + c];
           const float bm = aperture_sum(rowTA, ctn.B, ap);
           const float bp = aperture_sum(rowTB, ctn.A, ap);
           if (bm != 0.0f || bp != 0.0f) {
             const float2 echiB = echi_g[(long long)((N - j) & (N - 1)) * N + HALF];
             const float2 eB = cmul(rA, cconj(colph_g[(long long)b * (HALF + 1) + j]));
-            t[c] = trotter_point(cconj(nyqz[2 * lane + c]), bm, bp, echiB, eB, ak);
+            t[c] = trotter_point(cconj(nyqz[CPW * lane + c]), bm, bp, echiB, eB, ak);
           }
         }
       }
       // compensated xor tree over the lanes: two_sum's error term is exact,
       // so every lane holds the same (sum, error) pair
 #pragma unroll
-      for (int c = 0; c < 2; ++c) {
+      for (int c = 0; c < CPW; ++c) {
         df re{t[c].x, 0.f}, im{t[c].y, 0.f};
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) {
@@ -676,10 +717,10 @@ __device__ __forceinline__ void wave_trotter_item(const float2* __restrict__ H, 
   }
 #pragma unroll
   for (int m = 0; m < R; ++m) {
-    const int q = lane + 64 * m;
+    const int q = wave_qlane(lane) + 64 * m;
     const int qm = (N - q) & (N - 1);
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
+    for (int c = 0; c < CPW; ++c) {
       const int j = j0 + c;
       const float2 outA = make_float2(aR[c][m].s - aR[c][m].c, aI[c][m].s - aI[c][m].c);
       sp[(long long)j * N + q] = outA;
@@ -694,8 +735,8 @@ __device__ __forceinline__ void wave_trotter_item(const float2* __restrict__ H, 
   }
 }
 
-template <int N>
-__global__ void __launch_bounds__(64 * kWaveTrotterWaves)
+template <int N, int CPW, int NW, int WAVES_PER_SIMD>
+__global__ void __launch_bounds__(64 * NW, WAVES_PER_SIMD)
 k_trotter_wave(const float2* __restrict__ H, const BfInfo* __restrict__ info, GeomPtrs gp,
                const float2* __restrict__ rowph_g, const float2* __restrict__ colph_g,
                const float2* __restrict__ echi_g, ApIso ap, df rout2, const float2* __restrict__ tw_g, int groups,
@@ -703,21 +744,28 @@ k_trotter_wave(const float2* __restrict__ H, const BfInfo* __restrict__ info, Ge
   constexpr int HALF = N / 2;
   constexpr int R = N / 64;
   static_assert(R == 4, "the in-register pass is radix 4");
-  constexpr int W = kWaveTrotterWaves;
-  constexpr int CG = 2 * W;              // slot columns per workgroup
+  constexpr int W = NW;
+  constexpr int CG = CPW * W;            // slot columns per workgroup
   constexpr int NCG = HALF / CG;
+  // LG workgroups share the 16 slot columns of one 128-byte line of H
+  constexpr int LG = 16 / CG;
+  constexpr int NL = NCG / LG;           // line groups (8)
+  static_assert(CG <= 16 && 16 % CG == 0 && NL == 8, "workgroup columns must tile a line of H");
   constexpr int SP = 65;
   __shared__ float2 tw[N];
-  __shared__ float2 xbuf[W][2 * R * SP];
-  __shared__ float2 nyqz[W][2 * kWaveChunk];
+  __shared__ float2 xbuf[W][CPW * R * SP];
+  __shared__ float2 nyqz[W][CPW * kWaveChunk];
   __shared__ ColTerm colT[kWaveChunk][CG + 1];
-  __shared__ float2 echiS[CG + 1][N];    // e^{i chi} of the 16 columns and of column N/2
+  __shared__ float2 echiS[CG + 1][N];    // e^{i chi} of the CG columns and of column N/2
   __shared__ df qxS[N], q2xS[N];
   const int tid = threadIdx.x;
   const int w = tid >> 6;
+  // qxS, q2xS and echiS in lane order: entry 64 r + l is frequency
+  // wave_qlane(l) + 64 r (entry N/2 is q = N/2 itself)
   for (int i = tid; i < N; i += 64 * W) {
     tw[i] = tw_g[i];
-    const df qx{gp.qx[i], gp.qxl[i]};
+    const int q = wave_qlane(i & 63) + (i & ~63);
+    const df qx{gp.qx[q], gp.qxl[q]};
     qxS[i] = qx;
     q2xS[i] = df_mul(qx, qx);
   }
@@ -729,10 +777,15 @@ k_trotter_wave(const float2* __restrict__ H, const BfInfo* __restrict__ info, Ge
   const int gpp = groups / parts;
   const long long items = (long long)gpp * NCG;
   for (long long item = blockIdx.x; item < items; item += gridDim.x) {
-    // workgroups go round-robin over the 8 XCDs; rotating the column group
-    // by the BF group spreads the heavier slot-0 items over all of them
-    const int gl = (int)(item / NCG);
-    const int cg = (int)((item + gl) % NCG);
+    // workgroups go round-robin over the 8 XCDs: items i, i + 8, ... run
+    // together on one XCD, so they take the LG column groups of one line of
+    // H (read once per L2); rotating the line by the BF group spreads the
+    // heavier slot-0 items over all XCDs
+    const long long k8 = item >> 3;
+    const int sl = (int)(k8 % LG);
+    const long long u = (k8 / LG) * 8 + (item & 7);
+    const int gl = (int)(u / NL);
+    const int cg = (int)((u + gl) % NL) * LG + sl;
     const int g = part * gpp + gl;
     int b0, b1;
     group_range(gl, gpp, hi - lo, b0, b1);
@@ -740,17 +793,17 @@ k_trotter_wave(const float2* __restrict__ H, const BfInfo* __restrict__ info, Ge
     b1 += lo;
     __syncthreads();
     for (int i = tid; i < (CG + 1) * N; i += 64 * W) {
-      const int c = i / N, q = i - (i / N) * N;
+      const int c = i / N, e = i - (i / N) * N;
       const int col = (c < CG) ? cg * CG + c : HALF;
-      echiS[c][q] = echi_g[(long long)col * N + q];
+      echiS[c][e] = echi_g[(long long)col * N + wave_qlane(e & 63) + (e & ~63)];
     }
     // (the first column-term table's barrier orders these writes)
     float2* sp = S_part + (long long)g * N * N;
     if (cg == 0 && w == 0)
-      wave_trotter_item<N, true>(H, gp, rowph_g, colph_g, echi_g, ap, rout2, tw, xbuf[w], nyqz[w], colT, echiS, qxS,
+      wave_trotter_item<N, CPW, NW, true>(H, gp, rowph_g, colph_g, echi_g, ap, rout2, tw, xbuf[w], nyqz[w], colT, echiS, qxS,
                                  q2xS, cg, w, tid, b0, b1, sp);
     else
-      wave_trotter_item<N, false>(H, gp, rowph_g, colph_g, echi_g, ap, rout2, tw, xbuf[w], nyqz[w], colT, echiS, qxS,
+      wave_trotter_item<N, CPW, NW, false>(H, gp, rowph_g, colph_g, echi_g, ap, rout2, tw, xbuf[w], nyqz[w], colT, echiS, qxS,
                                   q2xS, cg, w, tid, b0, b1, sp);
   }
 }
@@ -950,15 +1003,18 @@ hipError_t launch_phasors(int N, const BfInfo* info, int nbf_hint, const GeomPtr
 // trotter on the row-transformed stack (the reconstruction path); variant 0
 // is the default, the others are the tuner's candidates.  SG is even there:
 // each lane's 16-byte staging copy moves two adjacent slots of a row of H.
-// At N = 256 variant 0 is the wave-local kernel (the object path of a
-// reconstruction: row-transformed stack, isotropic sampling) and 1..3 are
-// the workgroup-tile shapes below; elsewhere the tile shapes only.
-int trotter_variants(int N) { return N == 1024 ? 1 : (N == 256 ? 4 : 3); }
+// At N = 256 variants 0 and 1 are the wave-local kernel (the object path of
+// a reconstruction: row-transformed stack, isotropic sampling) with
+// kWaveColumnsDefault columns per wave and with the other count, and 2..4
+// are the workgroup-tile shapes below; elsewhere the tile shapes only.
+int trotter_variants(int N) { return N == 1024 ? 1 : (N == 256 ? 5 : 3); }
+
+static int wave_columns(int variant) { return variant == 0 ? kWaveColumnsDefault : 3 - kWaveColumnsDefault; }
 
 int trotter_groups_max(int N) { return N == 256 ? kWaveTrotterGroups : object_groups(N); }
 
 bool trotter_wave_path(int N, bool from_h, const Optics& o, bool do_trotter, bool has_gt, int variant) {
-  return N == 256 && from_h && optics_isotropic(o) && do_trotter && !has_gt && variant == 0;
+  return N == 256 && from_h && optics_isotropic(o) && do_trotter && !has_gt && (variant == 0 || variant == 1);
 }
 
 hipError_t launch_trotter_slots(int N, bool from_h, const float2* src, const BfInfo* info, const GeomPtrs& gp,
@@ -972,15 +1028,23 @@ hipError_t launch_trotter_slots(int N, bool from_h, const float2* src, const BfI
   if (trotter_wave_path(N, from_h, o, do_trotter, Gt != nullptr, variant)) {
     constexpr int groups = kWaveTrotterGroups;
     if (parts < 1 || part < 0 || part >= parts || groups % parts) return hipErrorInvalidValue;
-    const unsigned grid = grid_items((long long)(groups / parts) * ((256 / 2) / (2 * kWaveTrotterWaves)), 65536);
-    hipLaunchKernelGGL((k_trotter_wave<256>), dim3(grid), dim3(64 * kWaveTrotterWaves), 0, s, src, info, gp, rowph,
-                       colph, echi, ap, rout2, tw, groups, S_part, part, parts);
+    // one column per wave in 4-wave workgroups (3 per CU at <= 168 VGPRs),
+    // or two per wave in 8-wave workgroups (one per CU)
+    const int cpw = wave_columns(variant);
+    const int nw = cpw == 1 ? 4 : 8;
+    const unsigned grid = grid_items((long long)(groups / parts) * ((256 / 2) / (cpw * nw)), 65536);
+    if (cpw == 1)
+      hipLaunchKernelGGL((k_trotter_wave<256, 1, 4, 3>), dim3(grid), dim3(64 * 4), 0, s, src, info, gp, rowph, colph,
+                         echi, ap, rout2, tw, groups, S_part, part, parts);
+    else
+      hipLaunchKernelGGL((k_trotter_wave<256, 2, 8, 2>), dim3(grid), dim3(64 * 8), 0, s, src, info, gp, rowph, colph,
+                         echi, ap, rout2, tw, groups, S_part, part, parts);
     if (groups_used) *groups_used = groups;
     return hipGetLastError();
   }
   if (parts != 1) return hipErrorInvalidValue;
-  // workgroup-tile shapes: at N = 256 their variants follow the wave-local one
-  if (N == 256 && variant > 0) variant -= 1;
+  // workgroup-tile shapes: at N = 256 their variants follow the wave-local ones
+  if (N == 256 && variant > 0) variant = variant >= 2 ? variant - 2 : 0;
   const int groups = object_groups(N);
   if (groups_used) *groups_used = groups;
 #define QSSB_SLOTS_ONE(NN, SGV, BL, FH, IS)                                                                 \

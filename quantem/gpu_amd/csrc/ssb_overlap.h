@@ -45,6 +45,45 @@ __device__ __forceinline__ float aperture_sum(const df& a, const df& b, const Ap
   return aperture_from_d(df_add(a, b), ap);
 }
 
+// The aperture from a float32 estimate of d when it lies beyond the 1e-4
+// margin of the soft band (the value aperture_sum gives there); false when
+// the estimate is too close to the band to decide.
+__device__ __forceinline__ bool aperture_decided(float est, const ApIso& ap, float& a) {
+  constexpr float kMargin = 1e-4f;
+  a = (est >= ap.d_in + kMargin) ? 1.0f : 0.0f;
+  return est >= ap.d_in + kMargin || est <= ap.d_out - kMargin;
+}
+
+// The value unchanged, through an empty volatile asm: an expression built on
+// it cannot be hoisted out of the branch that holds it.
+__device__ __forceinline__ df df_in_branch(df x) {
+  asm volatile("" : "+v"(x.hi), "+v"(x.lo));
+  return x;
+}
+
+// Float32 estimates of the isotropic row terms of d(Q -/+ k),
+// -qx^2 + 2 qx kx and -qx^2 - 2 qx kx (kx2 = 2 kx): within ~1e-5 of the
+// double-float values for terms below 32 in magnitude, well inside the 1e-4
+// margin that aperture_decided leaves.
+__device__ __forceinline__ float row_estimate_minus(df qx, df kx2, df q2x) { return fmaf(qx.hi, kx2.hi, -q2x.hi); }
+__device__ __forceinline__ float row_estimate_plus(df qx, df kx2, df q2x) { return -fmaf(qx.hi, kx2.hi, q2x.hi); }
+
+// Isotropic aperture pair A(Q - k), A(Q + k) from d = row + col: decided
+// from the estimates uq, vq of the row terms wherever d lies beyond the soft
+// band's margin (exactly the value aperture_sum gives), otherwise from the
+// double-float row terms, which are formed inside that branch only -- the
+// few points near the soft edge pay for them, the rest do not.
+__device__ __forceinline__ void iso_aperture_pair(float uq, float vq, df qx, df kx2, df q2x, const df& colA,
+                                                  const df& colB, const ApIso& ap, float& am, float& app) {
+  const bool da = aperture_decided(uq + colA.hi, ap, am);
+  const bool dp = aperture_decided(vq + colB.hi, ap, app);
+  if (!(da && dp)) {
+    const df u = df_mul(df_in_branch(qx), kx2);
+    if (!da) am = aperture_from_d(df_add(df_sub(u, q2x), colA), ap);
+    if (!dp) app = aperture_from_d(df_add(df_neg(df_add(q2x, u)), colB), ap);
+  }
+}
+
 // Aperture pair A(Q - k), A(Q + k).  ISO: from d = (Rout^2 - |k|^2) - |Q|^2
 // +/- 2 Q.k; otherwise the general double-float soft-edge expression.
 template <bool ISO>

@@ -13,7 +13,7 @@ def test_candidates_start_with_the_default(n):
     pairs = tuner.candidates(n, "gather_pairs")
     assert pairs[0] == {128: 16, 256: 8, 512: 8, 1024: 4}[n] and len(pairs) == 2
     # at 256 the wave-local kernel comes first, then the three tile shapes
-    assert tuner.candidates(n, "trotter") == list(range(1 if n == 1024 else (4 if n == 256 else 3)))
+    assert tuner.candidates(n, "trotter") == list(range(1 if n == 1024 else (5 if n == 256 else 3)))
 
 
 def test_candidates_reject_unsupported_scan_sizes():
