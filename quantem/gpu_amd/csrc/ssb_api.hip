@@ -895,9 +895,6 @@ int ssb_batch_phase_loss_graph(ssb_plan* p, int n_trials, const double* mags_hos
     HIP_TRY(hipStreamSynchronize(s), "batch graph sync");
     if (p->gstream) HIP_TRY(hipStreamSynchronize(p->gstream), "batch graph sync");
     dfree(p->xt);
-  dfree(p->sup_part);
-  dfree(p->sup_box);
-  dfree(p->sup_C);
     p->xt = nullptr;
     p->xt_cap = 0;
     HIP_TRY(dmalloc(&p->xt, (size_t)n_trials), "batch table allocation");
@@ -1082,6 +1079,9 @@ static int recon_typed(ssb_plan* p, const ssb_plan* after, const void* data, int
                                  p->tw, p->S_part, nullptr, true, p->tune_trotter, s, &groups, 1, 2),
             "colfft_trotter");
   } else {
+    // (geometry and phasors on a second stream beside the gather measured
+    // slower: the cross-stream event pair costs more than the ~9 us they
+    // would hide)
     HIP_TRY(launch_gather_rowfft(p->n, dtype, data, p->D, p->det_cols, p->bf_rows, p->bf_cols, p->info,
                                  p->nbf_hint, p->tw, p->tune_gather_pairs, p->gain, p->H, s,
                                  sup ? p->sup_box : nullptr, sup ? p->sup_C : nullptr),

@@ -47,3 +47,31 @@ def test_superset_copy_gives_the_frame_gather_bits(cuda, case):
     torch.cuda.synchronize()
     assert torch.equal(out[1][0], out[2][0]) and torch.equal(out[1][1], out[2][1])
     assert eng.tune_get(SUPERSET_KNOB) == 2
+
+
+def test_superset_buffers_survive_a_growing_graph_batch(cuda):
+    # the hipGraph batch reallocates only its own trial table when a larger
+    # batch arrives; the superset buffers of the plan stay allocated and the
+    # next reconstruction gives the same bits
+    torch = cuda
+    cfg, data = synthetic(128, 64, {"C10": 150.0}, rotation_deg=2.0)
+    eng = engine_for_data(torch, data, semiangle_mrad=20.0, det_sampling=1.25, step=cfg.step_A)
+    eng.tune_set(SUPERSET_KNOB, 1)
+    d = torch.from_numpy(data).cuda()
+    n = data.shape[0]
+
+    def recon():
+        ph = torch.empty((n, n), dtype=torch.float32, device="cuda")
+        eng.recon_u16(d, 150.0, 0.0, 0.0, 0.0, 0.0, None, phase_out=ph)
+        return ph
+
+    first = recon()
+    eng.prepare_g(d)
+    for t in (2, 5):
+        mags = np.zeros((t, 14))
+        mags[:, 0] = np.linspace(100.0, 200.0, t)
+        losses = eng.batch_phase_loss(mags, np.zeros((t, 14)), graph=True)
+        assert torch.isfinite(losses).all()
+    second = recon()
+    torch.cuda.synchronize()
+    assert torch.equal(first, second)
