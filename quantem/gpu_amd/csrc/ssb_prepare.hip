@@ -351,6 +351,32 @@ __device__ __forceinline__ long long shfl_xor_i64(long long v, int off) {
   return ((long long)hi << 32) | (unsigned int)lo;
 }
 
+// three int32 sums at once (exact: the caller's totals fit in 32 bits)
+__device__ void block_sum3_i32(int& a, int& b, int& c, int* sh) {
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off, 64);
+    b += __shfl_xor(b, off, 64);
+    c += __shfl_xor(c, off, 64);
+  }
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) {
+    sh[w] = a;
+    sh[MASK_WAVES + w] = b;
+    sh[2 * MASK_WAVES + w] = c;
+  }
+  __syncthreads();
+  int ta = 0, tb = 0, tc = 0;
+  for (int k = 0; k < MASK_WAVES; ++k) {
+    ta += sh[k];
+    tb += sh[MASK_WAVES + k];
+    tc += sh[2 * MASK_WAVES + k];
+  }
+  __syncthreads();
+  a = ta;
+  b = tb;
+  c = tc;
+}
+
 // three int64 sums at once
 __device__ void block_sum3_i64(long long& a, long long& b, long long& c, long long* sh) {
   for (int off = 32; off > 0; off >>= 1) {
@@ -398,6 +424,8 @@ __device__ __forceinline__ float count_over_pi(float n) {
 }
 
 // CACHE: the mean DP (<= 16384 pixels) is staged in LDS once for the passes.
+// The passes over a thread's run are unrolled by 8, so their LDS reads are
+// issued together rather than one latency per pixel.
 #define SELECT_CACHE_PIX 16384
 template <bool CACHE>
 __global__ void __launch_bounds__(MASK_THREADS)
@@ -409,6 +437,7 @@ k_select_bf(const float* __restrict__ mean_g, const unsigned long long* __restri
   const SupBox box = boxp ? *boxp : SupBox{0, 0, 0, 0};
   __shared__ float sh_f[2 * MASK_WAVES];
   __shared__ long long sh_i[3 * MASK_WAVES];
+  __shared__ int sh_i32[3 * MASK_WAVES];
   __shared__ int sh_scan[MASK_WAVES];
   // the LDS copy holds one pad word after each thread's run of `per`
   // pixels, so the lanes of a wave, reading pixel p0 + j of their runs
@@ -420,21 +449,25 @@ k_select_bf(const float* __restrict__ mean_g, const unsigned long long* __restri
   // each thread owns the contiguous pixels [p0, p0 + per)
   const int per = (D + MASK_THREADS - 1) / MASK_THREADS;   // <= MASK_PER
   if (CACHE) {
+    // the padded slot of pixel i is i + i / per; q = i / per by a multiply
+    // and shift, exact for i < 2^16 and per <= 64
+    const unsigned magic = ((1u << 22) + (unsigned)per - 1u) / (unsigned)per;
 #pragma unroll 16
-    for (int i = tid; i < D; i += MASK_THREADS) smean[i + i / per] = mean_g[i];
+    for (int i = tid; i < D; i += MASK_THREADS) smean[i + (int)(((unsigned)i * magic) >> 22)] = mean_g[i];
     __syncthreads();
   }
   // owner-only view: mean[i] for i in [p0, p1) is pixel i
   const float* mean = CACHE ? smean + tid : mean_g;
+#define MV(i) mean[i]
   const int p0 = tid * per;
   const int p1 = min(D, p0 + per);
-#define FOR_PIX for (int i = p0; i < p1; ++i)
+#define FOR_PIX _Pragma("unroll 8") for (int i = p0; i < p1; ++i)
   // (row, col) of pixel i, advanced incrementally from p0 (no divisions)
   const int r_start = p0 / det_cols, c_start = p0 - (p0 / det_cols) * det_cols;
 #define FOR_PIX_RC \
-  for (int i = p0, r_ = r_start, c_ = c_start; i < p1; ++i, (++c_ == det_cols) ? (c_ = 0, ++r_) : 0)
+  _Pragma("unroll 8") for (int i = p0, r_ = r_start, c_ = c_start; i < p1; ++i, (++c_ == det_cols) ? (c_ = 0, ++r_) : 0)
   float lmax = -INFINITY;
-  FOR_PIX lmax = fmaxf(lmax, mean[i]);
+  FOR_PIX lmax = fmaxf(lmax, MV(i));
   const float mmax = block_max(lmax, sh_f);
   const float cut = mmax * threshold;
 
@@ -442,27 +475,28 @@ k_select_bf(const float* __restrict__ mean_g, const unsigned long long* __restri
   if (bf_radius <= 0.0f) {
     Kahan k1;
     k1.init();
-    FOR_PIX k1.add(mean[i]);
+    FOR_PIX k1.add(MV(i));
     const float total = block_sum_comp(k1.s, -k1.c, sh_f);
     const float mu = total / (float)D;
     Kahan k2;
     k2.init();
     FOR_PIX {
-      const float d = mean[i] - mu;
+      const float d = MV(i) - mu;
       k2.add(d * d);
     }
     const float ss = block_sum_comp(k2.s, -k2.c, sh_f);
     const float sd = sqrtf(ss / (float)D);
     const float thr = mu + sd;
-    long long cnt = 0, sr = 0, sc = 0;
+    // (counts and row / column sums below 2^31 for D <= 65536)
+    int cnt = 0, sr = 0, sc = 0;
     FOR_PIX_RC {
-      if (mean[i] > thr) {
+      if (MV(i) > thr) {
         cnt += 1;
         sr += r_;
         sc += c_;
       }
     }
-    block_sum3_i64(cnt, sr, sc, sh_i);
+    block_sum3_i32(cnt, sr, sc, sh_i32);
     if (cnt > 0) {
       center_r = (float)sr / (float)cnt;
       center_c = (float)sc / (float)cnt;
@@ -480,7 +514,7 @@ k_select_bf(const float* __restrict__ mean_g, const unsigned long long* __restri
     kc.init();
     long long n = 0, sr = 0, sc = 0;
     FOR_PIX_RC {
-      const float wv = mean[i];
+      const float wv = MV(i);
       if (wv > cut) {
         const int r = r_, c = c_;
         kw.add(wv);
@@ -516,7 +550,7 @@ k_select_bf(const float* __restrict__ mean_g, const unsigned long long* __restri
   Kahan kd;
   kd.init();
   FOR_PIX_RC {
-    if (mean[i] > cut) {
+    if (MV(i) > cut) {
       any_cand += 1;
       if (dist2_f32((float)r_, (float)c_, center_r, center_c) <= r2) {
         mask_bits |= 1ull << (i - p0);
@@ -579,6 +613,7 @@ k_select_bf(const float* __restrict__ mean_g, const unsigned long long* __restri
   }
 #undef FOR_PIX
 #undef FOR_PIX_RC
+#undef MV
   if (tid == 0) {
     int status = 0;
     if (any_cand == 0) status = QSSB_DEV_NO_CANDIDATES;
