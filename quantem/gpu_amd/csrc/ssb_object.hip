@@ -386,6 +386,11 @@ constexpr int kWaveChunk = 64;           // BF pixels per column-term table / Ny
 
 __device__ __forceinline__ void wave_lds_fence() { asm volatile("" ::: "memory"); }
 
+// Workgroup barrier for LDS only: this wave's LDS operations complete, then
+// s_barrier.  Unlike __syncthreads it does not drain the wave's global loads,
+// so prefetches of H stay in flight across it (their uses wait for them).
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 
 // Column terms of d(Q -/+ k) for one (BF pixel, column): (Rout^2 - |k|^2) -
 // qy^2 +/- 2 qy ky (isotropic sampling).
@@ -512,7 +517,7 @@ __device__ __forceinline__ void wave_trotter_item(const float2* __restrict__ H, 
     const int n = min(kWaveChunk, b1 - c0);
     // column terms of the chunk's BF pixels at the workgroup's CG columns
     // and column N/2, computed once for all waves
-    __syncthreads();
+    lds_barrier();
     for (int e = tid; e < n * (CG + 1); e += 64 * NW) {
       const int bi = e / (CG + 1), ci = e - (e / (CG + 1)) * (CG + 1);
       const int b = c0 + bi;
@@ -520,7 +525,7 @@ __device__ __forceinline__ void wave_trotter_item(const float2* __restrict__ H, 
       colT[bi][ci] = col_term(rout2, df{gp.kx[b], gp.kxl[b]}, df{gp.ky[b], gp.kyl[b]},
                               df{gp.qy[col], gp.qyl[col]});
     }
-    __syncthreads();
+    lds_barrier();
     for (int b = c0; b < c0 + n; ++b) {
       WaveBf<R, CPW> cu;
       if constexpr (BF_AHEAD) {
@@ -792,10 +797,24 @@ k_trotter_wave(const float2* __restrict__ H, const BfInfo* __restrict__ info, Ge
     b0 += lo;
     b1 += lo;
     __syncthreads();
-    for (int i = tid; i < (CG + 1) * N; i += 64 * W) {
-      const int c = i / N, e = i - (i / N) * N;
-      const int col = (c < CG) ? cg * CG + c : HALF;
-      echiS[c][e] = echi_g[(long long)col * N + wave_qlane(e & 63) + (e & ~63)];
+    {
+      // all of a thread's loads are issued before its LDS stores
+      constexpr int FILL = ((CG + 1) * N + 64 * W - 1) / (64 * W);
+      float2 ev[FILL];
+#pragma unroll
+      for (int k = 0; k < FILL; ++k) {
+        const int i = tid + k * 64 * W;
+        if (i < (CG + 1) * N) {
+          const int c = i / N, e = i - (i / N) * N;
+          const int col = (c < CG) ? cg * CG + c : HALF;
+          ev[k] = echi_g[(long long)col * N + wave_qlane(e & 63) + (e & ~63)];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < FILL; ++k) {
+        const int i = tid + k * 64 * W;
+        if (i < (CG + 1) * N) echiS[i / N][i - (i / N) * N] = ev[k];
+      }
     }
     // (the first column-term table's barrier orders these writes)
     float2* sp = S_part + (long long)g * N * N;
