@@ -782,15 +782,19 @@ k_trotter_wave(const float2* __restrict__ H, const BfInfo* __restrict__ info, Ge
   const int gpp = groups / parts;
   const long long items = (long long)gpp * NCG;
   for (long long item = blockIdx.x; item < items; item += gridDim.x) {
-    // workgroups go round-robin over the 8 XCDs: items i, i + 8, ... run
-    // together on one XCD, so they take the LG column groups of one line of
-    // H (read once per L2); rotating the line by the BF group spreads the
-    // heavier slot-0 items over all XCDs
+    // Workgroups go round-robin over the 8 XCDs and, within one, start in
+    // order as CUs free up.  XCD x takes the BF groups x, x + 8, ...; its
+    // items run line by line of H in increasing slot order -- the low slot
+    // columns hold more points inside the apertures, so the long items start
+    // first and the short ones fill in at the end -- and the LG items that
+    // share a line of H are consecutive (read once per L2).
+    const int xcd = (int)(item & 7);
     const long long k8 = item >> 3;
-    const int sl = (int)(k8 % LG);
-    const long long u = (k8 / LG) * 8 + (item & 7);
-    const int gl = (int)(u / NL);
-    const int cg = (int)((u + gl) % NL) * LG + sl;
+    const int gx = gpp / 8;                // BF groups per XCD
+    const int line = (int)(k8 / (gx * LG));
+    const int rem = (int)(k8 % (gx * LG));
+    const int gl = (rem / LG) * 8 + xcd;
+    const int cg = line * LG + rem % LG;
     const int g = part * gpp + gl;
     int b0, b1;
     group_range(gl, gpp, hi - lo, b0, b1);
@@ -1046,7 +1050,8 @@ hipError_t launch_trotter_slots(int N, bool from_h, const float2* src, const BfI
   const bool iso = optics_isotropic(o);
   if (trotter_wave_path(N, from_h, o, do_trotter, Gt != nullptr, variant)) {
     constexpr int groups = kWaveTrotterGroups;
-    if (parts < 1 || part < 0 || part >= parts || groups % parts) return hipErrorInvalidValue;
+    // (the item order gives each XCD groups / parts / 8 BF groups)
+    if (parts < 1 || part < 0 || part >= parts || groups % (8 * parts)) return hipErrorInvalidValue;
     // one column per wave in 4-wave workgroups (3 per CU at <= 168 VGPRs),
     // or two per wave in 8-wave workgroups (one per CU)
     const int cpw = wave_columns(variant);
