@@ -116,13 +116,26 @@ __device__ __forceinline__ float aperture_quick(float dapprox, const ApIso& a) {
   return -1.0f;
 }
 
+// The persistent workgroups take items from a counter in device memory
+// (queue[0]) until it passes total_items, so the long items (low columns,
+// more points inside the apertures) and the short ones share the CUs evenly;
+// the last workgroup to leave (queue[1]) sets both counters back to zero for
+// the next launch, so a captured graph replays the kernel as it is.
+// Two workgroups per CU (2 waves per SIMD, 256 registers with 2 spilled):
+// unbounded, the kernel took 258 and ran one workgroup per CU at half the rate.
 template <int TB>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 2)
 k_trotter_trials(const TrialSet* __restrict__ sets, int n_sets, long long total_items,
-                 const TrialChi* __restrict__ chis) {
+                 const TrialChi* __restrict__ chis, unsigned* __restrict__ queue) {
   __shared__ TrialChi tch[TB];
+  __shared__ unsigned s_item;
   const int tid = threadIdx.x;
-  for (long long item = blockIdx.x; item < total_items; item += gridDim.x) {
+  for (;;) {
+    __syncthreads();   // the previous item's readers of tch and s_item are done
+    if (tid == 0) s_item = atomicAdd(&queue[0], 1u);
+    __syncthreads();
+    const long long item = s_item;
+    if (item >= total_items) break;
     int d = 0;
     while (d + 1 < n_sets && item >= sets[d + 1].item0) ++d;
     const TrialSet ds = sets[d];
@@ -131,10 +144,11 @@ k_trotter_trials(const TrialSet* __restrict__ sets, int n_sets, long long total_
     const int NP = trial_points(N);      // plus the Nyquist row's other half
     const int pblocks = (NP + kTrialPoints - 1) / kTrialPoints;
     const long long loc = item - ds.item0;
-    const int tb = (int)(loc / pblocks), pb = (int)(loc - (long long)tb * pblocks);
+    // point-block major: the low columns (the long items) are taken first
+    const int tblocks = (ds.T + TB - 1) / TB;
+    const int pb = (int)(loc / tblocks), tb = (int)(loc - (long long)pb * tblocks);
     const int t0 = tb * TB;
     const int nt = min(TB, ds.T - t0);
-    __syncthreads();   // the previous item's readers of tch are done
     {
       constexpr int W = (int)(sizeof(TrialChi) / 4);
       const int* src = reinterpret_cast<const int*>(chis + t0);
@@ -196,6 +210,13 @@ k_trotter_trials(const TrialSet* __restrict__ sets, int n_sets, long long total_
 #pragma unroll
       for (int i = 0; i < TB; ++i)
         if (i < nt) ds.S[(long long)(t0 + i) * NP + p] = make_float2(ar[i].s - ar[i].c, ai[i].s - ai[i].c);
+    }
+  }
+  if (tid == 0) {
+    __threadfence();
+    if (atomicAdd(&queue[1], 1u) == gridDim.x - 1) {
+      atomicExch(&queue[0], 0u);
+      atomicExch(&queue[1], 0u);
     }
   }
 }
@@ -377,6 +398,7 @@ struct ssb_trials {
   hipGraphExec_t exec = nullptr;
   int graph_trials = 0;                  // n_trials the graph was captured for
   float* graph_out = nullptr;
+  unsigned* queue = nullptr;             // work-queue counters of k_trotter_trials (zero between launches)
 };
 
 namespace {
@@ -389,6 +411,7 @@ void trials_free(ssb_trials* tr) {
   if (tr->ev_out) (void)hipEventDestroy(tr->ev_out);
   if (tr->stream) (void)hipStreamDestroy(tr->stream);
   (void)hipFree(tr->chis);
+  (void)hipFree(tr->queue);
   (void)hipFree(tr->sets);
   for (auto* p : tr->tbf) (void)hipFree(p);
   for (auto* p : tr->S) (void)hipFree(p);
@@ -421,7 +444,7 @@ hipError_t enqueue_trials(ssb_trials* tr, float* out, hipStream_t s) {
   }
   const long long grid = total_items < 2048 ? total_items : 2048;   // persistent: 8 per CU
   hipLaunchKernelGGL((k_trotter_trials<kTrialBlock>), dim3((unsigned)grid), dim3(256), 0, s, tr->sets,
-                     (int)tr->sets_host.size(), total_items, tr->chis);
+                     (int)tr->sets_host.size(), total_items, tr->chis, tr->queue);
   for (size_t i = 0; i < tr->plans.size(); ++i) {
     ssb_plan* p = tr->plans[i];
     hipError_t e = launch_trials_finish(p->n, T, tr->S[i], p->info, p->tw, tr->W[i], tr->part[i],
@@ -455,6 +478,8 @@ int ssb_trials_create(ssb_plan* const* plans, int n_plans, int max_trials, ssb_t
   hipError_t e = hipGetDevice(&tr->device);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&tr->chis), sizeof(TrialChi) * max_trials);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&tr->sets), sizeof(TrialSet) * n_plans);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&tr->queue), 2 * sizeof(unsigned));
+  if (e == hipSuccess) e = hipMemset(tr->queue, 0, 2 * sizeof(unsigned));
   for (int i = 0; i < n_plans && e == hipSuccess; ++i) {
     ssb_plan* p = plans[i];
     const long long NP = trial_points(p->n);
