@@ -121,10 +121,14 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
     // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs,
     // so items i, i+8, ..., i+8(LG-1) run together on one XCD; give them the
     // LG slot groups that share each 128-byte line of H (read once per L2).
+    // From a resident stack, slot groups in increasing order over the BF
+    // groups: the low slot columns (more points inside the apertures, the
+    // long items) start first (measured faster there; from H the BF-group
+    // order is kept: the other order measured slower at scan 128).
     const long long sl = (item >> 3) % LG;
     const long long t = (item / (8 * LG)) * 8 + (item & 7);
-    const int g = (int)(t / (NSG / LG));
-    const int sg = (int)((t % (NSG / LG)) * LG + sl);
+    const int g = FROM_H ? (int)(t / (NSG / LG)) : (int)(t % groups);
+    const int sg = FROM_H ? (int)((t % (NSG / LG)) * LG + sl) : (int)((t / groups) * LG + sl);
     int b0, b1;
     group_range(g, groups, nbf, b0, b1);
     // copy the source tile of BF pixel b into landing buffer `slot`
@@ -1069,7 +1073,9 @@ hipError_t launch_trotter_slots(int N, bool from_h, const float2* src, const BfI
   if (parts != 1) return hipErrorInvalidValue;
   // workgroup-tile shapes: at N = 256 their variants follow the wave-local ones
   if (N == 256 && variant > 0) variant = variant >= 2 ? variant - 2 : 0;
-  const int groups = object_groups(N);
+  // a resident stack at N = 256: 32 BF groups (2048 items, two rounds of four
+  // workgroups per CU, so the heavy-first order can even out the CUs)
+  const int groups = (!from_h && N == 256) ? 2 * object_groups(N) : object_groups(N);
   if (groups_used) *groups_used = groups;
 #define QSSB_SLOTS_ONE(NN, SGV, BL, FH, IS)                                                                 \
   hipLaunchKernelGGL((k_trotter_slots<NN, SGV, BL, FH, IS>), dim3(grid), dim3(BL), 0, s, src, info, gp, rowph, \
