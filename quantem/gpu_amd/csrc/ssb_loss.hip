@@ -369,7 +369,9 @@ k_loss_final(const float* __restrict__ var_part, int nparts, int NN, float* __re
 // Launch wrappers
 // ------------------------------------------------------------------------
 int loss_chunk_bf(int N) { return N == 128 ? 512 : (N == 256 ? 256 : (N == 512 ? 32 : 8)); }
-int loss_splits(int N) { return N == 128 ? 32 : (N == 256 ? 8 : (N == 512 ? 2 : 1)); }
+// BF splits of the row pass (partial moment planes): 16 at N = 256 keeps
+// 2048 workgroups in flight (8 measured 2.6% slower per trial)
+int loss_splits(int N) { return N == 128 ? 32 : (N == 256 ? 16 : (N == 512 ? 2 : 1)); }
 
 hipError_t launch_phase_moments(int N, const float2* Gt, int gcols, const BfInfo* info, int nbf_hint, const GeomPtrs& gp,
                                 const float2* rowph, const float2* colph, const float2* echi, const ChiPoly* poly,
