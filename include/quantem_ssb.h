@@ -353,7 +353,9 @@ int ssb_trials_set(ssb_trials* trials, int n_trials, const double* mags_host, co
  * use_graph = 1 captures them once into a hipGraph (again when n_trials or
  * `out` change) and replays it on the handle's own stream, ordered after
  * and before `stream`: new trials need only ssb_trials_set.  Both give the
- * same bits.
+ * same bits.  The launch takes its work items from a counter owned by the
+ * handle (zero again when the launch ends), so calls on one handle must not
+ * overlap in time; separate handles may run concurrently.
  */
 int ssb_trials_phase_variance(ssb_trials* trials, float* out, int use_graph, void* stream);
 
