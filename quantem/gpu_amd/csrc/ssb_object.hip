@@ -361,16 +361,17 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
 
 // ------------------------------------------------------------------------
 // Wave-local fused column FFT + trotter of the reconstruction (row-transformed
-// stack H, isotropic detector sampling).  A workgroup of 8 waves takes one
-// BF group and 16 adjacent slot columns -- the 128 bytes of every row of H
-// that one cache line holds -- and each wave owns 2 of those columns for
-// every BF pixel of the group.  Lane l holds rows l + 64 m (m < R = N / 64):
-// one 16-byte load per row brings both columns.  The column FFT needs no
-// workgroup barrier: an R-point DFT in registers (stride-64 elements) with
-// twiddles w_N^(l k1), then 64-point Stockham radix-4 stages over the 2R
-// sequences through the wave's own LDS (every lane reads its operands before
-// any lane writes), so X[k1 + R k2] = Z_k1[k2] and lane l reads back its
-// frequencies q = l + 64 m.
+// stack H, isotropic detector sampling).  A workgroup of NW waves takes one
+// BF group and NW * CPW adjacent slot columns (8 waves x 2 columns: the 128
+// bytes of every row of H that one cache line holds), and each wave owns CPW
+// of those columns for every BF pixel of the group.  Lane l holds rows
+// l + 64 m (m < R = N / 64): one 16- or 8-byte load per row brings the
+// wave's columns.  The column FFT needs no workgroup barrier: an R-point DFT
+// in registers (stride-64 elements) with twiddles w_N^(l k1), then 64-point
+// Stockham radix-4 stages over the CPW R sequences through the wave's own
+// LDS (every lane reads its operands before any lane writes), so
+// X[k1 + R k2] = Z_k1[k2]; the last stage's outputs stay in the registers of
+// the lane that computed them, lane l holding q = wave_qlane(l) + 64 r.
 // Per point the trotter term is that of k_trotter_slots (same double-float
 // apertures, phasors and guard), with Kahan FP32 sums over the group's BF
 // pixels.  The aperture of a point far from the soft edge is decided from
@@ -379,10 +380,11 @@ k_trotter_slots(const float2* __restrict__ src, const BfInfo* __restrict__ info,
 // double-float.  Outputs B of the mirror points are written as -conj(A);
 // column N/2 (slot 0) is accumulated with A; the Nyquist-row outputs
 // S(N/2, N - j) are summed after the BF loop, one BF pixel per lane, from
-// the G_b(N/2, j) the loop leaves in LDS.  The column terms of d for each
-// (BF pixel, column) are computed once per workgroup into LDS, 64 BF pixels
-// at a time; H and the phasors of the next BF pixel are loaded while this
-// one is transformed.
+// the G_b(N/2, j) lane 0 holds.  The column terms of d for each (BF pixel,
+// column) are computed once per workgroup into LDS, 64 BF pixels at a time;
+// H and the phasors of the next BF pixel are loaded while this one is
+// transformed.  Items (BF group, column group) start heavy-first on each
+// XCD: the low slot columns hold more points inside the apertures.
 // ------------------------------------------------------------------------
 constexpr int kWaveColumnsDefault = 2;   // slot columns per wave of the default variant
 constexpr int kWaveTrotterGroups = 64;
