@@ -17,13 +17,16 @@
 //   k_bslz4_index   one thread per chunk: validates the header and walks the
 //                   block sizes into a descriptor table (bounds-checked);
 //   k_bslz4_decode  one 64-lane workgroup per block: stages the compressed
-//                   block in LDS, decodes it in LDS (every lane parses the
-//                   same sequence headers; literal and match bytes are
-//                   copied by all lanes, a match of offset d < length
-//                   copied in one pass as out[o + t] = out[o - d + t mod d]),
-//                   then inverts the bit-shuffle, 8 elements per lane step
-//                   (two 8x8 bit-matrix transposes by delta swaps and byte
-//                   permutes), with 16-byte stores.
+//                   block in LDS as whole words, decodes it in LDS in rounds
+//                   of up to 64 sequences (a uniform parse of the round's
+//                   headers into one lane per sequence, then all of the
+//                   round's literal runs at once and its matches in order;
+//                   a match of offset d < length is one pass of
+//                   out[o + t] = out[o - d + t mod d], a run of one byte is
+//                   written with 16-byte stores), then inverts the
+//                   bit-shuffle, 32 elements per lane step (two 8x8
+//                   bit-matrix transposes by delta swaps and byte permutes
+//                   per 8 elements), with 16-byte stores.
 // Malformed input never reads or writes out of bounds: it sets a bit in the
 // status word and the block is skipped.
 #include <cstring>
@@ -37,9 +40,10 @@ namespace {
 
 constexpr int kMaxBlockBytes = 8192;                                  // 4096 uint16 elements
 constexpr int kMaxCompBytes = kMaxBlockBytes + kMaxBlockBytes / 255 + 16;  // LZ4 worst case
-// Compressed blocks up to this size are staged in LDS (detector frames
-// compress 10-20x); larger ones are parsed from global memory.  The small
-// staging buffer leaves room for about twice as many workgroups per CU.
+// Compressed blocks up to this size (less the alignment bytes around them)
+// are staged in LDS (detector frames compress 10-20x); larger ones are
+// parsed from global memory.  The small staging buffer leaves room for
+// about twice as many workgroups per CU.
 constexpr int kStageBytes = 2048;
 
 struct BlockDesc {
@@ -122,15 +126,217 @@ k_bslz4_index(const uint8_t* __restrict__ comp, long long comp_bytes, const long
   }
 }
 
+// Sequence table entries per block: at most block_elems / 16 (256 for the
+// default 8192-byte blocks: an average of 32 or more decoded bytes per
+// sequence), and at least 64.  A block with more sequences is parsed by the
+// decode kernel itself.
+__host__ __device__ __forceinline__ int seq_cap(int block_elems) { return block_elems / 16 > 64 ? block_elems / 16 : 64; }
+
+// Entry k of block d's sequence table: blocks in groups of 64, entry-major
+// inside a group, so that the 64 lanes of a parse wave store one entry each
+// to 512 contiguous bytes.
+__device__ __forceinline__ long long seq_slot(long long d, int k, int cap) {
+  return ((d >> 6) * (long long)cap + k) * 64 + (d & 63);
+}
+
+// One thread per block: walks the LZ4 sequence headers of its block (every
+// length and offset checked as in the decode) and stores sequence k as
+// {literal start | literal length << 16, offset | match length << 16}
+// (offset and match length 0 for the block's last sequence).  nseq[d] is
+// the count, 0 for unused slots and raw tails, -1 when the block has more
+// sequences than the table holds (the decode kernel then parses it), -2
+// for a malformed block (its status bit set).  Each thread reads its block
+// through a 16-byte window of aligned vector loads.
+__global__ void __launch_bounds__(256)
+k_bslz4_parse(const uint8_t* __restrict__ comp, long long comp_bytes, const BlockDesc* __restrict__ table,
+              long long n_desc, int cap, uint2* __restrict__ seqs, int* __restrict__ nseq, int* __restrict__ status) {
+  const long long d = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (d >= n_desc) return;
+  const BlockDesc bd = table[d];
+  if (bd.elems == 0 || bd.csize < 0) {
+    nseq[d] = 0;
+    return;
+  }
+  const int cs = bd.csize;
+  const int osz = bd.elems * 2;
+  const uint8_t* const g = comp + bd.src;
+  const uint8_t* const lim = comp + comp_bytes;
+  int wb = -64;                                       // the window holds g[wb, wb + 16); g + wb 16-byte aligned
+  unsigned w0 = 0u, w1 = 0u, w2 = 0u, w3 = 0u;        // (separate words: a vector indexed by k would go to memory)
+  auto byte = [&](int p) -> int {                     // 0 <= p < cs
+    if ((unsigned)(p - wb) >= 16u) {
+      const uint8_t* a = reinterpret_cast<const uint8_t*>(reinterpret_cast<uintptr_t>(g + p) & ~(uintptr_t)15);
+      wb = (int)(a - g);
+      if (a + 16 <= lim) {
+        const uint4 v = *reinterpret_cast<const uint4*>(a);
+        w0 = v.x;
+        w1 = v.y;
+        w2 = v.z;
+        w3 = v.w;
+      } else {                                        // the buffer's last bytes: only those inside it
+        auto word = [&](int o) {
+          unsigned v = 0u;
+          for (int i = 0; i < 4; ++i)
+            if (a + o + i < lim) v |= (unsigned)a[o + i] << (8 * i);
+          return v;
+        };
+        w0 = word(0);
+        w1 = word(4);
+        w2 = word(8);
+        w3 = word(12);
+      }
+    }
+    const int k = p - wb;
+    const unsigned lo = (k & 4) ? w1 : w0, hi = (k & 4) ? w3 : w2;
+    return (int)((((k & 8) ? hi : lo) >> (8 * (k & 3))) & 0xFFu);
+  };
+  auto walk = [&](int& n) -> int {
+    int ip = 0, op = 0;
+    while (true) {
+      if (ip >= cs) return kErrStream;
+      const int token = byte(ip++);
+      int lit = token >> 4;
+      if (lit == 15) {
+        int b;
+        do {
+          if (ip >= cs) return kErrStream;
+          b = byte(ip++);
+          lit += b;
+        } while (b == 255);
+      }
+      if (lit > cs - ip || lit > osz - op) return kErrStream;
+      const int lip = ip;
+      ip += lit;
+      op += lit;
+      int off = 0, ml = 0;
+      const bool last = ip == cs;
+      if (!last) {
+        if (ip + 2 > cs) return kErrStream;
+        off = byte(ip) | (byte(ip + 1) << 8);
+        ip += 2;
+        ml = token & 15;
+        if (ml == 15) {
+          int b;
+          do {
+            if (ip >= cs) return kErrStream;
+            b = byte(ip++);
+            ml += b;
+          } while (b == 255);
+        }
+        ml += 4;
+        if (off == 0 || off > op || ml > osz - op) return kErrStream;
+        op += ml;
+      }
+      if (n == cap) {
+        n = -1;                                       // the decode kernel parses this block
+        return 0;
+      }
+      seqs[seq_slot(d, n, cap)] = make_uint2((unsigned)(lip | (lit << 16)), (unsigned)(off | (ml << 16)));
+      ++n;
+      if (last) return op == osz ? 0 : kErrStream;
+    }
+  };
+  int n = 0;
+  const int err = walk(n);
+  if (err) atomicOr(status, err);
+  nseq[d] = err ? -2 : n;
+}
+
+// Literal runs shorter than this are copied one lane per sequence; longer
+// ones by the whole wave.
+constexpr int kShortLit = 16;
+
+// The copies of one round of up to 64 sequences, lane j holding sequence j
+// ({literal start | length << 16, offset | match length << 16}; 0 beyond n),
+// output from op0; returns the round's output bytes.  Literal bytes come
+// from src (LDS or global).  Every literal run of the round is copied at
+// once (its destination follows from the lengths before it: a wave prefix
+// sum), then the matches in order, each one read of its source and one
+// write; a match only reads bytes before its own destination, so the
+// literals of later sequences, written first, are never read early.  The
+// workgroup is one wave, whose LDS operations complete in issue order, so
+// the phases need only compiler ordering fences.
+template <typename Src>
+__device__ __forceinline__ int lz4_round(uint2 e, int n, int op0, int lane, const Src* src, uint8_t* cout) {
+  const int lit_l = (int)(e.x >> 16), lip_l = (int)(e.x & 0xFFFFu);
+  const int ml_l = (int)(e.y >> 16), off_l = (int)(e.y & 0xFFFFu);
+  const bool act = lane < n;
+  const int len = act ? lit_l + ml_l : 0;
+  int incl = len;
+#pragma unroll
+  for (int sh = 1; sh < 64; sh <<= 1) {
+    const int y = __shfl_up(incl, sh);
+    if (lane >= sh) incl += y;
+  }
+  const int lop = op0 + incl - len;                   // this lane's sequence starts here
+  if (act && lit_l < kShortLit) {
+    uint8_t b[kShortLit - 1];
+#pragma unroll
+    for (int i = 0; i < kShortLit - 1; ++i)
+      if (i < lit_l) b[i] = src[lip_l + i];
+#pragma unroll
+    for (int i = 0; i < kShortLit - 1; ++i)
+      if (i < lit_l) cout[lop + i] = b[i];
+  }
+  unsigned long long longs = __ballot(act && lit_l >= kShortLit);
+  while (longs) {
+    const int j = __builtin_ctzll(longs);
+    longs &= longs - 1ull;
+    const int o = __builtin_amdgcn_readlane(lop, j), s = __builtin_amdgcn_readlane(lip_l, j);
+    const int L = __builtin_amdgcn_readlane(lit_l, j);
+    for (int t = lane; t < L; t += 64) cout[o + t] = src[s + t];
+  }
+  asm volatile("" ::: "memory");                      // literals before the matches read them
+  const int mdst = lop + lit_l;
+  for (int j = 0; j < n; ++j) {
+    const int ml = __builtin_amdgcn_readlane(ml_l, j);
+    if (ml == 0) continue;                            // the block's last sequence
+    const int off = __builtin_amdgcn_readlane(off_l, j);
+    const int dst = __builtin_amdgcn_readlane(mdst, j);
+    if (off >= ml) {                                  // no overlap: a plain copy
+      for (int t = lane; t < ml; t += 64) cout[dst + t] = cout[dst - off + t];
+    } else if (off == 1) {                            // a run of one byte: 16-byte stores between the ends
+      const unsigned v = (unsigned)cout[dst - 1] * 0x01010101u;
+      const int a1 = dst + ml;
+      const int h = min(a1, (dst + 15) & ~15);
+      const int body = a1 & ~15;
+      if (lane < h - dst) cout[dst + lane] = (uint8_t)v;
+      for (int q = h + 16 * lane; q + 16 <= body; q += 1024)
+        *reinterpret_cast<uint4*>(cout + q) = make_uint4(v, v, v, v);
+      const int ts = max(h, body);
+      if (lane < a1 - ts) cout[ts + lane] = (uint8_t)v;
+    } else {                                          // period off: out[dst + t] = out[dst - off + t mod off]
+      const float inv = __builtin_amdgcn_rcpf((float)off);
+      for (int t = lane; t < ml; t += 64) {
+        int r = t - (int)((float)t * inv) * off;      // the quotient is within 1 of t / off
+        r += (r < 0) ? off : 0;
+        r -= (r >= off) ? off : 0;
+        cout[dst + t] = cout[dst - off + r];
+      }
+    }
+    asm volatile("" ::: "memory");
+  }
+  return __builtin_amdgcn_readlane(incl, 63);
+}
+
 __global__ void __launch_bounds__(64)
-k_bslz4_decode(const uint8_t* __restrict__ comp, const BlockDesc* __restrict__ table, long long n_desc,
+k_bslz4_decode(const uint8_t* __restrict__ comp, long long comp_bytes, const BlockDesc* __restrict__ table,
+               long long n_desc, int cap, const uint2* __restrict__ seqs, const int* __restrict__ nseq,
                uint16_t* __restrict__ out, int* __restrict__ status) {
-  __shared__ uint8_t cin[kStageBytes];
+  __shared__ __attribute__((aligned(16))) unsigned cin32[kStageBytes / 4];
   __shared__ __attribute__((aligned(16))) uint8_t cout[kMaxBlockBytes];
+  uint8_t* const cin = reinterpret_cast<uint8_t*>(cin32);
   const int lane = threadIdx.x;
-  for (long long d = blockIdx.x; d < n_desc; d += gridDim.x) {
+  // Blocks in groups of 16 per XCD (workgroup w runs on XCD w mod 8 and
+  // gridDim.x is a multiple of 8), so the 16 blocks whose sequence entries
+  // share a 128-byte line read it through one L2.
+  const long long npad = (n_desc + 127) / 128 * 128;
+  for (long long w = blockIdx.x; w < npad; w += gridDim.x) {
+    const long long i = w >> 3;
+    const long long d = (i >> 4) * 128 + (w & 7) * 16 + (i & 15);
+    if (d >= n_desc) continue;                         // uniform
     const BlockDesc bd = table[d];
-    if (bd.elems == 0) continue;                       // uniform
+    if (bd.elems == 0) continue;
     if (bd.csize < 0) {                                // raw tail, little-endian
       if (lane < bd.elems) {
         const uint8_t* s = comp + bd.src + 2 * lane;
@@ -138,84 +344,104 @@ k_bslz4_decode(const uint8_t* __restrict__ comp, const BlockDesc* __restrict__ t
       }
       continue;
     }
+    const int n = nseq[d];
+    if (n == -2) continue;                             // malformed: flagged by the parse
     const int cs = bd.csize;
     const int osz = bd.elems * 2;
-    const bool staged = cs <= kStageBytes;             // uniform
     const uint8_t* gsrc = comp + bd.src;
+    const int s0 = (int)(reinterpret_cast<uintptr_t>(gsrc) & 3u);
+    const bool staged = s0 + cs <= kStageBytes;        // uniform
     __syncthreads();                                   // previous block's readers are done
-    if (staged)
-      for (int t = lane; t < cs; t += 64) cin[t] = gsrc[t];
-    __syncthreads();
-    // LZ4 block: every lane walks the same sequence headers.  The workgroup
-    // is one wave, whose LDS operations complete in issue order, so the
-    // copies need only a compiler ordering fence between them, not a
-    // barrier (4% faster on detector data than with barriers).  Holding the
-    // block in registers and reading the headers with readlane measured
-    // slower (157 vs 215 GB/s decoded).
-    auto parse = [&](auto byte_at, int& ip, int& op) -> int {
-      while (true) {
-        if (ip >= cs) return kErrStream;
-        const int token = byte_at(ip++);
-        int lit = token >> 4;
-        if (lit == 15) {
-          int b;
-          do {
-            if (ip >= cs) return kErrStream;
-            b = byte_at(ip++);
-            lit += b;
-          } while (b == 255);
-        }
-        if (ip + lit > cs || op + lit > osz) return kErrStream;
-        for (int t = lane; t < lit; t += 64) cout[op + t] = byte_at(ip + t);
-        ip += lit;
-        op += lit;
-        if (ip == cs) return 0;                          // the last sequence has no match
-        if (ip + 2 > cs) return kErrStream;
-        const int off = byte_at(ip) | (byte_at(ip + 1) << 8);
-        ip += 2;
-        int ml = token & 15;
-        if (ml == 15) {
-          int b;
-          do {
-            if (ip >= cs) return kErrStream;
-            b = byte_at(ip++);
-            ml += b;
-          } while (b == 255);
-        }
-        ml += 4;
-        if (off == 0 || off > op || op + ml > osz) return kErrStream;
-        asm volatile("" ::: "memory");                   // literals before the match reads them
-        if (off >= ml) {                                 // uniform: no overlap, a plain copy
-          for (int t = lane; t < ml; t += 64) cout[op + t] = cout[op - off + t];
-        } else if (off == 1) {                           // a run of one byte
-          const uint8_t r = cout[op - 1];
-          for (int t = lane; t < ml; t += 64) cout[op + t] = r;
+    if (staged) {
+      // the payload and the bytes around it up to whole 4-byte words, each
+      // word read only where it lies inside the compressed buffer
+      const uint8_t* const a0 = gsrc - s0;
+      const uint8_t* const lim = comp + comp_bytes;
+      const int nw = (s0 + cs + 3) >> 2;
+      for (int w4 = lane; w4 < nw; w4 += 64) {
+        const uint8_t* q = a0 + 4 * w4;
+        unsigned v = 0u;
+        if (q >= comp && q + 4 <= lim) {
+          v = *reinterpret_cast<const unsigned*>(q);
         } else {
-          for (int t = lane; t < ml; t += 64) cout[op + t] = cout[op - off + (t % off)];
+          for (int k = 0; k < 4; ++k)
+            if (q + k >= comp && q + k < lim) v |= (unsigned)q[k] << (8 * k);
         }
-        op += ml;
-        asm volatile("" ::: "memory");
+        cin32[w4] = v;
       }
-    };
-    int ip = 0, op = 0;
-    int err = staged ? parse([&](int i) -> int { return cin[i]; }, ip, op)
-                     : parse([&](int i) -> int { return gsrc[i]; }, ip, op);
-    if (!err && op != osz) err = kErrStream;
+      __syncthreads();
+    }
+    int err = 0;
+    if (n >= 0) {
+      int op = 0;
+      for (int r0 = 0; r0 < n; r0 += 64) {
+        const int m = min(64, n - r0);
+        const uint2 e = (lane < m) ? seqs[seq_slot(d, r0 + lane, cap)] : make_uint2(0u, 0u);
+        op += staged ? lz4_round(e, m, op, lane, cin + s0, cout) : lz4_round(e, m, op, lane, gsrc, cout);
+      }
+    } else {
+      // more sequences than the table holds: every lane walks the same
+      // headers here and the copies follow each header
+      auto parse = [&](auto byte_at, int& ip, int& op) -> int {
+        while (true) {
+          if (ip >= cs) return kErrStream;
+          const int token = byte_at(ip++);
+          int lit = token >> 4;
+          if (lit == 15) {
+            int b;
+            do {
+              if (ip >= cs) return kErrStream;
+              b = byte_at(ip++);
+              lit += b;
+            } while (b == 255);
+          }
+          if (ip + lit > cs || op + lit > osz) return kErrStream;
+          for (int t = lane; t < lit; t += 64) cout[op + t] = byte_at(ip + t);
+          ip += lit;
+          op += lit;
+          if (ip == cs) return 0;                        // the last sequence has no match
+          if (ip + 2 > cs) return kErrStream;
+          const int off = byte_at(ip) | (byte_at(ip + 1) << 8);
+          ip += 2;
+          int ml = token & 15;
+          if (ml == 15) {
+            int b;
+            do {
+              if (ip >= cs) return kErrStream;
+              b = byte_at(ip++);
+              ml += b;
+            } while (b == 255);
+          }
+          ml += 4;
+          if (off == 0 || off > op || op + ml > osz) return kErrStream;
+          asm volatile("" ::: "memory");                 // literals before the match reads them
+          if (off >= ml) {
+            for (int t = lane; t < ml; t += 64) cout[op + t] = cout[op - off + t];
+          } else {
+            for (int t = lane; t < ml; t += 64) cout[op + t] = cout[op - off + (t % off)];
+          }
+          op += ml;
+          asm volatile("" ::: "memory");
+        }
+      };
+      int ip = 0, op = 0;
+      err = staged ? parse([&](int k) -> int { return cin[s0 + k]; }, ip, op)
+                   : parse([&](int k) -> int { return gsrc[k]; }, ip, op);
+      if (!err && op != osz) err = kErrStream;
+    }
     if (err) {
       if (lane == 0) atomicOr(status, err);
       continue;                                        // uniform: every lane saw the same stream
     }
     __syncthreads();
-    // inverse bit-shuffle, 8 elements per step: the 16 plane bytes of the
+    // inverse bit-shuffle, 8 elements per group: the 16 plane bytes of the
     // group form two 8x8 bit matrices (planes 0-7 and 8-15, byte = plane,
-    // bit = element); each is transposed in a 64-bit word by three
-    // delta swaps, after which byte i holds element i's low or high byte,
-    // and v_perm interleaves them into the output words.
+    // bit = element); each is transposed in a 64-bit word by three delta
+    // swaps, after which byte i holds element i's low or high byte, and
+    // v_perm interleaves them into the output words.  With whole words per
+    // plane, a lane takes 4 groups: one LDS word per plane, 64 bytes out.
     const int planeb = bd.elems / 8;
-    for (int g = lane; g < planeb; g += 64) {
-      unsigned bt[16];
-#pragma unroll
-      for (int b = 0; b < 16; ++b) bt[b] = cout[b * planeb + g];
+    auto group = [&](const unsigned (&bt)[16]) -> uint4 {
       unsigned long long m[2];
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
@@ -238,12 +464,37 @@ k_bslz4_decode(const uint8_t* __restrict__ comp, const BlockDesc* __restrict__ t
       w.y = __builtin_amdgcn_perm(yl, xl, 0x07030602u);
       w.z = __builtin_amdgcn_perm(yh, xh, 0x05010400u);
       w.w = __builtin_amdgcn_perm(yh, xh, 0x07030602u);
-      if ((bd.dst & 7) == 0) {                       // uniform: 16-byte aligned groups
-        *reinterpret_cast<uint4*>(out + bd.dst + 8LL * g) = w;
-      } else {
-        const unsigned ww[4] = {w.x, w.y, w.z, w.w};
+      return w;
+    };
+    if ((planeb & 3) == 0 && (bd.dst & 7) == 0) {      // uniform
+      const unsigned* c32 = reinterpret_cast<const unsigned*>(cout);
+      const int pw = planeb >> 2;
+      for (int g4 = lane; g4 < pw; g4 += 64) {
+        unsigned wd[16];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) out[bd.dst + 8LL * g + i] = (uint16_t)(ww[i >> 1] >> (16 * (i & 1)));
+        for (int b = 0; b < 16; ++b) wd[b] = c32[b * pw + g4];
+        uint4* o = reinterpret_cast<uint4*>(out + bd.dst + 32LL * g4);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          unsigned bt[16];
+#pragma unroll
+          for (int b = 0; b < 16; ++b) bt[b] = (wd[b] >> (8 * k)) & 0xFFu;
+          o[k] = group(bt);
+        }
+      }
+    } else {
+      for (int g = lane; g < planeb; g += 64) {
+        unsigned bt[16];
+#pragma unroll
+        for (int b = 0; b < 16; ++b) bt[b] = cout[b * planeb + g];
+        const uint4 w = group(bt);
+        if ((bd.dst & 7) == 0) {                       // uniform: 16-byte aligned groups
+          *reinterpret_cast<uint4*>(out + bd.dst + 8LL * g) = w;
+        } else {
+          const unsigned ww[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+          for (int k = 0; k < 8; ++k) out[bd.dst + 8LL * g + k] = (uint16_t)(ww[k >> 1] >> (16 * (k & 1)));
+        }
       }
     }
   }
@@ -262,16 +513,39 @@ long long bslz4_slots(long long chunk_elems, int block_elems) {
   return full + ((rest - rest % 8) ? 1 : 0) + ((rest % 8) ? 1 : 0);
 }
 
+// Workspace: the status word, the block table, the per-block sequence
+// counts and the sequence table, each from a 256-byte boundary.
+namespace {
+long long round256(long long x) { return (x + 255) / 256 * 256; }
+struct WsLayout {
+  long long table, nseq, seqs, total;
+};
+WsLayout ws_layout(long long n_desc, int block_elems) {
+  WsLayout l;
+  l.table = 256;
+  l.nseq = l.table + round256(n_desc * (long long)sizeof(BlockDesc));
+  l.seqs = l.nseq + round256(n_desc * 4);
+  l.total = l.seqs + (n_desc + 63) / 64 * 64 * (long long)seq_cap(block_elems) * (long long)sizeof(uint2);
+  return l;
+}
+}  // namespace
+
 long long bslz4_workspace_bytes(int n_chunks, long long chunk_elems, int block_elems) {
-  return 256 + (long long)n_chunks * bslz4_slots(chunk_elems, block_elems) * (long long)sizeof(BlockDesc);
+  return ws_layout((long long)n_chunks * bslz4_slots(chunk_elems, block_elems), block_elems).total;
 }
 
 hipError_t launch_bslz4_decode_u16(const uint8_t* comp, long long comp_bytes, const long long* chunk_off,
                                    const long long* chunk_len, int n_chunks, long long chunk_elems, int block_elems,
                                    uint16_t* out, void* workspace, hipStream_t s) {
-  int* status = reinterpret_cast<int*>(workspace);
-  BlockDesc* table = reinterpret_cast<BlockDesc*>(reinterpret_cast<char*>(workspace) + 256);
   const long long slots = bslz4_slots(chunk_elems, block_elems);
+  const long long n_desc = (long long)n_chunks * slots;
+  const WsLayout l = ws_layout(n_desc, block_elems);
+  char* const ws = reinterpret_cast<char*>(workspace);
+  int* status = reinterpret_cast<int*>(ws);
+  BlockDesc* table = reinterpret_cast<BlockDesc*>(ws + l.table);
+  int* nseq = reinterpret_cast<int*>(ws + l.nseq);
+  uint2* seqs = reinterpret_cast<uint2*>(ws + l.seqs);
+  const int cap = seq_cap(block_elems);
   hipLaunchKernelGGL(k_bslz4_clear, dim3(1), dim3(64), 0, s, status);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -279,9 +553,15 @@ hipError_t launch_bslz4_decode_u16(const uint8_t* comp, long long comp_bytes, co
                      chunk_off, chunk_len, n_chunks, chunk_elems, block_elems, (int)slots, table, status);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const long long n_desc = (long long)n_chunks * slots;
-  const unsigned grid = (unsigned)(n_desc < 65536 ? n_desc : 65536);
-  hipLaunchKernelGGL(k_bslz4_decode, dim3(grid > 0 ? grid : 1), dim3(64), 0, s, comp, table, n_desc, out, status);
+  if (n_desc == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_bslz4_parse, dim3((unsigned)((n_desc + 255) / 256)), dim3(256), 0, s, comp, comp_bytes, table,
+                     n_desc, cap, seqs, nseq, status);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const long long npad = (n_desc + 127) / 128 * 128;   // a multiple of 8: workgroup w stays on XCD w mod 8
+  const unsigned grid = (unsigned)(npad < 262144 ? npad : 262144);
+  hipLaunchKernelGGL(k_bslz4_decode, dim3(grid), dim3(64), 0, s, comp, comp_bytes, table, n_desc, cap, seqs, nseq,
+                     out, status);
   return hipGetLastError();
 }
 

@@ -18,7 +18,23 @@ pytestmark = pytest.mark.gpu
 def _frames(kind, n, shape, seed):
     rng = np.random.default_rng(seed)
     size = (n,) + shape
-    return {"poisson": lambda: rng.poisson(0.8, size), "random": lambda: rng.integers(0, 65536, size),
+    if kind == "disk":                  # a bright disk of radius 16 on a dark detector
+        yy, xx = np.mgrid[:shape[0], :shape[1]]
+        inside = (yy - shape[0] / 2) ** 2 + (xx - shape[1] / 2) ** 2 < 16 ** 2
+        return (rng.poisson(60, size) * inside).astype(np.uint16)
+    if kind.startswith("runs"):         # every bit-plane in random runs of whole 4- or 8-byte words
+        g = int(kind[4:])
+        m = int(np.prod(shape))
+        v = np.zeros((n, m), np.uint32)
+        for b in range(16):
+            v |= np.repeat(rng.integers(0, 2, (n, m // g)), g, axis=1).astype(np.uint32) << b
+        return v.reshape(size).astype(np.uint16)
+    if kind == "periodic":              # bit-planes that repeat every 2 to 30 bytes
+        p = np.arange(int(np.prod(shape)))
+        v = ((p % 24) * 2749 + (p % 16) * 17 + (p % 40) * 3) % 65536
+        return np.broadcast_to(v.reshape(shape), size).astype(np.uint16)
+    return {"poisson": lambda: rng.poisson(0.8, size), "noisy": lambda: rng.poisson(20, size),
+            "random": lambda: rng.integers(0, 65536, size),
             "zeros": lambda: np.zeros(size), "max": lambda: np.full(size, 65535)}[kind]().astype(np.uint16)
 
 
@@ -30,6 +46,12 @@ def _frames(kind, n, shape, seed):
     ("zeros", 2, (128, 128), 1),        # long overlapping matches
     ("max", 2, (33, 33), 1),            # 1089 = 1088 + 1 raw
     ("poisson", 8, (32, 32), 4),        # chunks of 4 frames
+    ("disk", 4, (128, 128), 1),         # ~150 sequences per block: several rounds of 64
+    ("periodic", 2, (64, 64), 1),       # matches with an offset below their length
+    ("noisy", 2, (64, 64), 1),          # ~40 sequences in ~2.7 KB: literals read from global memory
+    ("random", 2, (64, 64), 1),         # incompressible: one 8 KB literal run from global memory
+    ("runs64", 2, (64, 64), 1),         # ~500 sequences in ~1.7 KB: more than the sequence table holds
+    ("runs32", 2, (64, 64), 1),         # ~1000 sequences in ~3.2 KB, the same from global memory
 ])
 def test_decode_is_bit_exact(cuda, kind, n, shape, per_chunk):
     from quantem.gpu_amd.ssb.compute.hip.bslz4 import decode_u16
