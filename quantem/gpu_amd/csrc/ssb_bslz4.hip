@@ -146,7 +146,7 @@ __device__ __forceinline__ long long seq_slot(long long d, int k, int cap) {
 // the count, 0 for unused slots and raw tails, -1 when the block has more
 // sequences than the table holds (the decode kernel then parses it), -2
 // for a malformed block (its status bit set).  Each thread reads its block
-// through a 16-byte window of aligned vector loads.
+// through a 32-byte window of aligned vector loads.
 __global__ void __launch_bounds__(256)
 k_bslz4_parse(const uint8_t* __restrict__ comp, long long comp_bytes, const BlockDesc* __restrict__ table,
               long long n_desc, int cap, uint2* __restrict__ seqs, int* __restrict__ nseq, int* __restrict__ status) {
@@ -161,18 +161,17 @@ k_bslz4_parse(const uint8_t* __restrict__ comp, long long comp_bytes, const Bloc
   const int osz = bd.elems * 2;
   const uint8_t* const g = comp + bd.src;
   const uint8_t* const lim = comp + comp_bytes;
-  int wb = -64;                                       // the window holds g[wb, wb + 16); g + wb 16-byte aligned
-  unsigned w0 = 0u, w1 = 0u, w2 = 0u, w3 = 0u;        // (separate words: a vector indexed by k would go to memory)
+  int wb = -64;                                       // the window holds g[wb, wb + 32); g + wb 32-byte aligned
+  // the window's words in separate registers (an indexed array would be placed in memory)
+  unsigned w0 = 0u, w1 = 0u, w2 = 0u, w3 = 0u, w4 = 0u, w5 = 0u, w6 = 0u, w7 = 0u;
   auto byte = [&](int p) -> int {                     // 0 <= p < cs
-    if ((unsigned)(p - wb) >= 16u) {
-      const uint8_t* a = reinterpret_cast<const uint8_t*>(reinterpret_cast<uintptr_t>(g + p) & ~(uintptr_t)15);
+    if ((unsigned)(p - wb) >= 32u) {
+      const uint8_t* a = reinterpret_cast<const uint8_t*>(reinterpret_cast<uintptr_t>(g + p) & ~(uintptr_t)31);
       wb = (int)(a - g);
-      if (a + 16 <= lim) {
-        const uint4 v = *reinterpret_cast<const uint4*>(a);
-        w0 = v.x;
-        w1 = v.y;
-        w2 = v.z;
-        w3 = v.w;
+      if (a + 32 <= lim) {
+        const uint4 v0 = reinterpret_cast<const uint4*>(a)[0], v1 = reinterpret_cast<const uint4*>(a)[1];
+        w0 = v0.x, w1 = v0.y, w2 = v0.z, w3 = v0.w;
+        w4 = v1.x, w5 = v1.y, w6 = v1.z, w7 = v1.w;
       } else {                                        // the buffer's last bytes: only those inside it
         auto word = [&](int o) {
           unsigned v = 0u;
@@ -180,15 +179,15 @@ k_bslz4_parse(const uint8_t* __restrict__ comp, long long comp_bytes, const Bloc
             if (a + o + i < lim) v |= (unsigned)a[o + i] << (8 * i);
           return v;
         };
-        w0 = word(0);
-        w1 = word(4);
-        w2 = word(8);
-        w3 = word(12);
+        w0 = word(0), w1 = word(4), w2 = word(8), w3 = word(12);
+        w4 = word(16), w5 = word(20), w6 = word(24), w7 = word(28);
       }
     }
     const int k = p - wb;
-    const unsigned lo = (k & 4) ? w1 : w0, hi = (k & 4) ? w3 : w2;
-    return (int)((((k & 8) ? hi : lo) >> (8 * (k & 3))) & 0xFFu);
+    const bool b2 = k & 4, b3 = k & 8;
+    const unsigned x0 = b2 ? w1 : w0, x1 = b2 ? w3 : w2, x2 = b2 ? w5 : w4, x3 = b2 ? w7 : w6;
+    const unsigned y0 = b3 ? x1 : x0, y1 = b3 ? x3 : x2;
+    return (int)((((k & 16) ? y1 : y0) >> (8 * (k & 3))) & 0xFFu);
   };
   auto walk = [&](int& n) -> int {
     int ip = 0, op = 0;
@@ -249,15 +248,16 @@ constexpr int kShortLit = 16;
 // The copies of one round of up to 64 sequences, lane j holding sequence j
 // ({literal start | length << 16, offset | match length << 16}; 0 beyond n),
 // output from op0; returns the round's output bytes.  Literal bytes come
-// from src (LDS or global).  Every literal run of the round is copied at
-// once (its destination follows from the lengths before it: a wave prefix
-// sum), then the matches in order, each one read of its source and one
-// write; a match only reads bytes before its own destination, so the
-// literals of later sequences, written first, are never read early.  The
-// workgroup is one wave, whose LDS operations complete in issue order, so
-// the phases need only compiler ordering fences.
+// from src[0, src_end) (LDS or global).  Every literal run of the round is
+// copied at once (its destination follows from the lengths before it: a
+// wave prefix sum), then the matches in order, each one read of its source
+// and one write; a match only reads bytes before its own destination, so
+// the literals of later sequences, written first, are never read early.
+// The workgroup is one wave, whose LDS operations complete in issue order,
+// so the phases need only compiler ordering fences.
 template <typename Src>
-__device__ __forceinline__ int lz4_round(uint2 e, int n, int op0, int lane, const Src* src, uint8_t* cout) {
+__device__ __forceinline__ int lz4_round(uint2 e, int n, int op0, int lane, const Src* src, int src_end,
+                                         uint8_t* cout) {
   const int lit_l = (int)(e.x >> 16), lip_l = (int)(e.x & 0xFFFFu);
   const int ml_l = (int)(e.y >> 16), off_l = (int)(e.y & 0xFFFFu);
   const bool act = lane < n;
@@ -270,10 +270,9 @@ __device__ __forceinline__ int lz4_round(uint2 e, int n, int op0, int lane, cons
   }
   const int lop = op0 + incl - len;                   // this lane's sequence starts here
   if (act && lit_l < kShortLit) {
-    uint8_t b[kShortLit - 1];
+    uint8_t b[kShortLit - 1];                         // read together (clamped inside src), then written
 #pragma unroll
-    for (int i = 0; i < kShortLit - 1; ++i)
-      if (i < lit_l) b[i] = src[lip_l + i];
+    for (int i = 0; i < kShortLit - 1; ++i) b[i] = src[min(lip_l + i, src_end - 1)];
 #pragma unroll
     for (int i = 0; i < kShortLit - 1; ++i)
       if (i < lit_l) cout[lop + i] = b[i];
@@ -287,14 +286,24 @@ __device__ __forceinline__ int lz4_round(uint2 e, int n, int op0, int lane, cons
     for (int t = lane; t < L; t += 64) cout[o + t] = src[s + t];
   }
   asm volatile("" ::: "memory");                      // literals before the matches read them
-  const int mdst = lop + lit_l;
+  const int mdst = lop + lit_l;                       // match destination and source
+  const int msrc = mdst - off_l;
   for (int j = 0; j < n; ++j) {
     const int ml = __builtin_amdgcn_readlane(ml_l, j);
-    if (ml == 0) continue;                            // the block's last sequence
-    const int off = __builtin_amdgcn_readlane(off_l, j);
     const int dst = __builtin_amdgcn_readlane(mdst, j);
-    if (off >= ml) {                                  // no overlap: a plain copy
-      for (int t = lane; t < ml; t += 64) cout[dst + t] = cout[dst - off + t];
+    const int sp = __builtin_amdgcn_readlane(msrc, j);
+    const int off = dst - sp;
+    if (ml <= 64 && off >= ml) {                      // the common case (and the last sequence, ml = 0)
+      if (lane < ml) cout[dst + lane] = cout[sp + lane];
+    } else if (off >= ml) {                           // a long plain copy, 4 reads in flight per lane
+      for (int t = lane; t < ml; t += 256) {
+        uint8_t v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = cout[sp + min(t + 64 * u, ml - 1)];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (t + 64 * u < ml) cout[dst + t + 64 * u] = v[u];
+      }
     } else if (off == 1) {                            // a run of one byte: 16-byte stores between the ends
       const unsigned v = (unsigned)cout[dst - 1] * 0x01010101u;
       const int a1 = dst + ml;
@@ -305,13 +314,21 @@ __device__ __forceinline__ int lz4_round(uint2 e, int n, int op0, int lane, cons
         *reinterpret_cast<uint4*>(cout + q) = make_uint4(v, v, v, v);
       const int ts = max(h, body);
       if (lane < a1 - ts) cout[ts + lane] = (uint8_t)v;
-    } else {                                          // period off: out[dst + t] = out[dst - off + t mod off]
-      const float inv = __builtin_amdgcn_rcpf((float)off);
-      for (int t = lane; t < ml; t += 64) {
-        int r = t - (int)((float)t * inv) * off;      // the quotient is within 1 of t / off
-        r += (r < 0) ? off : 0;
-        r -= (r >= off) ? off : 0;
-        cout[dst + t] = cout[dst - off + r];
+    } else {                                          // period off: out[dst + t] = out[sp + t mod off],
+      const float inv = __builtin_amdgcn_rcpf((float)off);   // all read from [sp, dst), 4 in flight per lane
+      for (int t = lane; t < ml; t += 256) {
+        uint8_t v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int tt = t + 64 * u;
+          int r = tt - (int)((float)tt * inv) * off;  // the quotient is within 1 of tt / off
+          r += (r < 0) ? off : 0;
+          r -= (r >= off) ? off : 0;
+          v[u] = cout[sp + r];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (t + 64 * u < ml) cout[dst + t + 64 * u] = v[u];
       }
     }
     asm volatile("" ::: "memory");
@@ -377,7 +394,7 @@ k_bslz4_decode(const uint8_t* __restrict__ comp, long long comp_bytes, const Blo
       for (int r0 = 0; r0 < n; r0 += 64) {
         const int m = min(64, n - r0);
         const uint2 e = (lane < m) ? seqs[seq_slot(d, r0 + lane, cap)] : make_uint2(0u, 0u);
-        op += staged ? lz4_round(e, m, op, lane, cin + s0, cout) : lz4_round(e, m, op, lane, gsrc, cout);
+        op += staged ? lz4_round(e, m, op, lane, cin + s0, cs, cout) : lz4_round(e, m, op, lane, gsrc, cs, cout);
       }
     } else {
       // more sequences than the table holds: every lane walks the same
