@@ -399,8 +399,11 @@ int ssb_recon_u16_after(ssb_plan* plan, const ssb_plan* after, const uint16_t* d
  * the filter's block size (8192 for the default uint16 blocks; a multiple of
  * 16 up to 8192); every chunk header must agree with it and with
  * chunk_elems.  workspace (device, 16-byte aligned, at least
- * ssb_bslz4_workspace_bytes(...) bytes) holds the status word and the
- * block table.  With status_host non-NULL the call waits and reports a
+ * ssb_bslz4_workspace_bytes(...) bytes) holds the status word, the
+ * block table and a table of LZ4 sequences (8 bytes per entry,
+ * block_bytes / 32 entries per block, at least 64 and at most what a block
+ * can hold: 2 KiB per 8 KiB block, a quarter of the decoded size; blocks
+ * with more sequences are still decoded).  With status_host non-NULL the call waits and reports a
  * malformed chunk as QSSB_ERR_ARG; with NULL it stays asynchronous and
  * the status word (int at workspace[0], 0 = ok) can be read later.
  * Replaces the CUDA bslz4 decoder (qg/io/backends/cuda/decoder.py:258-1146;

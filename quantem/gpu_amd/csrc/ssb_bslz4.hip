@@ -126,11 +126,15 @@ k_bslz4_index(const uint8_t* __restrict__ comp, long long comp_bytes, const long
   }
 }
 
-// Sequence table entries per block: at most block_elems / 16 (256 for the
-// default 8192-byte blocks: an average of 32 or more decoded bytes per
-// sequence), and at least 64.  A block with more sequences is parsed by the
-// decode kernel itself.
-__host__ __device__ __forceinline__ int seq_cap(int block_elems) { return block_elems / 16 > 64 ? block_elems / 16 : 64; }
+// Sequence table entries per block: block_elems / 16 (256 for the default
+// 8192-byte blocks: an average of 32 or more decoded bytes per sequence),
+// at least 64, and never more than a block can hold (every sequence but
+// the last decodes at least 4 bytes).  A block with more sequences is
+// parsed by the decode kernel itself.
+__host__ __device__ __forceinline__ int seq_cap(int block_elems) {
+  const int c = block_elems / 16 > 64 ? block_elems / 16 : 64;
+  return c < block_elems / 2 + 1 ? c : block_elems / 2 + 1;
+}
 
 // Entry k of block d's sequence table: blocks in groups of 64, entry-major
 // inside a group, so that the 64 lanes of a parse wave store one entry each

@@ -61,6 +61,19 @@ def test_decode_is_bit_exact(cuda, kind, n, shape, per_chunk):
     np.testing.assert_array_equal(out.cpu().numpy(), v)
 
 
+@pytest.mark.parametrize("block_bytes", [16, 256, 2048])
+@pytest.mark.parametrize("kind", ["poisson", "runs32"])
+def test_decode_other_block_sizes(cuda, kind, block_bytes):
+    """Filter block sizes other than 8192 bytes: the sequence table holds
+    block_bytes / 32 entries (at least 64, at most what a block can hold),
+    so these blocks also reach the walk inside the decode kernel."""
+    from quantem.gpu_amd.ssb.compute.hip.bslz4 import decode_u16
+    v = _frames(kind, 3, (40, 64), block_bytes)
+    chunks = [B.encode_chunk(v[i:i + 1], block_bytes) for i in range(3)]
+    out = decode_u16(chunks, (1, 40, 64), block_bytes=block_bytes)
+    np.testing.assert_array_equal(out.cpu().numpy(), v)
+
+
 def test_malformed_chunks_are_rejected(cuda):
     from quantem.gpu_amd.ssb.compute.hip.bslz4 import decode_u16
     v = _frames("poisson", 1, (64, 64), 0)
