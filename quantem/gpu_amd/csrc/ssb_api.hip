@@ -1079,16 +1079,15 @@ static int recon_typed(ssb_plan* p, const ssb_plan* after, const void* data, int
                                  p->tw, p->S_part, nullptr, true, p->tune_trotter, s, &groups, 1, 2),
             "colfft_trotter");
   } else {
-    // (geometry and phasors on a second stream beside the gather measured
-    // slower: the cross-stream event pair costs more than the ~9 us they
-    // would hide)
+    // The gather's threads also write the phasor tables (one launch fewer;
+    // geometry and phasors on a second stream beside the gather measured
+    // slower: the cross-stream event pair costs more than they would hide).
+    const PhasorJob job{p->info, geom_ptrs(p), m, p->n, p->rowph, p->colph, p->echi};
     HIP_TRY(launch_gather_rowfft(p->n, dtype, data, p->D, p->det_cols, p->bf_rows, p->bf_cols, p->info,
                                  p->nbf_hint, p->tw, p->tune_gather_pairs, p->gain, p->H, s,
-                                 sup ? p->sup_box : nullptr, sup ? p->sup_C : nullptr),
+                                 sup ? p->sup_box : nullptr, sup ? p->sup_C : nullptr, 0, 1, &job),
             "gather_rowfft");
     mark(6);
-    HIP_TRY(launch_phasors(p->n, p->info, p->nbf_hint, geom_ptrs(p), m, p->rowph, p->colph, p->echi, s),
-            "phasors");
     HIP_TRY(launch_trotter_slots(p->n, true, p->H, p->info, geom_ptrs(p), p->rowph, p->colph, p->echi, p->optics,
                                  p->tw, p->S_part, nullptr, true, p->tune_trotter, s, &groups),
             "colfft_trotter");

@@ -82,6 +82,56 @@ struct QuadChi {
   float mxx, mxy, myy;
 };
 
+// Phasor tables of the factorised overlap e^{i theta} = e^{i qx a_b} e^{i qy b_b},
+// (a_b, b_b) = 2 M k_b: rowph[b][q] for all rows, colph[b][c] for columns
+// c = 0..N/2 (the other columns are their conjugates); and the plane
+// echi[c][q] = e^{i chi(Q)} (column-major, like the partial sums).  Written
+// by k_phasors, or by the BF gather of a reconstruction on the side (the
+// same function: the same values).
+struct PhasorJob {
+  const BfInfo* info;
+  GeomPtrs gp;
+  QuadChi m;
+  int N;
+  float2* rowph;
+  float2* colph;
+  float2* echi;
+};
+
+// The items [gid, gid + stride, ...) of a phasor job.
+__device__ __forceinline__ void phasor_work(const PhasorJob& j, long long gid, long long stride) {
+  const QuadChi m = j.m;
+  const int N = j.N;
+  const int nbf = j.info->nbf;
+  const int ncol = N / 2 + 1;
+  const long long per_b = (long long)N + ncol;
+  const long long total = (long long)nbf * per_b;
+  const long long NN = (long long)N * N;
+  for (long long idx = gid; idx < NN; idx += stride) {
+    const int c = (int)(idx / N), q = (int)(idx - (long long)c * N);
+    const float x = j.gp.qx[q], y = j.gp.qy[c];
+    float sn, cs;
+    sincosf(m.mxx * x * x + 2.0f * m.mxy * x * y + m.myy * y * y, &sn, &cs);
+    j.echi[idx] = make_float2(cs, sn);
+  }
+  for (long long idx = gid; idx < total; idx += stride) {
+    const int b = (int)(idx / per_b);
+    const int r = (int)(idx - (long long)b * per_b);
+    const float kx = j.gp.kx[b], ky = j.gp.ky[b];
+    float sn, cs;
+    if (r < N) {
+      const float ta = 2.0f * (m.mxx * kx + m.mxy * ky);
+      sincosf(j.gp.qx[r] * ta, &sn, &cs);
+      j.rowph[(long long)b * N + r] = make_float2(cs, sn);
+    } else {
+      const int c = r - N;
+      const float tb = 2.0f * (m.mxy * kx + m.myy * ky);
+      sincosf(j.gp.qy[c] * tb, &sn, &cs);
+      j.colph[(long long)b * ncol + c] = make_float2(cs, sn);
+    }
+  }
+}
+
 // Bytes per sample of a QSSB_DTYPE_* code (0 = unknown); whether its
 // stream-pass partials are 64-bit (uint32, int32, float32 samples).
 int dtype_size(int dtype);
@@ -131,10 +181,13 @@ int gather_pairs_candidates(int N, int* out);   // out[2]; returns the count
 // Samples of type dtype become float32 times gain[pixel] (gain may be NULL).
 // With box and C (uint16 frames): when info->sup_ok the samples come from
 // the compact superset copy C instead of the frames (the same values).
+// With job non-NULL the gather's threads also write that phasor job's
+// tables (in place of a k_phasors launch after the gather).
 hipError_t launch_gather_rowfft(int N, int dtype, const void* data, int D, int det_cols, const int* rows,
                                 const int* cols, const BfInfo* info, int nbf_hint, const float2* tw, int pairs,
                                 const float* gain, float2* H, hipStream_t s, const SupBox* box = nullptr,
-                                const uint16_t* C = nullptr, int part = 0, int parts = 1);
+                                const uint16_t* C = nullptr, int part = 0, int parts = 1,
+                                const PhasorJob* job = nullptr);
 // Exact BF columns out[b][s] for BF pixels [bf_offset, bf_offset + count)
 // of the list rows/cols: uint8 frames -> uint8 (in_u8), uint16 frames ->
 // uint16, or uint8 when narrow.

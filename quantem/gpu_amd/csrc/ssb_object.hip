@@ -15,41 +15,13 @@ namespace qssb {
 
 int object_groups(int N) { return N == 128 ? 64 : (N == 256 ? 16 : (N == 512 ? 8 : 4)); }
 
-// Phasor tables of the factorised overlap e^{i theta} = e^{i qx a_b} e^{i qy b_b},
-// (a_b, b_b) = 2 M k_b: rowph[b][q] for all rows, colph[b][c] for columns
-// c = 0..N/2 (the other columns are their conjugates); and the plane
-// echi[c][q] = e^{i chi(Q)} (column-major, like the partial sums).
+// The phasor tables (PhasorJob) on their own launch; m_dev, when given,
+// holds the quadratic chi instead of m_arg.
 __global__ void __launch_bounds__(256)
 k_phasors(const BfInfo* __restrict__ info, GeomPtrs gp, QuadChi m_arg, int N, float2* __restrict__ rowph,
           float2* __restrict__ colph, float2* __restrict__ echi, const QuadChi* __restrict__ m_dev) {
-  const QuadChi m = m_dev ? *m_dev : m_arg;
-  const int nbf = info->nbf;
-  const int ncol = N / 2 + 1;
-  const long long per_b = (long long)N + ncol;
-  const long long total = (long long)nbf * per_b;
-  const long long NN = (long long)N * N;
-  for (long long idx = (long long)blockIdx.x * 256 + threadIdx.x; idx < NN; idx += (long long)gridDim.x * 256) {
-    const int c = (int)(idx / N), q = (int)(idx - (long long)c * N);
-    float sn, cs;
-    sincosf(chi_quad(m, gp.qx[q], gp.qy[c]), &sn, &cs);
-    echi[idx] = make_float2(cs, sn);
-  }
-  for (long long idx = (long long)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (long long)gridDim.x * 256) {
-    const int b = (int)(idx / per_b);
-    const int r = (int)(idx - (long long)b * per_b);
-    const float kx = gp.kx[b], ky = gp.ky[b];
-    float sn, cs;
-    if (r < N) {
-      const float ta = 2.0f * (m.mxx * kx + m.mxy * ky);
-      sincosf(gp.qx[r] * ta, &sn, &cs);
-      rowph[(long long)b * N + r] = make_float2(cs, sn);
-    } else {
-      const int c = r - N;
-      const float tb = 2.0f * (m.mxy * kx + m.myy * ky);
-      sincosf(gp.qy[c] * tb, &sn, &cs);
-      colph[(long long)b * ncol + c] = make_float2(cs, sn);
-    }
-  }
+  const PhasorJob j{info, gp, m_dev ? *m_dev : m_arg, N, rowph, colph, echi};
+  phasor_work(j, (long long)blockIdx.x * 256 + threadIdx.x, (long long)gridDim.x * 256);
 }
 
 // ------------------------------------------------------------------------

@@ -793,7 +793,10 @@ k_gather_rowfft(const T* __restrict__ data, int D, int det_cols,
                 const int* __restrict__ bf_rows, const int* __restrict__ bf_cols,
                 const BfInfo* __restrict__ info, const float2* __restrict__ tw_g,
                 const float* __restrict__ gain, float2* __restrict__ H, const SupBox* __restrict__ boxp,
-                const uint16_t* __restrict__ C, int part, int parts) {
+                const uint16_t* __restrict__ C, int part, int parts, PhasorJob job) {
+  // the phasor tables on the side (a few hundred thousand sincos over the
+  // whole grid, before the first gather loads)
+  if (job.rowph) phasor_work(job, (long long)blockIdx.x * 256 + threadIdx.x, (long long)gridDim.x * 256);
   constexpr int STRIDE = N + 1;
   constexpr int CH = 2 * PAIRS;
   __shared__ float2 z[PAIRS * STRIDE];
@@ -1185,8 +1188,10 @@ int gather_pairs_candidates(int N, int* out) {
 hipError_t launch_gather_rowfft(int N, int dtype, const void* data, int D, int det_cols, const int* rows,
                                 const int* cols, const BfInfo* info, int nbf_hint, const float2* tw, int pairs,
                                 const float* gain, float2* H, hipStream_t s, const SupBox* box, const uint16_t* C,
-                                int part, int parts) {
+                                int part, int parts, const PhasorJob* job) {
   if (!box) C = nullptr;
+  PhasorJob pj{};
+  if (job) pj = *job;
   if (pairs <= 0) pairs = gather_pairs_default(N);
   if (parts < 1 || part < 0 || part >= parts) return hipErrorInvalidValue;
   const int part_hint = (nbf_hint + parts - 1) / parts + 1;   // BF pixels of the largest part
@@ -1200,7 +1205,7 @@ hipError_t launch_gather_rowfft(int N, int dtype, const void* data, int D, int d
   if (N == NN && pairs == PP) {                                                                             \
     hipLaunchKernelGGL((k_gather_rowfft<NN, PP, T>), grid_for(2 * PP), dim3(256), 0, s,                     \
                        reinterpret_cast<const T*>(data), D, det_cols, rows, cols, info, tw, gain, H, box, C,  \
-                       part, parts);                                                                        \
+                       part, parts, pj);                                                                    \
     return hipGetLastError();                                                                               \
   }
   if (dtype == QSSB_DTYPE_U16) {
