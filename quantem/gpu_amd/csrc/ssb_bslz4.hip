@@ -13,18 +13,22 @@
 // elements raw.  A block decodes to 16 bit-planes of m/8 bytes: bit b of
 // element p is bit (p mod 8) of byte b*m/8 + p/8.
 //
-// Two kernels:
+// Three kernels:
 //   k_bslz4_index   one thread per chunk: validates the header and walks the
 //                   block sizes into a descriptor table (bounds-checked);
+//   k_bslz4_parse   one thread per block: walks the block's LZ4 sequence
+//                   headers (every length and offset checked) into a table
+//                   of {literal start, literal length, offset, match length}
+//                   per sequence;
 //   k_bslz4_decode  one 64-lane workgroup per block: stages the compressed
-//                   block in LDS as whole words, decodes it in LDS in rounds
-//                   of up to 64 sequences (a uniform parse of the round's
-//                   headers into one lane per sequence, then all of the
-//                   round's literal runs at once and its matches in order;
-//                   a match of offset d < length is one pass of
+//                   block in LDS as whole words and decodes it in LDS in
+//                   rounds of up to 64 table entries (one lane per
+//                   sequence: all of the round's literal runs at once, then
+//                   its matches in order; a match of offset d < length is
 //                   out[o + t] = out[o - d + t mod d], a run of one byte is
-//                   written with 16-byte stores), then inverts the
-//                   bit-shuffle, 32 elements per lane step (two 8x8
+//                   written with 16-byte stores; a block whose sequences
+//                   overflow the table is walked here instead), then inverts
+//                   the bit-shuffle, 32 elements per lane step (two 8x8
 //                   bit-matrix transposes by delta swaps and byte permutes
 //                   per 8 elements), with 16-byte stores.
 // Malformed input never reads or writes out of bounds: it sets a bit in the
